@@ -1,0 +1,115 @@
+"""Thin host wrappers over the C-ABI for tests and benchmarks.
+
+Device polynomials are torch int64 tensors holding uint64 residues
+(limb-major, limb r at offset r*N); the kernels see raw pointers.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, load
+
+FORWARD, INVERSE = 0, 1
+ADD, SUB, MUL, NEGATE, MULADD = 0, 1, 2, 3, 5
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class Params:
+    """aoa_params handle: one (N, k, kl, prec) parameter set resident in HBM."""
+
+    def __init__(self, N, k, kl=0, prec=48):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.aoa_params_create(N, k, kl, prec, ctypes.byref(h)))
+        self.h = h
+        n, kk, kkl, pr = (ctypes.c_int() for _ in range(4))
+        check(self.lib.aoa_params_info(h, ctypes.byref(n), ctypes.byref(kk), ctypes.byref(kkl),
+                                       ctypes.byref(pr)))
+        self.N, self.k, self.kl, self.prec = n.value, kk.value, kkl.value, pr.value
+        buf = (ctypes.c_uint64 * (self.k + self.kl))()
+        check(self.lib.aoa_params_primes(h, buf, self.k + self.kl))
+        self.primes = [int(x) for x in buf]
+
+    def close(self):
+        if self.h:
+            self.lib.aoa_params_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def table(self, which):
+        n = self.lib.aoa_params_table(self.h, which, None, 0)
+        if n < 0:
+            raise ValueError("unknown table %d" % which)
+        out = np.zeros(n, dtype=np.uint64)
+        self.lib.aoa_params_table(self.h, which, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n)
+        return out
+
+    # ---- device helpers -------------------------------------------------
+    def zeros(self, nlimbs, device="cuda"):
+        import torch
+        return torch.zeros(nlimbs * self.N, dtype=torch.int64, device=device)
+
+    def to_device(self, arr):
+        import torch
+        a = np.ascontiguousarray(arr, dtype=np.uint64).view(np.int64)
+        return torch.from_numpy(a.copy()).cuda()
+
+    @staticmethod
+    def to_host(t):
+        return t.detach().cpu().numpy().view(np.uint64).copy()
+
+    def workspace(self, level):
+        import torch
+        nbytes = self.lib.aoa_workspace_bytes(self.h, level)
+        return torch.empty(nbytes // 8, dtype=torch.int64, device="cuda"), nbytes
+
+    def ntt(self, d, res0, nres, direction, stream=None):
+        check(self.lib.aoa_ntt(self.h, _stream(stream), _ptr(d), res0, nres, direction))
+
+    def eltwise(self, op, out, a, b=None, c=None, res0=0, nres=None, stream=None):
+        nres = nres if nres is not None else self.k
+        check(self.lib.aoa_eltwise(self.h, _stream(stream), op, _ptr(out), _ptr(a), _ptr(b), _ptr(c), res0,
+                                   nres))
+
+    def rescale(self, c0, c1, level, stream=None):
+        check(self.lib.aoa_rescale(self.h, _stream(stream), _ptr(c0), _ptr(c1), level))
+
+    def modup(self, qb, level, stream=None):
+        check(self.lib.aoa_modup(self.h, _stream(stream), _ptr(qb), level))
+
+    def moddown(self, out, qb, level, stream=None):
+        check(self.lib.aoa_moddown(self.h, _stream(stream), _ptr(out), _ptr(qb), level))
+
+    def automorphism(self, out, inp, nres, rot_slots, stream=None):
+        check(self.lib.aoa_automorphism(self.h, _stream(stream), _ptr(out), _ptr(inp), nres, rot_slots))
+
+    def keyswitch(self, out0, out1, d, key_b, key_a, level, ws, add0=None, stream=None):
+        w, nb = ws
+        check(self.lib.aoa_keyswitch(self.h, _stream(stream), _ptr(out0), _ptr(out1), _ptr(add0), _ptr(d),
+                                     _ptr(key_b), _ptr(key_a), level, _ptr(w), nb))
+
+    def hommul(self, c3, c1, c2, evk, level, ws, stream=None):
+        w, nb = ws
+        check(self.lib.aoa_hommul_relin_rescale(
+            self.h, _stream(stream), _ptr(c3[0]), _ptr(c3[1]), _ptr(c1[0]), _ptr(c1[1]), _ptr(c2[0]),
+            _ptr(c2[1]), _ptr(evk[0]), _ptr(evk[1]), level, _ptr(w), nb))
+
+    def rotate_left(self, out, c, rtk, rot_slots, level, ws, stream=None):
+        w, nb = ws
+        check(self.lib.aoa_rotate_left(self.h, _stream(stream), _ptr(out[0]), _ptr(out[1]), _ptr(c[0]),
+                                       _ptr(c[1]), _ptr(rtk[0]), _ptr(rtk[1]), rot_slots, level, _ptr(w),
+                                       nb))

@@ -1,0 +1,56 @@
+// Host-side launchers of the gfx950 kernels (all stream-ordered, no sync).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+#include "params.hpp"
+
+namespace aoa {
+
+enum EltOp { ELT_ADD = 0, ELT_SUB = 1, ELT_MUL = 2, ELT_NEG = 3, ELT_MULADD = 5 };
+
+int ntt_default_n1(int N);
+// Forward / inverse NTT of every limb in `m`.  N1 = 0 picks the default split.
+// With key0/key1 set, the inverse multiplies limb li of m.src[0] by limb
+// prime_of(li) of key y before transforming into m.base[y] (y = 0, 1).
+int ntt_forward(const DevParams& P, LimbMap m, int N1, hipStream_t st);
+int ntt_inverse(const DevParams& P, LimbMap m, int N1, const uint64_t* key0, const uint64_t* key1,
+                hipStream_t st);
+
+// out = a op b (out = a*b + c for MULADD) over nlimbs limbs mapped to primes
+// as in LimbMap (qlimbs Q limbs from prime0, then special limbs).
+void launch_eltwise(const DevParams& P, int op, uint64_t* out, const uint64_t* a, const uint64_t* b,
+                    const uint64_t* c, int nlimbs, int qlimbs, int prime0, hipStream_t st);
+
+// d0 = a0*b0, d1 = a0*b1 + a1*b0, d2 = a1*b1 on L Q-limbs (NTT domain).
+void launch_tensor(const DevParams& P, uint64_t* d0, uint64_t* d1, uint64_t* d2, const uint64_t* a0,
+                   const uint64_t* a1, const uint64_t* b0, const uint64_t* b1, int L, hipStream_t st);
+
+// ModUp: x holds Q limbs 0..level (coefficient domain, stride N); writes K
+// special limbs to outB (stride N).
+void launch_modup(const DevParams& P, const uint64_t* x, uint64_t* outB, int level, hipStream_t st);
+
+// ModDown of npolys polys.  inQ[y] holds Q limbs 0..level, inB[y] the K
+// special limbs (coefficient domain).  out[y] limb i = (inQ_i - conv_i(inB)) *
+// B^-1 mod q_i (+ add[y] limb i when add[y] != null).  With rescale, the
+// result is further rescaled by q_level and only limbs 0..level-1 are written.
+struct ModDownArgs {
+  const uint64_t* inQ[2];
+  const uint64_t* inB[2];
+  const uint64_t* add[2];
+  uint64_t* out[2];
+  int npolys;
+  int level;
+  int rescale;
+};
+void launch_moddown(const DevParams& P, const ModDownArgs& a, hipStream_t st);
+
+// Rescale c0 and c1 in place (coefficient domain) from `level`.
+void launch_rescale(const DevParams& P, uint64_t* c0, uint64_t* c1, int level, hipStream_t st);
+
+// Galois automorphism X -> X^g (g odd) in the coefficient domain, gather form.
+void launch_automorphism(const DevParams& P, uint64_t* out, const uint64_t* in, int nlimbs,
+                         uint64_t galois, hipStream_t st);
+
+}  // namespace aoa

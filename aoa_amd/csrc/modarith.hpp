@@ -1,0 +1,147 @@
+// Modular arithmetic on uint64 residues, usable from host C++ and gfx950 HIP.
+//
+// Primes are < 2^63 (the first Q prime is 63-bit, the rest 45..55-bit), so
+// every canonical residue fits in 63 bits and a + b never wraps.
+//
+//  * mul_shoup      constant operand w with precomputed floor(w * 2^64 / p):
+//                   twiddles and RNS constants (one mulhi, two mullo).
+//  * mul_barrett    variable x variable, exact for a, b < p (HAC 14.42 with a
+//                   128-bit final correction so that the 63-bit prime is safe).
+//  * ref_barrett    the reference's own Barrett reduction
+//                   (aoa-ntt/src/cuda/htrans/common.cu:257-291).  It is exact
+//                   for a < p^2 but not for the larger products that ModDown
+//                   feeds it (aoa-ntt/src/cuda/manager.cu:331-334), where it
+//                   returns exact - (2^nb - p) for a small fraction of terms.
+//                   Bit-exact ModDown must therefore use this formula for that
+//                   one product.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define AOA_HD __host__ __device__ __forceinline__
+#else
+#define AOA_HD static inline
+#endif
+
+namespace aoa {
+
+AOA_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul64hi(a, b);
+#else
+  return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+AOA_HD uint64_t add_mod(uint64_t a, uint64_t b, uint64_t p) {
+  uint64_t s = a + b;
+  return s >= p ? s - p : s;
+}
+
+AOA_HD uint64_t sub_mod(uint64_t a, uint64_t b, uint64_t p) {
+  return a >= b ? a - b : a + (p - b);
+}
+
+AOA_HD uint64_t neg_mod(uint64_t a, uint64_t p) { return a ? p - a : 0; }
+
+// floor(w * 2^64 / p) for w < p.
+AOA_HD uint64_t shoup_quot(uint64_t w, uint64_t p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Not used on the device: all Shoup quotients are precomputed on the host.
+  return 0;
+#else
+  return (uint64_t)(((unsigned __int128)w << 64) / p);
+#endif
+}
+
+// x * w mod p, x any 64-bit value, w < p < 2^63, wq = shoup_quot(w, p).
+AOA_HD uint64_t mul_shoup(uint64_t x, uint64_t w, uint64_t wq, uint64_t p) {
+  uint64_t q = mulhi64(x, wq);
+  uint64_t r = x * w - q * p;  // in [0, 2p)
+  return r >= p ? r - p : r;
+}
+
+// Same, result left in [0, 2p).
+AOA_HD uint64_t mul_shoup_lazy(uint64_t x, uint64_t w, uint64_t wq, uint64_t p) {
+  uint64_t q = mulhi64(x, wq);
+  return x * w - q * p;
+}
+
+// Barrett constants for p with nb = bitlen(p) (2 <= nb <= 63).
+struct barrett_t {
+  uint64_t p;
+  uint64_t mu;  // floor(2^(2nb) / p), < 2^(nb+1)
+  int nb;
+};
+
+// Exact (hi:lo) mod p for a 128-bit value below p * 2^nb (covers a*b, a,b < p).
+AOA_HD uint64_t barrett_reduce(uint64_t hi, uint64_t lo, uint64_t p, uint64_t mu, int nb) {
+  // q1 = floor(z / 2^(nb-1)) < 2^(nb+1) <= 2^64
+  const int s = nb - 1;
+  uint64_t q1 = (lo >> s) | (hi << (64 - s));
+  // q3 = floor(q1 * mu / 2^(nb+1)); q1*mu < 2^(2nb+2)
+  uint64_t ph = mulhi64(q1, mu), pl = q1 * mu;
+  const int t = nb + 1;
+  uint64_t q3 = (t == 64) ? ph : ((pl >> t) | (ph << (64 - t)));
+  // r = z - q3*p in [0, 3p): compute in 128 bits, then correct.
+  uint64_t qh = mulhi64(q3, p), ql = q3 * p;
+  uint64_t rl = lo - ql;
+  uint64_t rh = hi - qh - (lo < ql ? 1u : 0u);
+  // at most two corrections
+  if (rh != 0 || rl >= p) {
+    uint64_t nl = rl - p;
+    rh -= (rl < p ? 1u : 0u);
+    rl = nl;
+  }
+  if (rh != 0 || rl >= p) rl -= p;
+  return rl;
+}
+
+AOA_HD uint64_t mul_barrett(uint64_t a, uint64_t b, uint64_t p, uint64_t mu, int nb) {
+  return barrett_reduce(mulhi64(a, b), a * b, p, mu, nb);
+}
+
+// The reference's Barrett reduction of a 128-bit value (hi:lo) with
+// MU = floor(4^nb / p) (aoa-ntt/src/cuda/htrans/common.cu:257-291, constants at
+// :485-506).  Written from its arithmetic: q = floor(floor(a / 2^(2nb-64)) *
+// MU / 2^64) (truncated to 64 bits), A = a - q*p, and the result is
+// (A mod 2^nb - (A >> nb) * p) mod 2^nb followed by one conditional
+// subtraction of p.  Valid for 33 <= nb <= 63.
+AOA_HD uint64_t ref_barrett(uint64_t hi, uint64_t lo, uint64_t p, uint64_t mu, int nb) {
+  const uint64_t mask = (((uint64_t)1) << nb) - 1;
+  const int s = 2 * nb - 64;  // 2..62 for nb in 33..63
+  uint64_t xlo = (lo >> s) | (hi << (64 - s));
+  uint64_t xhi = hi >> s;
+  uint64_t q = mulhi64(xlo, mu) + xhi * mu;
+  uint64_t qh = mulhi64(q, p), ql = q * p;
+  // A = a - q*p as a 128-bit difference (mod 2^128), split at bit nb.
+  uint64_t al = lo - ql;
+  uint64_t ah = hi - qh - (lo < ql ? 1u : 0u);
+  uint64_t a_lo = al & mask;
+  uint64_t a_hi = (al >> nb) | (ah << (64 - nb));
+  uint64_t r = (a_lo - a_hi * p) & mask;
+  return r >= p ? r - p : r;
+}
+
+// Bit length of a nonzero 64-bit value.
+AOA_HD int bitlen64(uint64_t x) {
+  int n = 0;
+  while (x) {
+    ++n;
+    x >>= 1;
+  }
+  return n;
+}
+
+AOA_HD uint64_t pow_mod(uint64_t a, uint64_t e, uint64_t p, uint64_t mu, int nb) {
+  uint64_t r = 1 % p, s = a % p;
+  while (e) {
+    if (e & 1) r = mul_barrett(r, s, p, mu, nb);
+    s = mul_barrett(s, s, p, mu, nb);
+    e >>= 1;
+  }
+  return r;
+}
+
+}  // namespace aoa

@@ -1,0 +1,389 @@
+// Negacyclic NTT over RNS limbs for gfx950.
+//
+// Output order and values equal the reference's hierarchical NTT
+// (aoa-ntt/src/cuda/htrans/ntt.cu:321-460): out[pos] = sum_n x[n] *
+// psi^(n * (2*brev(pos) + 1)) mod p, i.e. the standard Cooley-Tukey
+// negacyclic NTT with merged twist and bit-reversed output; the inverse is the
+// exact inverse (Gentleman-Sande, N^-1 folded into the last stage).
+//
+// The N-point transform is split as N = N1 x N2.  The column pass runs the
+// first log2(N1) stages on S = N1 elements with stride N2; the row pass runs
+// the remaining log2(N2) stages on contiguous rows of S = N2.  Both passes are
+// in place, so no ping-pong buffer is needed.  A 256-thread workgroup
+// transforms a 4096-element tile: TS = S/16 threads per sub-transform, 16
+// values per thread in registers, four stages per register round and one LDS
+// exchange between rounds.  Twiddles psi^brev(i) come from a per-prime table
+// with Shoup quotients.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace aoa {
+
+template <int S>
+struct Geo {
+  static constexpr int LOGS = (S == 64) ? 6 : (S == 128 ? 7 : 8);
+  static constexpr int TS = S / 16;   // threads per sub-transform
+  static constexpr int C = 256 / TS;  // sub-transforms per tile
+};
+
+__device__ __forceinline__ void ct_bfly(uint64_t& a, uint64_t& b, const tw2 t, uint64_t p) {
+  uint64_t v = mul_shoup(b, t.w, t.q, p);
+  uint64_t u = a;
+  uint64_t s = u + v;
+  a = s >= p ? s - p : s;
+  b = u >= v ? u - v : u + (p - v);
+}
+
+__device__ __forceinline__ void gs_bfly(uint64_t& a, uint64_t& b, const tw2 t, uint64_t p) {
+  uint64_t u = a, v = b;
+  uint64_t s = u + v;
+  a = s >= p ? s - p : s;
+  b = mul_shoup(u + (p - v), t.w, t.q, p);
+}
+
+// ---------------------------------------------------------------- forward --
+
+// Column pass: global stages 0..log2(S)-1 on columns of an S x N2 matrix.
+template <int S>
+__global__ __launch_bounds__(256) void k_ntt_fwd_col(DevParams P, LimbMap m, int N2) {
+  using G = Geo<S>;
+  constexpr int C = G::C, TS = G::TS, LOGS = G::LOGS, LDW = C + 1;
+  __shared__ uint64_t lds[S * LDW];
+  const int tiles = N2 / C;
+  const int li = blockIdx.x / tiles;
+  const int col0 = (blockIdx.x % tiles) * C;
+  const int y = blockIdx.y;
+  const int pr = limb_prime(m, P.k, li);
+  const uint64_t p = P.primes[pr].p;
+  const size_t N = (size_t)P.N;
+  const uint64_t* src = m.src[y] + (size_t)li * N + col0;
+  uint64_t* dst = m.base[y] + (size_t)li * N + col0;
+  const uint64_t* T = P.tw_fwd + (size_t)pr * N * 2;
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int sl = tid / C;
+  uint64_t x[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = src[(size_t)(sl + TS * k) * N2 + c];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int half = 8 >> r;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & half) continue;
+      const tw2 t = load_tw(T, (1 << r) + (k >> (4 - r)));
+      ct_bfly(x[k], x[k + half], t, p);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) lds[(sl + TS * k) * LDW + c] = x[k];
+  __syncthreads();
+  const int sh = sl;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = lds[(16 * sh + k) * LDW + c];
+#pragma unroll
+  for (int r = 4; r < LOGS; ++r) {
+    const int half = 1 << (LOGS - 1 - r);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & half) continue;
+      const tw2 t = load_tw(T, (1 << r) + (sh << (r - (LOGS - 4))) + (k >> (LOGS - r)));
+      ct_bfly(x[k], x[k + half], t, p);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) dst[(size_t)(16 * sh + k) * N2 + c] = x[k];
+}
+
+__device__ __forceinline__ int swz(int s) { return s + (s >> 4); }
+
+// Row pass: global stages g0..g0+log2(S)-1 on contiguous rows of length S.
+template <int S>
+__global__ __launch_bounds__(256) void k_ntt_fwd_row(DevParams P, LimbMap m, int g0) {
+  using G = Geo<S>;
+  constexpr int C = G::C, TS = G::TS, LOGS = G::LOGS, PS = S + S / 16;
+  __shared__ uint64_t lds[C * PS];
+  const size_t N = (size_t)P.N;
+  const int N1 = P.N / S;
+  const int tiles = N1 / C;
+  const int li = blockIdx.x / tiles;
+  const int row0 = (blockIdx.x % tiles) * C;
+  const int y = blockIdx.y;
+  const int pr = limb_prime(m, P.k, li);
+  const uint64_t p = P.primes[pr].p;
+  const uint64_t* T = P.tw_fwd + (size_t)pr * N * 2;
+  const int tid = threadIdx.x;
+  const int rl = tid / TS;
+  const int sl = tid % TS;
+  const int row = row0 + rl;
+  const uint64_t* src = m.src[y] + (size_t)li * N + (size_t)row * S;
+  uint64_t* dst = m.base[y] + (size_t)li * N + (size_t)row * S;
+  uint64_t* L = lds + rl * PS;
+  uint64_t x[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = src[sl + TS * k];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int half = 8 >> r;
+    const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & half) continue;
+      const tw2 t = load_tw(T, base + (k >> (4 - r)));
+      ct_bfly(x[k], x[k + half], t, p);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) L[swz(sl + TS * k)] = x[k];
+  __syncthreads();
+  const int sh = sl;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = L[swz(16 * sh + k)];
+#pragma unroll
+  for (int r = 4; r < LOGS; ++r) {
+    const int half = 1 << (LOGS - 1 - r);
+    const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r) + ((size_t)sh << (r - (LOGS - 4)));
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & half) continue;
+      const tw2 t = load_tw(T, base + (k >> (LOGS - r)));
+      ct_bfly(x[k], x[k + half], t, p);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) L[swz(16 * sh + k)] = x[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) dst[sl + TS * k] = L[swz(sl + TS * k)];
+}
+
+// ---------------------------------------------------------------- inverse --
+
+// Row pass of the inverse: global stages g0+log2(S)-1 .. g0.  With KEYMUL the
+// input limb is first multiplied by limb `prime` of key poly blockIdx.y
+// (key switching: INTT(x * key_b), INTT(x * key_a) from one input).
+template <int S, bool KEYMUL>
+__global__ __launch_bounds__(256) void k_ntt_inv_row(DevParams P, LimbMap m, int g0,
+                                                      const uint64_t* key0, const uint64_t* key1) {
+  using G = Geo<S>;
+  constexpr int C = G::C, TS = G::TS, LOGS = G::LOGS, PS = S + S / 16;
+  __shared__ uint64_t lds[C * PS];
+  const size_t N = (size_t)P.N;
+  const int N1 = P.N / S;
+  const int tiles = N1 / C;
+  const int li = blockIdx.x / tiles;
+  const int row0 = (blockIdx.x % tiles) * C;
+  const int y = blockIdx.y;
+  const int pr = limb_prime(m, P.k, li);
+  const DevPrime dp = P.primes[pr];
+  const uint64_t p = dp.p;
+  const uint64_t* T = P.tw_inv + (size_t)pr * N * 2;
+  const int tid = threadIdx.x;
+  const int rl = tid / TS;
+  const int sl = tid % TS;
+  const int row = row0 + rl;
+  const uint64_t* src = m.src[KEYMUL ? 0 : y] + (size_t)li * N + (size_t)row * S;
+  uint64_t* dst = m.base[y] + (size_t)li * N + (size_t)row * S;
+  uint64_t* L = lds + rl * PS;
+  uint64_t x[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = src[sl + TS * k];
+  if (KEYMUL) {
+    const uint64_t* kp = (y == 0 ? key0 : key1) + (size_t)pr * N + (size_t)row * S;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = mul_barrett(x[k], kp[sl + TS * k], p, dp.mu, (int)dp.nb);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) L[swz(sl + TS * k)] = x[k];
+  __syncthreads();
+  const int sh = sl;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = L[swz(16 * sh + k)];
+#pragma unroll
+  for (int r = LOGS - 1; r >= 4; --r) {
+    const int half = 1 << (LOGS - 1 - r);
+    const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r) + ((size_t)sh << (r - (LOGS - 4)));
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & half) continue;
+      const tw2 t = load_tw(T, base + (k >> (LOGS - r)));
+      gs_bfly(x[k], x[k + half], t, p);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) L[swz(16 * sh + k)] = x[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = L[swz(sl + TS * k)];
+#pragma unroll
+  for (int r = 3; r >= 0; --r) {
+    const int half = 8 >> r;
+    const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & half) continue;
+      const tw2 t = load_tw(T, base + (k >> (4 - r)));
+      gs_bfly(x[k], x[k + half], t, p);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) dst[sl + TS * k] = x[k];
+}
+
+// Column pass of the inverse: global stages log2(S)-1 .. 0, N^-1 folded into
+// stage 0.
+template <int S>
+__global__ __launch_bounds__(256) void k_ntt_inv_col(DevParams P, LimbMap m, int N2) {
+  using G = Geo<S>;
+  constexpr int C = G::C, TS = G::TS, LOGS = G::LOGS, LDW = C + 1;
+  __shared__ uint64_t lds[S * LDW];
+  const int tiles = N2 / C;
+  const int li = blockIdx.x / tiles;
+  const int col0 = (blockIdx.x % tiles) * C;
+  const int y = blockIdx.y;
+  const int pr = limb_prime(m, P.k, li);
+  const DevPrime dp = P.primes[pr];
+  const uint64_t p = dp.p;
+  const size_t N = (size_t)P.N;
+  const uint64_t* src = m.src[y] + (size_t)li * N + col0;
+  uint64_t* dst = m.base[y] + (size_t)li * N + col0;
+  const uint64_t* T = P.tw_inv + (size_t)pr * N * 2;
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int sh = tid / C;
+  uint64_t x[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = src[(size_t)(16 * sh + k) * N2 + c];
+#pragma unroll
+  for (int r = LOGS - 1; r >= 4; --r) {
+    const int half = 1 << (LOGS - 1 - r);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & half) continue;
+      const tw2 t = load_tw(T, (1 << r) + (sh << (r - (LOGS - 4))) + (k >> (LOGS - r)));
+      gs_bfly(x[k], x[k + half], t, p);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) lds[(16 * sh + k) * LDW + c] = x[k];
+  __syncthreads();
+  const int sl = sh;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = lds[(sl + TS * k) * LDW + c];
+#pragma unroll
+  for (int r = 3; r >= 1; --r) {
+    const int half = 8 >> r;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & half) continue;
+      const tw2 t = load_tw(T, (1 << r) + (k >> (4 - r)));
+      gs_bfly(x[k], x[k + half], t, p);
+    }
+  }
+  // stage 0 with N^-1: a' = (u+v) N^-1, b' = (u-v) psi^-brev(1) N^-1
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    uint64_t u = x[k], v = x[k + 8];
+    x[k] = mul_shoup(u + v, dp.ninv, dp.ninv_q, p);
+    x[k + 8] = mul_shoup(u + (p - v), dp.w1ninv, dp.w1ninv_q, p);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) dst[(size_t)(sl + TS * k) * N2 + c] = x[k];
+}
+
+// ---------------------------------------------------------------- launch ---
+
+static bool valid_split(int N, int N1) {
+  if (N1 != 64 && N1 != 128 && N1 != 256) return false;
+  int N2 = N / N1;
+  return N2 == 64 || N2 == 128 || N2 == 256;
+}
+
+int ntt_default_n1(int N) {
+  switch (N) {
+    case 4096: return 64;
+    case 8192: return 128;
+    case 16384: return 128;
+    case 32768: return 128;
+    case 65536: return 256;
+    default: return 0;
+  }
+}
+
+template <int S>
+static void launch_fwd_col(const DevParams& P, const LimbMap& m, int N2, hipStream_t st) {
+  const int tiles = N2 / Geo<S>::C;
+  dim3 grid(tiles * m.nlimbs, m.npolys);
+  k_ntt_fwd_col<S><<<grid, 256, 0, st>>>(P, m, N2);
+}
+template <int S>
+static void launch_fwd_row(const DevParams& P, const LimbMap& m, int g0, hipStream_t st) {
+  const int tiles = (P.N / S) / Geo<S>::C;
+  dim3 grid(tiles * m.nlimbs, m.npolys);
+  k_ntt_fwd_row<S><<<grid, 256, 0, st>>>(P, m, g0);
+}
+template <int S>
+static void launch_inv_row(const DevParams& P, const LimbMap& m, int g0, const uint64_t* k0,
+                           const uint64_t* k1, hipStream_t st) {
+  const int tiles = (P.N / S) / Geo<S>::C;
+  dim3 grid(tiles * m.nlimbs, m.npolys);
+  if (k0)
+    k_ntt_inv_row<S, true><<<grid, 256, 0, st>>>(P, m, g0, k0, k1);
+  else
+    k_ntt_inv_row<S, false><<<grid, 256, 0, st>>>(P, m, g0, nullptr, nullptr);
+}
+template <int S>
+static void launch_inv_col(const DevParams& P, const LimbMap& m, int N2, hipStream_t st) {
+  const int tiles = N2 / Geo<S>::C;
+  dim3 grid(tiles * m.nlimbs, m.npolys);
+  k_ntt_inv_col<S><<<grid, 256, 0, st>>>(P, m, N2);
+}
+
+#define AOA_DISPATCH_S(S_, CALL) \
+  switch (S_) {                  \
+    case 64: CALL(64); break;    \
+    case 128: CALL(128); break;  \
+    case 256: CALL(256); break;  \
+    default: return -1;          \
+  }
+
+int ntt_forward(const DevParams& P, LimbMap m, int N1, hipStream_t st) {
+  if (N1 == 0) N1 = ntt_default_n1(P.N);
+  if (!valid_split(P.N, N1)) return -1;
+  const int N2 = P.N / N1;
+  int g0 = 0;
+  while ((1 << g0) < N1) ++g0;
+  for (int y = 0; y < m.npolys; ++y)
+    if (!m.src[y]) m.src[y] = m.base[y];
+#define CALLC(S) launch_fwd_col<S>(P, m, N2, st)
+  AOA_DISPATCH_S(N1, CALLC)
+#undef CALLC
+  LimbMap m2 = m;
+  for (int y = 0; y < m2.npolys; ++y) m2.src[y] = m2.base[y];
+#define CALLR(S) launch_fwd_row<S>(P, m2, g0, st)
+  AOA_DISPATCH_S(N2, CALLR)
+#undef CALLR
+  return 0;
+}
+
+int ntt_inverse(const DevParams& P, LimbMap m, int N1, const uint64_t* key0, const uint64_t* key1,
+                hipStream_t st) {
+  if (N1 == 0) N1 = ntt_default_n1(P.N);
+  if (!valid_split(P.N, N1)) return -1;
+  const int N2 = P.N / N1;
+  int g0 = 0;
+  while ((1 << g0) < N1) ++g0;
+  for (int y = 0; y < m.npolys; ++y)
+    if (!m.src[y]) m.src[y] = m.base[y];
+#define CALLR(S) launch_inv_row<S>(P, m, g0, key0, key1, st)
+  AOA_DISPATCH_S(N2, CALLR)
+#undef CALLR
+  LimbMap m2 = m;
+  for (int y = 0; y < m2.npolys; ++y) m2.src[y] = m2.base[y];
+#define CALLC(S) launch_inv_col<S>(P, m2, N2, st)
+  AOA_DISPATCH_S(N1, CALLC)
+#undef CALLC
+  return 0;
+}
+
+}  // namespace aoa

@@ -1,0 +1,201 @@
+// Host-side construction of a parameter set (primes, twiddles, RNS constants).
+#include "params.hpp"
+
+#include <cstring>
+
+namespace aoa {
+
+#include "prime_table.inc"
+
+uint32_t bit_reverse(uint32_t x, int bits) {
+  uint32_t r = 0;
+  for (int i = 0; i < bits; ++i) {
+    r = (r << 1) | (x & 1u);
+    x >>= 1;
+  }
+  return r;
+}
+
+uint64_t inv_mod(uint64_t a, uint64_t p) {
+  // extended Euclid on signed 128-bit integers
+  __int128 t = 0, nt = 1, r = p, nr = a % p;
+  while (nr != 0) {
+    __int128 q = r / nr, tmp;
+    tmp = t - q * nt; t = nt; nt = tmp;
+    tmp = r - q * nr; r = nr; nr = tmp;
+  }
+  if (t < 0) t += p;
+  return (uint64_t)t;
+}
+
+static uint64_t mulm(uint64_t a, uint64_t b, uint64_t p) {
+  return (uint64_t)(((unsigned __int128)a * b) % p);
+}
+
+static uint64_t powm(uint64_t a, uint64_t e, uint64_t p) {
+  uint64_t r = 1 % p, s = a % p;
+  while (e) {
+    if (e & 1) r = mulm(r, s, p);
+    s = mulm(s, s, p);
+    e >>= 1;
+  }
+  return r;
+}
+
+// Prime layout of the reference (aoa-ntt/src/cuda/manager.cu:426-464): the
+// 63-bit prime first, then k-1 primes of `prec` bits; the special base B is
+// 55-bit for prec 48/52 and the continuation of the same bucket for 45/55.
+static std::string select_primes(int k, int kl, int prec, std::vector<aoa_prime_entry>* q,
+                                 std::vector<aoa_prime_entry>* b) {
+  const aoa_prime_entry* qb = nullptr;
+  int qn = 0;
+  const aoa_prime_entry* bb = nullptr;
+  int bn = 0, boff = 0;
+  switch (prec) {
+    case 45: qb = AOA_BUCKET_45; qn = 100; bb = AOA_BUCKET_45; bn = 100; boff = k - 1; break;
+    case 48: qb = AOA_BUCKET_48; qn = 100; bb = AOA_BUCKET_55; bn = 500; boff = 0; break;
+    case 52: qb = AOA_BUCKET_52; qn = 30; bb = AOA_BUCKET_55; bn = 500; boff = 0; break;
+    case 55: qb = AOA_BUCKET_55; qn = 500; bb = AOA_BUCKET_55; bn = 500; boff = k - 1; break;
+    default: return "We don't support this precision.";
+  }
+  if (k < 1 || kl < 1) return "k and kl must be positive";
+  // ModUp / ModDown keep one 2 KiB LDS column per limb per workgroup.
+  if (k > 64 || kl > 64) return "at most 64 Q limbs and 64 special limbs are supported";
+  if (k - 1 > qn) return "not enough primes of this precision for k";
+  if (boff + kl > bn) return "not enough special primes for kl";
+  q->clear();
+  b->clear();
+  q->push_back(AOA_BUCKET_63[0]);
+  for (int i = 0; i < k - 1; ++i) q->push_back(qb[i]);
+  for (int j = 0; j < kl; ++j) b->push_back(bb[boff + j]);
+  return "";
+}
+
+std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
+  if (N < 4096 || N > 65536 || (N & (N - 1)) != 0) return "NTT is not working for this N";
+  if (kl <= 0) kl = k + 1;
+  std::vector<aoa_prime_entry> qe, be;
+  std::string err = select_primes(k, kl, prec, &qe, &be);
+  if (!err.empty()) return err;
+
+  HostParams& P = *out;
+  P = HostParams();
+  P.N = N;
+  P.logN = 0;
+  while ((1 << P.logN) < N) ++P.logN;
+  P.k = k;
+  P.K = kl;
+  P.prec = prec;
+  for (auto& e : qe) { P.primes.push_back(e.p); P.roots.push_back(e.g); }
+  for (auto& e : be) { P.primes.push_back(e.p); P.roots.push_back(e.g); }
+  const int R = k + kl;
+
+  for (int r = 0; r < R; ++r) {
+    uint64_t p = P.primes[r];
+    if ((p - 1) % (2 * (uint64_t)N) != 0) return "prime does not support this ring dimension";
+    DevPrime d;
+    std::memset(&d, 0, sizeof(d));
+    d.p = p;
+    d.nb = (uint32_t)bitlen64(p);
+    d.mu = (uint64_t)((((unsigned __int128)1) << (2 * d.nb)) / p);
+    d.ninv = inv_mod((uint64_t)N % p, p);
+    d.ninv_q = shoup_quot(d.ninv, p);
+    d.eps = (((uint64_t)1) << d.nb) - p;
+    d.r64 = (uint64_t)((((unsigned __int128)1) << 64) % p);
+    d.r64_q = shoup_quot(d.r64, p);
+    P.dprimes.push_back(d);
+  }
+
+  // Twiddles: psi = g^((p-1)/2N); forward table psi^brev(i), inverse psi^-brev(i).
+  P.tw_fwd.assign((size_t)R * N * 2, 0);
+  P.tw_inv.assign((size_t)R * N * 2, 0);
+  std::vector<uint64_t> pw(N), ipw(N);
+  for (int r = 0; r < R; ++r) {
+    uint64_t p = P.primes[r];
+    uint64_t psi = powm(P.roots[r], (p - 1) / (2 * (uint64_t)N), p);
+    uint64_t ipsi = inv_mod(psi, p);
+    pw[0] = 1;
+    ipw[0] = 1;
+    for (int i = 1; i < N; ++i) {
+      pw[i] = mulm(pw[i - 1], psi, p);
+      ipw[i] = mulm(ipw[i - 1], ipsi, p);
+    }
+    uint64_t* tf = &P.tw_fwd[(size_t)r * N * 2];
+    uint64_t* ti = &P.tw_inv[(size_t)r * N * 2];
+    for (int i = 0; i < N; ++i) {
+      uint32_t e = bit_reverse((uint32_t)i, P.logN);
+      tf[2 * i] = pw[e];
+      tf[2 * i + 1] = shoup_quot(pw[e], p);
+      ti[2 * i] = ipw[e];
+      ti[2 * i + 1] = shoup_quot(ipw[e], p);
+    }
+    DevPrime& d = P.dprimes[r];
+    d.w1ninv = mulm(ti[2], d.ninv, p);
+    d.w1ninv_q = shoup_quot(d.w1ninv, p);
+  }
+
+  // RNS constants (aoa-ntt/src/cuda/manager.cu:566-759), as residues of
+  // products of primes; no big integers are needed.
+  const std::vector<uint64_t> Qp(P.primes.begin(), P.primes.begin() + k);
+  const std::vector<uint64_t> Bp(P.primes.begin() + k, P.primes.end());
+  P.inv_qhat.assign((size_t)k * k * 2, 0);
+  P.qhat_mod_b.assign((size_t)k * kl * k, 0);
+  P.inv_ql_mod_qi.assign((size_t)k * k * 2, 0);
+  for (int l = 0; l < k; ++l) {
+    for (int i = 0; i <= l; ++i) {
+      // (Q_l / q_i) mod q_i, Q_l = q_0 ... q_l
+      uint64_t qi = Qp[i];
+      uint64_t prod = 1 % qi;
+      for (int m = 0; m <= l; ++m)
+        if (m != i) prod = mulm(prod, Qp[m] % qi, qi);
+      uint64_t inv = inv_mod(prod, qi);
+      P.inv_qhat[((size_t)l * k + i) * 2] = inv;
+      P.inv_qhat[((size_t)l * k + i) * 2 + 1] = shoup_quot(inv, qi);
+      for (int j = 0; j < kl; ++j) {
+        uint64_t bj = Bp[j];
+        uint64_t pr = 1 % bj;
+        for (int m = 0; m <= l; ++m)
+          if (m != i) pr = mulm(pr, Qp[m] % bj, bj);
+        P.qhat_mod_b[((size_t)l * kl + j) * k + i] = pr;
+      }
+    }
+    for (int i = 0; i < k; ++i) {
+      if (i == l) continue;
+      uint64_t v = inv_mod(Qp[l] % Qp[i], Qp[i]);
+      P.inv_ql_mod_qi[((size_t)l * k + i) * 2] = v;
+      P.inv_ql_mod_qi[((size_t)l * k + i) * 2 + 1] = shoup_quot(v, Qp[i]);
+    }
+  }
+  P.inv_bhat.assign((size_t)kl * 2, 0);
+  for (int j = 0; j < kl; ++j) {
+    uint64_t bj = Bp[j];
+    uint64_t pr = 1 % bj;
+    for (int m = 0; m < kl; ++m)
+      if (m != j) pr = mulm(pr, Bp[m] % bj, bj);
+    uint64_t inv = inv_mod(pr, bj);
+    P.inv_bhat[2 * j] = inv;
+    P.inv_bhat[2 * j + 1] = shoup_quot(inv, bj);
+  }
+  P.bhat_mod_q.assign((size_t)k * kl, 0);
+  P.inv_b_mod_q.assign((size_t)k * 2, 0);
+  P.b_mod_q.assign((size_t)k * 2, 0);
+  for (int i = 0; i < k; ++i) {
+    uint64_t qi = Qp[i];
+    uint64_t bfull = 1 % qi;
+    for (int m = 0; m < kl; ++m) bfull = mulm(bfull, Bp[m] % qi, qi);
+    for (int j = 0; j < kl; ++j) {
+      uint64_t pr = 1 % qi;
+      for (int m = 0; m < kl; ++m)
+        if (m != j) pr = mulm(pr, Bp[m] % qi, qi);
+      P.bhat_mod_q[(size_t)i * kl + j] = pr;
+    }
+    uint64_t inv = inv_mod(bfull, qi);
+    P.inv_b_mod_q[2 * i] = inv;
+    P.inv_b_mod_q[2 * i + 1] = shoup_quot(inv, qi);
+    P.b_mod_q[2 * i] = bfull;
+    P.b_mod_q[2 * i + 1] = shoup_quot(bfull, qi);
+  }
+  return "";
+}
+
+}  // namespace aoa

@@ -1,0 +1,70 @@
+// Parameter set of the CKKS evaluation core: RNS primes, NTT twiddles and the
+// basis-conversion constants, computed on the host and mirrored in HBM.
+//
+// Mirrors CUDAManager::init(k, kl, N, prec) (aoa-ntt/src/cuda/manager.cu:762-800)
+// but as an immutable object instead of a process-wide singleton, so several
+// ring dimensions can live in one process.
+#pragma once
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "modarith.hpp"
+
+namespace aoa {
+
+struct aoa_prime_entry {
+  uint64_t p;
+  uint32_t g;  // smallest primitive root of p
+};
+
+// Per-prime constants used by every kernel (48 bytes, read through the scalar cache).
+struct DevPrime {
+  uint64_t p;
+  uint64_t mu;       // floor(4^nb / p): Barrett constant (same value as the reference's MU)
+  uint64_t ninv;     // N^-1 mod p
+  uint64_t ninv_q;   // its Shoup quotient
+  uint64_t eps;      // 2^nb - p (mod 2^64)
+  uint64_t w1ninv;   // psi^-brev(1) * N^-1 mod p: last inverse stage with N^-1 folded in
+  uint64_t w1ninv_q; // its Shoup quotient
+  uint64_t r64;      // 2^64 mod p (reduction of 128-bit sums)
+  uint64_t r64_q;    // its Shoup quotient
+  uint32_t nb;       // bit length of p
+  uint32_t pad[3];
+};
+
+// Pointers into one device allocation holding every table of a parameter set.
+// Passed to kernels by value.
+struct DevParams {
+  int N, logN, k, K;
+  const DevPrime* primes;        // [k+K]: Q primes then B primes
+  const uint64_t* tw_fwd;        // [k+K][N][2]: psi^brev(i) and Shoup quotient
+  const uint64_t* tw_inv;        // [k+K][N][2]: psi^-brev(i) and Shoup quotient
+  const uint64_t* inv_qhat;      // [k][k][2]: (l,i) -> (Q_l/q_i)^-1 mod q_i, Shoup
+  const uint64_t* qhat_mod_b;    // [k][K][k]: (l,j,i) -> (Q_l/q_i) mod b_j
+  const uint64_t* inv_bhat;      // [K][2]: (B/b_j)^-1 mod b_j, Shoup
+  const uint64_t* bhat_mod_q;    // [k][K]: (i,j) -> (B/b_j) mod q_i
+  const uint64_t* inv_b_mod_q;   // [k][2]: B^-1 mod q_i, Shoup
+  const uint64_t* b_mod_q;       // [k][2]: B mod q_i, Shoup
+  const uint64_t* inv_ql_mod_qi; // [k][k][2]: (l,i) -> q_l^-1 mod q_i, Shoup
+};
+
+struct HostParams {
+  int N = 0, logN = 0, k = 0, K = 0, prec = 0;
+  std::vector<uint64_t> primes;  // k + K
+  std::vector<uint32_t> roots;
+  std::vector<DevPrime> dprimes;
+  std::vector<uint64_t> tw_fwd, tw_inv, inv_qhat, qhat_mod_b, inv_bhat, bhat_mod_q,
+      inv_b_mod_q, b_mod_q, inv_ql_mod_qi;
+};
+
+// Builds every table.  Returns an empty string on success, else the reason
+// (unsupported precision, bucket too small, N not supported by a prime).
+std::string build_host_params(int N, int k, int kl, int prec, HostParams* out);
+
+// Modular inverse of a mod p (p prime, a != 0 mod p).
+uint64_t inv_mod(uint64_t a, uint64_t p);
+
+uint32_t bit_reverse(uint32_t x, int bits);
+
+}  // namespace aoa

@@ -1,0 +1,221 @@
+// RNS kernels for gfx950: element-wise ops, the HomMul tensor product, fast
+// basis conversion (ModUp / ModDown), rescale and the Galois automorphism.
+//
+// Semantics follow the reference kernels (aoa-ntt/src/cuda/manager.cu:90-367,
+// aoa-ntt/src/cuda/htrans/ntt.cu:479-667, aoa-ntt/src/ckks/encoder.cu:300-322);
+// every product is exact except the one ModDown cross-base product, which uses
+// the reference's Barrett formula so that residues stay bit-identical.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace aoa {
+
+__device__ __forceinline__ uint64_t reduce128(uint64_t hi, uint64_t lo, const DevPrime& d) {
+  uint64_t a = mul_shoup(hi, d.r64, d.r64_q, d.p);
+  uint64_t b = barrett_reduce(0, lo, d.p, d.mu, (int)d.nb);
+  return add_mod(a, b, d.p);
+}
+
+// ------------------------------------------------------------ element-wise --
+
+// Two elements per thread; a 256-thread block covers 512 elements of one limb.
+__global__ __launch_bounds__(256) void k_eltwise(DevParams P, int op, uint64_t* out, const uint64_t* a,
+                                                 const uint64_t* b, const uint64_t* c, int qlimbs, int prime0) {
+  const size_t e0 = (size_t)blockIdx.x * 512;
+  const int li = (int)(e0 >> P.logN);
+  const int pr = li < qlimbs ? prime0 + li : P.k + (li - qlimbs);
+  const DevPrime d = P.primes[pr];
+  const size_t e = e0 + (size_t)threadIdx.x * 2;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const size_t i = e + t;
+    uint64_t r;
+    switch (op) {
+      case ELT_ADD: r = add_mod(a[i], b[i], d.p); break;
+      case ELT_SUB: r = sub_mod(a[i], b[i], d.p); break;
+      case ELT_MUL: r = mul_barrett(a[i], b[i], d.p, d.mu, (int)d.nb); break;
+      case ELT_NEG: r = neg_mod(a[i], d.p); break;
+      case ELT_MULADD: r = add_mod(mul_barrett(a[i], b[i], d.p, d.mu, (int)d.nb), c[i], d.p); break;
+      default: r = 0; break;
+    }
+    out[i] = r;
+  }
+}
+
+void launch_eltwise(const DevParams& P, int op, uint64_t* out, const uint64_t* a, const uint64_t* b,
+                    const uint64_t* c, int nlimbs, int qlimbs, int prime0, hipStream_t st) {
+  const size_t total = (size_t)nlimbs * P.N;
+  k_eltwise<<<dim3((unsigned)(total / 512)), 256, 0, st>>>(P, op, out, a, b, c, qlimbs, prime0);
+}
+
+// d0 = a0 b0, d2 = a1 b1, d1 = (a0 + a1)(b0 + b1) - d0 - d2 (= a0 b1 + a1 b0).
+__global__ __launch_bounds__(256) void k_tensor(DevParams P, uint64_t* d0, uint64_t* d1, uint64_t* d2,
+                                                const uint64_t* a0, const uint64_t* a1,
+                                                const uint64_t* b0, const uint64_t* b1) {
+  const size_t e0 = (size_t)blockIdx.x * 512;
+  const int li = (int)(e0 >> P.logN);
+  const DevPrime d = P.primes[li];
+  const size_t e = e0 + (size_t)threadIdx.x * 2;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const size_t i = e + t;
+    const uint64_t x0 = a0[i], x1 = a1[i], y0 = b0[i], y1 = b1[i];
+    const uint64_t p0 = mul_barrett(x0, y0, d.p, d.mu, (int)d.nb);
+    const uint64_t p2 = mul_barrett(x1, y1, d.p, d.mu, (int)d.nb);
+    const uint64_t s = mul_barrett(add_mod(x0, x1, d.p), add_mod(y0, y1, d.p), d.p, d.mu, (int)d.nb);
+    d0[i] = p0;
+    d2[i] = p2;
+    d1[i] = sub_mod(sub_mod(s, p0, d.p), p2, d.p);
+  }
+}
+
+void launch_tensor(const DevParams& P, uint64_t* d0, uint64_t* d1, uint64_t* d2, const uint64_t* a0,
+                   const uint64_t* a1, const uint64_t* b0, const uint64_t* b1, int L, hipStream_t st) {
+  const size_t total = (size_t)L * P.N;
+  k_tensor<<<dim3((unsigned)(total / 512)), 256, 0, st>>>(P, d0, d1, d2, a0, a1, b0, b1);
+}
+
+// ----------------------------------------------------------------- ModUp ----
+// For j < K: out_j = sum_{i<=l} ([x_i (Q_l/q_i)^-1]_{q_i} mod b_j) (Q_l/q_i mod b_j)
+// mod b_j (aoa-ntt/src/cuda/manager.cu:180-223).  The per-i values live in LDS
+// (one private column per thread); the sum over i is accumulated exactly in
+// 128 bits and reduced once.
+__global__ __launch_bounds__(256) void k_modup(DevParams P, const uint64_t* x, uint64_t* outB, int level) {
+  extern __shared__ uint64_t tl[];
+  const int L = level + 1;
+  const size_t N = (size_t)P.N;
+  const int tid = threadIdx.x;
+  const size_t cid = (size_t)blockIdx.x * 256 + tid;
+  const tw2* iq = reinterpret_cast<const tw2*>(P.inv_qhat) + (size_t)level * P.k;
+  for (int i = 0; i < L; ++i) {
+    const uint64_t p = P.primes[i].p;
+    const tw2 c = iq[i];
+    tl[i * 256 + tid] = mul_shoup(x[(size_t)i * N + cid], c.w, c.q, p);
+  }
+  for (int j = 0; j < P.K; ++j) {
+    const DevPrime db = P.primes[P.k + j];
+    const uint64_t* cj = P.qhat_mod_b + ((size_t)level * P.K + j) * P.k;
+    uint64_t hi = 0, lo = 0;
+    for (int i = 0; i < L; ++i) {
+      uint64_t v = tl[i * 256 + tid];
+      if (P.primes[i].p > db.p) v = barrett_reduce(0, v, db.p, db.mu, (int)db.nb);
+      const uint64_t c = cj[i];
+      const uint64_t ph = mulhi64(v, c), pl = v * c;
+      lo += pl;
+      hi += ph + (lo < pl ? 1u : 0u);
+    }
+    outB[(size_t)j * N + cid] = reduce128(hi, lo, db);
+  }
+}
+
+void launch_modup(const DevParams& P, const uint64_t* x, uint64_t* outB, int level, hipStream_t st) {
+  const size_t lds = (size_t)(level + 1) * 256 * sizeof(uint64_t);
+  k_modup<<<dim3(P.N / 256), 256, lds, st>>>(P, x, outB, level);
+}
+
+// --------------------------------------------------------------- ModDown ----
+// out_i = (x_i - sum_j ref([x_bj (B/b_j)^-1]_bj * (B/b_j mod q_i))) B^-1 mod q_i
+// (aoa-ntt/src/cuda/manager.cu:312-341), optionally + add_i, optionally
+// rescaled: out_i <- (out_i - out_l) q_l^-1 mod q_i (manager.cu:90-118).
+__global__ __launch_bounds__(256) void k_moddown(DevParams P, ModDownArgs A) {
+  extern __shared__ uint64_t aux[];
+  const size_t N = (size_t)P.N;
+  const int tid = threadIdx.x;
+  const int y = blockIdx.y;
+  const size_t cid = (size_t)blockIdx.x * 256 + tid;
+  const int K = P.K, k = P.k, level = A.level;
+  const uint64_t* inB = A.inB[y];
+  const uint64_t* inQ = A.inQ[y];
+  const uint64_t* add = A.add[y];
+  uint64_t* out = A.out[y];
+  const tw2* ib = reinterpret_cast<const tw2*>(P.inv_bhat);
+  for (int j = 0; j < K; ++j) {
+    const uint64_t bp = P.primes[k + j].p;
+    const tw2 c = ib[j];
+    aux[j * 256 + tid] = mul_shoup(inB[(size_t)j * N + cid], c.w, c.q, bp);
+  }
+  const tw2* ibq = reinterpret_cast<const tw2*>(P.inv_b_mod_q);
+  auto limb = [&](int i) -> uint64_t {
+    const DevPrime dq = P.primes[i];
+    const uint64_t* ci = P.bhat_mod_q + (size_t)i * K;
+    uint64_t acc = 0;
+    for (int j = 0; j < K; ++j) {
+      const uint64_t a = aux[j * 256 + tid];
+      const uint64_t c = ci[j];
+      const uint64_t t = ref_barrett(mulhi64(a, c), a * c, dq.p, dq.mu, (int)dq.nb);
+      acc = add_mod(acc, t, dq.p);
+    }
+    const tw2 bi = ibq[i];
+    uint64_t v = mul_shoup(sub_mod(inQ[(size_t)i * N + cid], acc, dq.p), bi.w, bi.q, dq.p);
+    if (add) v = add_mod(v, add[(size_t)i * N + cid], dq.p);
+    return v;
+  };
+  if (!A.rescale) {
+    for (int i = 0; i <= level; ++i) out[(size_t)i * N + cid] = limb(i);
+  } else {
+    const uint64_t top = limb(level);
+    const tw2* iql = reinterpret_cast<const tw2*>(P.inv_ql_mod_qi) + (size_t)level * k;
+    for (int i = 0; i < level; ++i) {
+      const uint64_t p = P.primes[i].p;
+      const uint64_t v = limb(i);
+      const tw2 s = iql[i];
+      out[(size_t)i * N + cid] = mul_shoup(sub_mod(v, top, p), s.w, s.q, p);
+    }
+  }
+}
+
+void launch_moddown(const DevParams& P, const ModDownArgs& a, hipStream_t st) {
+  const size_t lds = (size_t)P.K * 256 * sizeof(uint64_t);
+  k_moddown<<<dim3(P.N / 256, a.npolys), 256, lds, st>>>(P, a);
+}
+
+// --------------------------------------------------------------- rescale ----
+__global__ __launch_bounds__(256) void k_rescale(DevParams P, uint64_t* c0, uint64_t* c1, int level) {
+  const size_t N = (size_t)P.N;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int i = (int)(((size_t)blockIdx.x * 256) >> P.logN);
+  const size_t cid = e & (N - 1);
+  uint64_t* c = blockIdx.y ? c1 : c0;
+  const uint64_t p = P.primes[i].p;
+  const tw2 s = reinterpret_cast<const tw2*>(P.inv_ql_mod_qi)[(size_t)level * P.k + i];
+  const uint64_t top = c[(size_t)level * N + cid];
+  c[e] = mul_shoup(sub_mod(c[e], top, p), s.w, s.q, p);
+}
+
+void launch_rescale(const DevParams& P, uint64_t* c0, uint64_t* c1, int level, hipStream_t st) {
+  if (level <= 0) return;
+  const size_t total = (size_t)level * P.N;
+  k_rescale<<<dim3((unsigned)(total / 256), 2), 256, 0, st>>>(P, c0, c1, level);
+}
+
+// ---------------------------------------------------------- automorphism ----
+// Forward map (encoder.cu:300-322): X^c -> X^(c g mod 2N), negated on wrap.
+// Gather form: out[j] = in[t] if t = j g^-1 mod 2N < N, else -in[t - N].
+__global__ __launch_bounds__(256) void k_automorphism(DevParams P, uint64_t* out, const uint64_t* in,
+                                                      uint64_t ginv) {
+  const size_t N = (size_t)P.N;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int li = (int)(((size_t)blockIdx.x * 256) >> P.logN);
+  const uint64_t j = e & (N - 1);
+  const uint64_t p = P.primes[li].p;
+  const uint64_t t = (j * ginv) & (2 * N - 1);
+  const uint64_t* src = in + (size_t)li * N;
+  uint64_t v = t < N ? src[t] : neg_mod(src[t - N], p);
+  out[e] = v;
+}
+
+void launch_automorphism(const DevParams& P, uint64_t* out, const uint64_t* in, int nlimbs,
+                         uint64_t galois, hipStream_t st) {
+  // g^-1 mod 2N: the group (Z/2NZ)* has exponent N/2, so g^-1 = g^(N/2 - 1).
+  const uint64_t M = 2 * (uint64_t)P.N;
+  uint64_t r = 1, s = galois % M, e = (uint64_t)P.N / 2 - 1;
+  while (e) {
+    if (e & 1) r = (r * s) % M;
+    s = (s * s) % M;
+    e >>= 1;
+  }
+  const size_t total = (size_t)nlimbs * P.N;
+  k_automorphism<<<dim3((unsigned)(total / 256)), 256, 0, st>>>(P, out, in, r);
+}
+
+}  // namespace aoa

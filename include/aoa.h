@@ -1,0 +1,140 @@
+/*
+ * aoa.h -- C-ABI of the MI355X (gfx950) CKKS evaluation core.
+ *
+ * Plain C: opaque handles, raw device pointers (uint64_t residues, limb-major:
+ * limb r of a polynomial is at ptr + r*N), sizes and integer status codes.
+ * Every call is ordered on the HIP stream passed as `stream` (a hipStream_t,
+ * NULL = default stream) and never synchronises the device.  No call
+ * allocates or frees a caller buffer; scratch memory is passed in as a
+ * workspace of aoa_*_workspace_bytes() bytes.
+ *
+ * Layouts follow the reference (aoa-ntt/include/newckks/defines.h:66-72,
+ * aoa-ntt/src/arithmetic/poly_t.cpp:6-31): a Q polynomial has k limbs (primes
+ * q_0..q_{k-1}); a QB polynomial (switching keys) has k+kl limbs, Q limbs
+ * first.  At level l only limbs 0..l of a Q polynomial are meaningful.
+ */
+#ifndef AOA_H
+#define AOA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct aoa_params aoa_params;
+
+/* Status codes. */
+enum {
+  AOA_OK = 0,
+  AOA_ERR_ARG = 1,         /* bad argument (null pointer, size, level) */
+  AOA_ERR_HIP = 2,         /* a HIP runtime call failed */
+  AOA_ERR_LEVEL = 3,       /* level 0: no more rescaling is possible */
+  AOA_ERR_UNSUPPORTED = 4, /* N / prec / k not supported */
+  AOA_ERR_WORKSPACE = 5    /* workspace smaller than required */
+};
+
+/* Transform directions: transform_directions, aoa-ntt/include/newckks/defines.h:4-7. */
+enum { AOA_FORWARD = 0, AOA_INVERSE = 1 };
+
+/* Element-wise ops: supported_operations, aoa-ntt/include/newckks/defines.h:14-21. */
+enum { AOA_ADD = 0, AOA_SUB = 1, AOA_MUL = 2, AOA_NEGATE = 3, AOA_MULADD = 5 };
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* aoa_last_error(void);
+
+/* Parameter set: primes, NTT twiddles and RNS constants, resident in HBM.
+ * Replaces CUDAManager::init(k, kl, N, prec) (aoa-ntt/src/cuda/manager.cu:762-800,
+ * prime layout :426-464, constants :566-759, NTT tables
+ * aoa-ntt/src/cuda/htrans/ntt.cu:750-1052); kl <= 0 selects k+1 as the
+ * reference does.  Unlike the reference this is not a process singleton:
+ * handles are immutable and may be shared between threads and streams. */
+int aoa_params_create(int N, int k, int kl, int prec, aoa_params** out);
+/* CUDAManager::destroy (manager.cu:802-821). */
+void aoa_params_destroy(aoa_params* params);
+int aoa_params_info(const aoa_params* params, int* N, int* k, int* kl, int* prec);
+/* The k Q primes followed by the kl special primes (CUDAManager::RNSQPrimes,
+ * RNSBPrimes, manager.h:35-36). */
+int aoa_params_primes(const aoa_params* params, uint64_t* out, int n);
+
+/* Host copy of one constant table (for checking against independent math).
+ * which: 0 forward twiddles [k+kl][N] (psi^brev(i)), 1 inverse twiddles,
+ * 2 (Q_l/q_i)^-1 mod q_i [k][k], 3 (Q_l/q_i) mod b_j [k][kl][k] (l, j, i),
+ * 4 (B/b_j)^-1 mod b_j [kl], 5 (B/b_j) mod q_i [k][kl], 6 B^-1 mod q_i [k],
+ * 7 B mod q_i [k], 8 q_l^-1 mod q_i [k][k], 9 Barrett MU floor(4^nb/p) [k+kl].
+ * Returns the number of words written (or needed, if out is NULL). */
+int64_t aoa_params_table(const aoa_params* params, int which, uint64_t* out, int64_t n);
+
+/* Forward / inverse negacyclic NTT of limbs res0 .. res0+nres-1 of the
+ * polynomial at d_data (limb r at d_data + r*N, prime r).  Output order is the
+ * reference's bit-reversed order.  Replaces COMMONEngine::execute
+ * (aoa-ntt/src/cuda/htrans/common.cu:397-410 -> ntt.cu:463-475). */
+int aoa_ntt(const aoa_params* params, void* stream, uint64_t* d_data, int res0, int nres, int dir);
+
+/* out = a op b (MULADD: out = a*b + c) on limbs res0 .. res0+nres-1 (limb r
+ * uses prime r).  Replaces NTTEngine::execute_op / execute_seqop
+ * (aoa-ntt/src/cuda/htrans/ntt.cu:511-529, :650-667). */
+int aoa_eltwise(const aoa_params* params, void* stream, int op, uint64_t* out, const uint64_t* a,
+                const uint64_t* b, const uint64_t* c, int res0, int nres);
+
+/* Rescale c0 and c1 in place from `level` (coefficient domain): limb i < level
+ * becomes (c_i - c_level) q_level^-1 mod q_i.  Replaces
+ * CUDAManager::execute_rescale (aoa-ntt/src/cuda/manager.cu:90-139). */
+int aoa_rescale(const aoa_params* params, void* stream, uint64_t* c0, uint64_t* c1, int level);
+
+/* ModUp in place on a QB polynomial: reads Q limbs 0..level (coefficient
+ * domain) and writes its kl special limbs.  Replaces CUDAManager::execute_modup
+ * (aoa-ntt/src/cuda/manager.cu:178-249). */
+int aoa_modup(const aoa_params* params, void* stream, uint64_t* d_qb, int level);
+
+/* ModDown: d_q_out limbs 0..level <- ModDown of the QB polynomial d_qb_in
+ * (coefficient domain).  Replaces CUDAManager::execute_moddown
+ * (aoa-ntt/src/cuda/manager.cu:312-367), including its Barrett reduction of
+ * the cross-base products, so residues are bit-identical. */
+int aoa_moddown(const aoa_params* params, void* stream, uint64_t* d_q_out, const uint64_t* d_qb_in,
+                int level);
+
+/* Galois automorphism X -> X^(5^rot_slots mod 2N) of limbs 0..nres-1 in the
+ * coefficient domain (negated on wrap): rotate_slots_left's kernel
+ * (aoa-ntt/src/ckks/encoder.cu:300-353).  out and in must not overlap. */
+int aoa_automorphism(const aoa_params* params, void* stream, uint64_t* out, const uint64_t* in, int nres,
+                     int rot_slots);
+
+/* Workspace of aoa_keyswitch / aoa_rotate_left / aoa_hommul_relin_rescale at
+ * `level`. */
+size_t aoa_workspace_bytes(const aoa_params* params, int level);
+
+/* Key switching of d_in (Q limbs 0..level, coefficient domain) with the QB
+ * switching key (key_b, key_a) (NTT domain, k+kl limbs each):
+ * out0 = ModDown(ModUp(d) * key_b) (+ add0 if non-null), out1 = ModDown(ModUp(d)
+ * * key_a), coefficient domain, limbs 0..level.  The key-switch half of
+ * ckks_rotate_left / ckks_mul_without_rescale (aoa-ntt/src/ckks/ckks.cpp:302-315,
+ * :982-1009). */
+int aoa_keyswitch(const aoa_params* params, void* stream, uint64_t* out0, uint64_t* out1,
+                  const uint64_t* add0, const uint64_t* d_in, const uint64_t* key_b,
+                  const uint64_t* key_a, int level, void* workspace, size_t ws_bytes);
+
+/* HomMul + relinearisation + rescale: (c3_0, c3_1) <- rescale(relin(c1 * c2))
+ * at `level` (the minimum level of the inputs, >= 1).  Inputs in the NTT
+ * domain (limbs 0..level); outputs in the coefficient domain, limbs
+ * 0..level-1 (new level = level-1).  evk in the NTT domain.  Replaces
+ * ckks_mul (aoa-ntt/src/ckks/ckks.cpp:273-358).  Outputs must not alias inputs. */
+int aoa_hommul_relin_rescale(const aoa_params* params, void* stream, uint64_t* c3_0, uint64_t* c3_1,
+                             const uint64_t* c1_0, const uint64_t* c1_1, const uint64_t* c2_0,
+                             const uint64_t* c2_1, const uint64_t* evk_b, const uint64_t* evk_a, int level,
+                             void* workspace, size_t ws_bytes);
+
+/* Rotation by rot_slots slots to the left: automorphism of both polys
+ * (coefficient domain input) + key switch of c1 with rtk (NTT domain QB key).
+ * Output coefficient domain, limbs 0..level.  Replaces ckks_rotate_left
+ * (aoa-ntt/src/ckks/ckks.cpp:982-1009).  Outputs must not alias inputs. */
+int aoa_rotate_left(const aoa_params* params, void* stream, uint64_t* out0, uint64_t* out1,
+                    const uint64_t* c0, const uint64_t* c1, const uint64_t* rtk_b, const uint64_t* rtk_a,
+                    int rot_slots, int level, void* workspace, size_t ws_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AOA_H */

@@ -1,0 +1,145 @@
+"""Bit-exact parity of the HIP kernels (through the C-ABI) with the CPU oracle.
+
+Every integer output (NTT, element-wise, rescale, ModUp, ModDown, key switch,
+HomMul+relin+rescale, rotation) must equal the oracle's residues exactly on
+the same seeded uniform inputs; for Q polynomials at level l the contract is
+limbs 0..l (the reference leaves stale values above the level).
+"""
+import numpy as np
+import pytest
+
+from orc import Oracle
+
+pytestmark = pytest.mark.gpu
+
+# (N, k, kl, prec): C1 shape, small/medium shapes, and the reference's test params.
+SMALL = [(4096, 4, 5, 52), (8192, 6, 0, 48), (16384, 5, 5, 55), (32768, 4, 0, 48), (4096, 3, 3, 45)]
+
+
+def _params(N, k, kl, prec):
+    from aoa_amd import Params
+    return Params(N, k, kl, prec), Oracle(N, k, kl, prec)
+
+
+@pytest.mark.parametrize("shape", SMALL + [(65536, 3, 0, 48)])
+def test_ntt_parity(gpu, shape):
+    P, O = _params(*shape)
+    rng = np.random.default_rng(11)
+    R = O.k + O.kl
+    x = O.random_poly(rng, R)
+    d = P.to_device(x)
+    P.ntt(d, 0, R, 0)
+    ref = x.copy()
+    O.ntt(ref, 0, R, 0)
+    got = P.to_host(d)
+    assert np.array_equal(got, ref), "forward NTT mismatch %s" % (shape,)
+    P.ntt(d, 0, R, 1)
+    assert np.array_equal(P.to_host(d), x), "inverse NTT is not the exact inverse"
+    # a sub-range of limbs (special limbs only) uses the right primes
+    y = O.random_poly(rng, O.kl, first=O.k)
+    full = np.zeros(R * O.N, dtype=np.uint64)
+    full[O.k * O.N:] = y
+    dd = P.to_device(full)
+    P.ntt(dd, O.k, O.kl, 0)
+    ref2 = full.copy()
+    O.ntt(ref2, O.k, O.kl, 0)
+    assert np.array_equal(P.to_host(dd), ref2)
+
+
+@pytest.mark.parametrize("op", [0, 1, 2, 3])
+def test_eltwise_parity(gpu, op):
+    P, O = _params(8192, 5, 0, 48)
+    rng = np.random.default_rng(3 + op)
+    R = O.k + O.kl
+    a = O.random_poly(rng, R)
+    b = O.random_poly(rng, R)
+    a[:7] = 0  # zeros exercise the NEGATE special case
+    out = P.zeros(R)
+    P.eltwise(op, out, P.to_device(a), P.to_device(b), res0=0, nres=R)
+    ref = np.zeros_like(a)
+    O.eltwise(op, ref, a, b, R)
+    assert np.array_equal(P.to_host(out), ref)
+
+
+@pytest.mark.parametrize("shape", SMALL)
+def test_rescale_modup_moddown_parity(gpu, shape):
+    P, O = _params(*shape)
+    rng = np.random.default_rng(5)
+    k, kl, N = O.k, O.kl, O.N
+    for level in sorted({1, k - 1}):
+        # rescale
+        a = O.random_poly(rng, k)
+        b = O.random_poly(rng, k)
+        da, db = P.to_device(a), P.to_device(b)
+        P.rescale(da, db, level)
+        O.rescale(a, b, level)
+        assert np.array_equal(P.to_host(da)[:level * N], a[:level * N])
+        assert np.array_equal(P.to_host(db)[:level * N], b[:level * N])
+        # ModUp: Q limbs 0..level -> special limbs
+        qb = O.random_poly(rng, k + kl)
+        dqb = P.to_device(qb)
+        P.modup(dqb, level)
+        O.modup(qb, level)
+        assert np.array_equal(P.to_host(dqb)[k * N:], qb[k * N:]), "ModUp level %d" % level
+        # ModDown: QB -> Q limbs 0..level
+        qb2 = O.random_poly(rng, k + kl)
+        out = P.zeros(k)
+        P.moddown(out, P.to_device(qb2), level)
+        ref = np.zeros(k * N, dtype=np.uint64)
+        O.moddown(ref, qb2, level)
+        assert np.array_equal(P.to_host(out)[:(level + 1) * N], ref[:(level + 1) * N]), "ModDown level %d" % level
+
+
+@pytest.mark.parametrize("shape", SMALL)
+def test_hommul_parity(gpu, shape):
+    P, O = _params(*shape)
+    rng = np.random.default_rng(7)
+    k, kl, N = O.k, O.kl, O.N
+    for level in sorted({1, k - 1}):
+        c1 = [O.random_poly(rng, k) for _ in range(2)]
+        c2 = [O.random_poly(rng, k) for _ in range(2)]
+        evk = [O.random_poly(rng, k + kl) for _ in range(2)]
+        ref0, ref1 = O.hommul(c1, c2, evk, level)
+        d = [P.to_device(x) for x in c1 + c2 + evk]
+        out = [P.zeros(k), P.zeros(k)]
+        ws = P.workspace(level)
+        P.hommul(out, (d[0], d[1]), (d[2], d[3]), (d[4], d[5]), level, ws)
+        n = level * N
+        assert np.array_equal(P.to_host(out[0])[:n], ref0[:n]), "c0 level %d" % level
+        assert np.array_equal(P.to_host(out[1])[:n], ref1[:n]), "c1 level %d" % level
+
+
+@pytest.mark.parametrize("shape", SMALL[:3])
+def test_rotate_parity(gpu, shape):
+    P, O = _params(*shape)
+    rng = np.random.default_rng(9)
+    k, kl, N = O.k, O.kl, O.N
+    level = k - 1
+    for rot in (1, 4, N // 4 - 1):
+        c = [O.random_poly(rng, k) for _ in range(2)]
+        rtk = [O.random_poly(rng, k + kl) for _ in range(2)]
+        ref0, ref1 = O.rotate_left(c, rtk, rot, level)
+        d = [P.to_device(x) for x in c + rtk]
+        out = [P.zeros(k), P.zeros(k)]
+        P.rotate_left(out, (d[0], d[1]), (d[2], d[3]), rot, level, P.workspace(level))
+        n = (level + 1) * N
+        assert np.array_equal(P.to_host(out[0])[:n], ref0[:n]), "rot %d c0" % rot
+        assert np.array_equal(P.to_host(out[1])[:n], ref1[:n]), "rot %d c1" % rot
+
+
+def test_hommul_c2_parity(gpu):
+    """Config 2 shape: N=2^16, 30 Q limbs (prec 48), 31 special limbs, top level."""
+    P, O = _params(65536, 30, 0, 48)
+    rng = np.random.default_rng(2)
+    k, kl, N = O.k, O.kl, O.N
+    level = k - 1
+    c1 = [O.random_poly(rng, k) for _ in range(2)]
+    c2 = [O.random_poly(rng, k) for _ in range(2)]
+    evk = [O.random_poly(rng, k + kl) for _ in range(2)]
+    ref0, ref1 = O.hommul(c1, c2, evk, level)
+    d = [P.to_device(x) for x in c1 + c2 + evk]
+    out = [P.zeros(k), P.zeros(k)]
+    P.hommul(out, (d[0], d[1]), (d[2], d[3]), (d[4], d[5]), level, P.workspace(level))
+    n = level * N
+    assert np.array_equal(P.to_host(out[0])[:n], ref0[:n])
+    assert np.array_equal(P.to_host(out[1])[:n], ref1[:n])
