@@ -135,7 +135,7 @@ extern "C" int64_t aoa_params_table(const aoa_params* h, int which, uint64_t* ou
     case 2: take(H.inv_qhat, 2); break;
     case 3: take(H.qhat_mod_b, 1); break;
     case 4: take(H.inv_bhat, 2); break;
-    case 5: take(H.bhat_mod_q, 1); break;
+    case 5: take(H.bhat_mod_q, 2); break;
     case 6: take(H.inv_b_mod_q, 2); break;
     case 7: take(H.b_mod_q, 2); break;
     case 8: take(H.inv_ql_mod_qi, 2); break;

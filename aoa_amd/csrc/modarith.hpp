@@ -124,6 +124,20 @@ AOA_HD uint64_t ref_barrett(uint64_t hi, uint64_t lo, uint64_t p, uint64_t mu, i
   return r >= p ? r - p : r;
 }
 
+// ref_barrett(a * w) for a constant w < p with Shoup quotient wq, computed
+// from the exact remainder r = a*w mod p.  Writing A = a*w - qhat*p =
+// delta*p + r (qhat the reference's Barrett quotient, delta >= 0 its error)
+// and eps = 2^nb - p, the reference returns r - eps when eps <= r < delta*eps
+// and r otherwise, provided delta*eps < 2^nb.  [lo, thr) must cover
+// [eps, delta_max*eps) for every product that can occur; inside it the
+// literal formula is evaluated.
+AOA_HD uint64_t ref_mulmod_shoup(uint64_t a, uint64_t w, uint64_t wq, uint64_t p, uint64_t mu, int nb,
+                                 uint64_t lo, uint64_t thr) {
+  uint64_t r = mul_shoup(a, w, wq, p);
+  if (r >= lo && r < thr) r = ref_barrett(mulhi64(a, w), a * w, p, mu, nb);
+  return r;
+}
+
 // Bit length of a nonzero 64-bit value.
 AOA_HD int bitlen64(uint64_t x) {
   int n = 0;

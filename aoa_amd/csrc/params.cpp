@@ -176,9 +176,11 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
     P.inv_bhat[2 * j] = inv;
     P.inv_bhat[2 * j + 1] = shoup_quot(inv, bj);
   }
-  P.bhat_mod_q.assign((size_t)k * kl, 0);
+  P.bhat_mod_q.assign((size_t)k * kl * 2, 0);
   P.inv_b_mod_q.assign((size_t)k * 2, 0);
   P.b_mod_q.assign((size_t)k * 2, 0);
+  uint64_t bmax = 0;
+  for (uint64_t b : Bp) bmax = b > bmax ? b : bmax;
   for (int i = 0; i < k; ++i) {
     uint64_t qi = Qp[i];
     uint64_t bfull = 1 % qi;
@@ -187,7 +189,23 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
       uint64_t pr = 1 % qi;
       for (int m = 0; m < kl; ++m)
         if (m != j) pr = mulm(pr, Bp[m] % qi, qi);
-      P.bhat_mod_q[(size_t)i * kl + j] = pr;
+      P.bhat_mod_q[((size_t)i * kl + j) * 2] = pr;
+      P.bhat_mod_q[((size_t)i * kl + j) * 2 + 1] = shoup_quot(pr, qi);
+    }
+    // Barrett quotient error bound for products below bmax * q_i:
+    // delta < (bmax q_i) / 2^(2nb) + 2.  The reference result deviates from
+    // the exact remainder only for eps <= r < delta * eps, which needs
+    // delta * eps < 2^nb; otherwise every product takes the literal formula.
+    DevPrime& d = P.dprimes[i];
+    unsigned __int128 pmax = (unsigned __int128)bmax * qi;
+    unsigned __int128 dmax = (pmax >> (2 * d.nb)) + 3;
+    unsigned __int128 thr = dmax * d.eps;
+    if (thr < ((unsigned __int128)1 << (d.nb - 1))) {
+      d.quirk_lo = d.eps;
+      d.quirk_thr = (uint64_t)thr;
+    } else {
+      d.quirk_lo = 0;
+      d.quirk_thr = ~(uint64_t)0;
     }
     uint64_t inv = inv_mod(bfull, qi);
     P.inv_b_mod_q[2 * i] = inv;

@@ -29,8 +29,14 @@ struct DevPrime {
   uint64_t w1ninv_q; // its Shoup quotient
   uint64_t r64;      // 2^64 mod p (reduction of 128-bit sums)
   uint64_t r64_q;    // its Shoup quotient
+  // ModDown cross-base products a*c (a < max special prime, c < p): the
+  // reference's Barrett result equals the exact remainder r unless
+  // quirk_lo <= r < quirk_thr (see k_moddown); quirk_lo = eps and
+  // quirk_thr = delta_max * eps, or [0, 2^64) when that bound is not usable.
+  uint64_t quirk_lo;
+  uint64_t quirk_thr;
   uint32_t nb;       // bit length of p
-  uint32_t pad[3];
+  uint32_t pad;
 };
 
 // Pointers into one device allocation holding every table of a parameter set.
@@ -43,7 +49,7 @@ struct DevParams {
   const uint64_t* inv_qhat;      // [k][k][2]: (l,i) -> (Q_l/q_i)^-1 mod q_i, Shoup
   const uint64_t* qhat_mod_b;    // [k][K][k]: (l,j,i) -> (Q_l/q_i) mod b_j
   const uint64_t* inv_bhat;      // [K][2]: (B/b_j)^-1 mod b_j, Shoup
-  const uint64_t* bhat_mod_q;    // [k][K]: (i,j) -> (B/b_j) mod q_i
+  const uint64_t* bhat_mod_q;    // [k][K][2]: (i,j) -> (B/b_j) mod q_i, Shoup
   const uint64_t* inv_b_mod_q;   // [k][2]: B^-1 mod q_i, Shoup
   const uint64_t* b_mod_q;       // [k][2]: B mod q_i, Shoup
   const uint64_t* inv_ql_mod_qi; // [k][k][2]: (l,i) -> q_l^-1 mod q_i, Shoup

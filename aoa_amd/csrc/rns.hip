@@ -75,32 +75,42 @@ void launch_tensor(const DevParams& P, uint64_t* d0, uint64_t* d1, uint64_t* d2,
   k_tensor<<<dim3((unsigned)(total / 512)), 256, 0, st>>>(P, d0, d1, d2, a0, a1, b0, b1);
 }
 
+// Basis conversion workgroups: 64 consecutive coefficients x 4 limb groups
+// (wave g of the workgroup handles limbs g, g+4, g+8, ...), so a launch has
+// N/64 workgroups per polynomial and each wave reads 512 contiguous bytes.
+constexpr int BC_COEF = 64;
+constexpr int BC_GROUPS = 4;
+
 // ----------------------------------------------------------------- ModUp ----
 // For j < K: out_j = sum_{i<=l} ([x_i (Q_l/q_i)^-1]_{q_i} mod b_j) (Q_l/q_i mod b_j)
-// mod b_j (aoa-ntt/src/cuda/manager.cu:180-223).  The per-i values live in LDS
-// (one private column per thread); the sum over i is accumulated exactly in
-// 128 bits and reduced once.
+// mod b_j (aoa-ntt/src/cuda/manager.cu:180-223).  The per-i values are shared
+// through LDS; the sum over i is accumulated exactly in 128 bits and reduced
+// once, which equals the reference's term-by-term modular sum.
 __global__ __launch_bounds__(256) void k_modup(DevParams P, const uint64_t* x, uint64_t* outB, int level) {
-  extern __shared__ uint64_t tl[];
+  extern __shared__ uint64_t tl[];  // [L][64]
   const int L = level + 1;
   const size_t N = (size_t)P.N;
-  const int tid = threadIdx.x;
-  const size_t cid = (size_t)blockIdx.x * 256 + tid;
+  const int c = threadIdx.x % BC_COEF;
+  // wave-uniform limb group: keeps limb indices and constant loads scalar
+  const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / BC_COEF));
+  const size_t cid = (size_t)blockIdx.x * BC_COEF + c;
   const tw2* iq = reinterpret_cast<const tw2*>(P.inv_qhat) + (size_t)level * P.k;
-  for (int i = 0; i < L; ++i) {
+  for (int i = g; i < L; i += BC_GROUPS) {
     const uint64_t p = P.primes[i].p;
-    const tw2 c = iq[i];
-    tl[i * 256 + tid] = mul_shoup(x[(size_t)i * N + cid], c.w, c.q, p);
+    const tw2 w = iq[i];
+    tl[i * BC_COEF + c] = mul_shoup(x[(size_t)i * N + cid], w.w, w.q, p);
   }
-  for (int j = 0; j < P.K; ++j) {
+  __syncthreads();
+  for (int j = g; j < P.K; j += BC_GROUPS) {
     const DevPrime db = P.primes[P.k + j];
     const uint64_t* cj = P.qhat_mod_b + ((size_t)level * P.K + j) * P.k;
     uint64_t hi = 0, lo = 0;
+#pragma unroll 4
     for (int i = 0; i < L; ++i) {
-      uint64_t v = tl[i * 256 + tid];
+      uint64_t v = tl[i * BC_COEF + c];
       if (P.primes[i].p > db.p) v = barrett_reduce(0, v, db.p, db.mu, (int)db.nb);
-      const uint64_t c = cj[i];
-      const uint64_t ph = mulhi64(v, c), pl = v * c;
+      const uint64_t w = cj[i];
+      const uint64_t ph = mulhi64(v, w), pl = v * w;
       lo += pl;
       hi += ph + (lo < pl ? 1u : 0u);
     }
@@ -109,64 +119,90 @@ __global__ __launch_bounds__(256) void k_modup(DevParams P, const uint64_t* x, u
 }
 
 void launch_modup(const DevParams& P, const uint64_t* x, uint64_t* outB, int level, hipStream_t st) {
-  const size_t lds = (size_t)(level + 1) * 256 * sizeof(uint64_t);
-  k_modup<<<dim3(P.N / 256), 256, lds, st>>>(P, x, outB, level);
+  const size_t lds = (size_t)(level + 1) * BC_COEF * sizeof(uint64_t);
+  k_modup<<<dim3(P.N / BC_COEF), 256, lds, st>>>(P, x, outB, level);
 }
 
 // --------------------------------------------------------------- ModDown ----
 // out_i = (x_i - sum_j ref([x_bj (B/b_j)^-1]_bj * (B/b_j mod q_i))) B^-1 mod q_i
 // (aoa-ntt/src/cuda/manager.cu:312-341), optionally + add_i, optionally
 // rescaled: out_i <- (out_i - out_l) q_l^-1 mod q_i (manager.cu:90-118).
+//
+// ref(a*c) is the reference's Barrett reduction of the product (ref_barrett).
+// With A = a*c - qhat*p = delta*p + r (r the exact remainder, delta the
+// Barrett quotient error, 0 <= delta*eps < 2^nb, eps = 2^nb - p), its result
+// is r - eps when eps <= r < delta*eps and r otherwise.  So the exact Shoup
+// remainder is used unless r falls in [eps, delta_max*eps), a window of
+// probability about 2^-17 per term at 48-bit q_i, where the literal formula
+// is evaluated.
+__device__ __forceinline__ uint64_t moddown_term(uint64_t a, const tw2 w, const DevPrime& dq) {
+  return ref_mulmod_shoup(a, w.w, w.q, dq.p, dq.mu, (int)dq.nb, dq.quirk_lo, dq.quirk_thr);
+}
+
 __global__ __launch_bounds__(256) void k_moddown(DevParams P, ModDownArgs A) {
-  extern __shared__ uint64_t aux[];
+  extern __shared__ uint64_t aux[];  // [K][64] then [4][64] partial sums
   const size_t N = (size_t)P.N;
-  const int tid = threadIdx.x;
+  const int c = threadIdx.x % BC_COEF;
+  // wave-uniform limb group: keeps limb indices and constant loads scalar
+  const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / BC_COEF));
   const int y = blockIdx.y;
-  const size_t cid = (size_t)blockIdx.x * 256 + tid;
+  const size_t cid = (size_t)blockIdx.x * BC_COEF + c;
   const int K = P.K, k = P.k, level = A.level;
   const uint64_t* inB = A.inB[y];
   const uint64_t* inQ = A.inQ[y];
   const uint64_t* add = A.add[y];
   uint64_t* out = A.out[y];
   const tw2* ib = reinterpret_cast<const tw2*>(P.inv_bhat);
-  for (int j = 0; j < K; ++j) {
+  for (int j = g; j < K; j += BC_GROUPS) {
     const uint64_t bp = P.primes[k + j].p;
-    const tw2 c = ib[j];
-    aux[j * 256 + tid] = mul_shoup(inB[(size_t)j * N + cid], c.w, c.q, bp);
+    const tw2 w = ib[j];
+    aux[j * BC_COEF + c] = mul_shoup(inB[(size_t)j * N + cid], w.w, w.q, bp);
   }
+  __syncthreads();
   const tw2* ibq = reinterpret_cast<const tw2*>(P.inv_b_mod_q);
-  auto limb = [&](int i) -> uint64_t {
+  auto conv = [&](int i, int j0, int j1) -> uint64_t {
     const DevPrime dq = P.primes[i];
-    const uint64_t* ci = P.bhat_mod_q + (size_t)i * K;
+    const tw2* ci = reinterpret_cast<const tw2*>(P.bhat_mod_q) + (size_t)i * K;
     uint64_t acc = 0;
-    for (int j = 0; j < K; ++j) {
-      const uint64_t a = aux[j * 256 + tid];
-      const uint64_t c = ci[j];
-      const uint64_t t = ref_barrett(mulhi64(a, c), a * c, dq.p, dq.mu, (int)dq.nb);
-      acc = add_mod(acc, t, dq.p);
-    }
+#pragma unroll 4
+    for (int j = j0; j < j1; ++j) acc = add_mod(acc, moddown_term(aux[j * BC_COEF + c], ci[j], dq), dq.p);
+    return acc;
+  };
+  auto finish = [&](int i, uint64_t acc) -> uint64_t {
+    const uint64_t p = P.primes[i].p;
     const tw2 bi = ibq[i];
-    uint64_t v = mul_shoup(sub_mod(inQ[(size_t)i * N + cid], acc, dq.p), bi.w, bi.q, dq.p);
-    if (add) v = add_mod(v, add[(size_t)i * N + cid], dq.p);
+    uint64_t v = mul_shoup(sub_mod(inQ[(size_t)i * N + cid], acc, p), bi.w, bi.q, p);
+    if (add) v = add_mod(v, add[(size_t)i * N + cid], p);
     return v;
   };
   if (!A.rescale) {
-    for (int i = 0; i <= level; ++i) out[(size_t)i * N + cid] = limb(i);
-  } else {
-    const uint64_t top = limb(level);
-    const tw2* iql = reinterpret_cast<const tw2*>(P.inv_ql_mod_qi) + (size_t)level * k;
-    for (int i = 0; i < level; ++i) {
-      const uint64_t p = P.primes[i].p;
-      const uint64_t v = limb(i);
-      const tw2 s = iql[i];
-      out[(size_t)i * N + cid] = mul_shoup(sub_mod(v, top, p), s.w, s.q, p);
-    }
+    for (int i = g; i <= level; i += BC_GROUPS) out[(size_t)i * N + cid] = finish(i, conv(i, 0, K));
+    return;
+  }
+  // Rescale needs limb `level` everywhere: its sum over j is split across the
+  // four groups and combined through LDS.
+  uint64_t* part = aux + (size_t)K * BC_COEF;
+  const int jq = (K + BC_GROUPS - 1) / BC_GROUPS;
+  const int j0 = g * jq, j1 = min(K, j0 + jq);
+  part[g * BC_COEF + c] = conv(level, j0, j1 > j0 ? j1 : j0);
+  __syncthreads();
+  const uint64_t pl = P.primes[level].p;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int t = 0; t < BC_GROUPS; ++t) acc = add_mod(acc, part[t * BC_COEF + c], pl);
+  const uint64_t top = finish(level, acc);
+  const tw2* iql = reinterpret_cast<const tw2*>(P.inv_ql_mod_qi) + (size_t)level * k;
+  for (int i = g; i < level; i += BC_GROUPS) {
+    const uint64_t p = P.primes[i].p;
+    const uint64_t v = finish(i, conv(i, 0, K));
+    const tw2 s = iql[i];
+    out[(size_t)i * N + cid] = mul_shoup(sub_mod(v, top, p), s.w, s.q, p);
   }
 }
 
 void launch_moddown(const DevParams& P, const ModDownArgs& a, hipStream_t st) {
-  const size_t lds = (size_t)P.K * 256 * sizeof(uint64_t);
-  k_moddown<<<dim3(P.N / 256, a.npolys), 256, lds, st>>>(P, a);
+  const size_t lds = (size_t)(P.K + BC_GROUPS) * BC_COEF * sizeof(uint64_t);
+  k_moddown<<<dim3(P.N / BC_COEF, a.npolys), 256, lds, st>>>(P, a);
 }
 
 // --------------------------------------------------------------- rescale ----

@@ -1,0 +1,356 @@
+"""CPU pins of the oracle (and of the product's host tables) against
+mathematics that does not repeat the oracle's own formulas:
+
+* the reference's Barrett `mod` vs the exact 128-bit remainder in range, and
+  vs the product's independent restatement everywhere;
+* the NTT vs the O(N^2) closed form sum_n x[n] psi^(n(2 brev(pos)+1));
+  INTT o NTT = id; NTT products vs schoolbook negacyclic convolution;
+* RNS constants vs Python big integers; ModUp vs the CRT identity; ModDown
+  vs the approximate-base-conversion identity (where the reference's
+  Barrett is exact) and vs the deviation window elsewhere; rescale vs
+  (a - a_l)/q_l; the automorphism vs a(X^g) mod X^N + 1.
+"""
+import math
+import random
+
+import numpy as np
+import pytest
+
+import native_chk
+from orc import Oracle
+
+
+def brev(x, bits):
+    r = 0
+    for _ in range(bits):
+        r = (r << 1) | (x & 1)
+        x >>= 1
+    return r
+
+
+def is_primitive_root(g, p):
+    n = p - 1
+    fs, m, d = set(), n, 2
+    while d * d <= m and d < 1 << 20:
+        while m % d == 0:
+            fs.add(d)
+            m //= d
+        d += 1
+    if m > 1:
+        # remaining cofactor: prime if no small factor and Fermat-probable
+        fs.add(m)
+    return all(pow(g, n // f, p) != 1 for f in fs)
+
+
+# ------------------------------------------------------------------ primes ---
+
+def test_primes_layout():
+    # manager.cu:426-464: q0 is the first 63-bit prime, then prec-bit primes;
+    # special primes are 55-bit (prec 48/52) or continue the bucket (45/55).
+    o = Oracle(4096, 4, 5, 52)
+    assert o.primes[0] == 9223372036801560577
+    assert all(p.bit_length() == 52 for p in o.primes[1:4])
+    assert all(p.bit_length() == 55 for p in o.primes[4:])
+    o55 = Oracle(4096, 3, 3, 55)
+    assert o55.primes[3] < o55.primes[2]  # B continues the descending 55-bit bucket
+    for o_ in (o, o55, Oracle(65536, 30, 31, 48)):
+        assert all(o_.primes[i] > o_.primes[i + 1] for i in range(1, o_.k - 1))
+        for p, g in zip(o_.primes, o_.roots):
+            assert (p - 1) % (2 * 65536) == 0 or o_.N < 65536
+            assert pow(g, p - 1, p) == 1
+    # the product carries the same primes
+    assert list(native_chk.table(65536, 30, 31, 48, 10)) == Oracle(65536, 30, 31, 48).primes
+
+
+def test_roots_are_smallest_primitive_roots():
+    o = Oracle(65536, 30, 31, 48)
+    for p, g in list(zip(o.primes, o.roots))[:6] + list(zip(o.primes, o.roots))[-3:]:
+        assert is_primitive_root(g, p)
+        assert not any(is_primitive_root(h, p) for h in range(2, g))
+
+
+# ------------------------------------------------------------- Barrett mod ---
+
+def test_reference_mod_exact_in_range():
+    rng = random.Random(1)
+    for shape in [(4096, 4, 5, 52), (8192, 6, 0, 48), (4096, 3, 3, 55), (4096, 3, 3, 45)]:
+        o = Oracle(*shape)
+        for rid, p in enumerate(o.primes):
+            for t in range(300):
+                a, b = rng.randrange(p), rng.randrange(p)
+                if t < 4:
+                    a, b = [(0, 0), (p - 1, p - 1), (1, p - 1), (p - 1, 2)][t]
+                assert o.mod128(rid, a * b) == (a * b) % p
+
+
+def test_reference_mod_restatements_agree():
+    """The oracle's and the product's restatements of common.cu:257-291 agree on
+    products far outside the exact range (ModDown's a < b_j times c < q_i)."""
+    chk = native_chk.load()
+    rng = random.Random(2)
+    o = Oracle(65536, 30, 31, 48)
+    mu = o.table(9)
+    for rid in range(o.k):
+        p = o.primes[rid]
+        nb = p.bit_length()
+        assert int(mu[rid]) == (1 << (2 * nb)) // p
+        for _ in range(200):
+            v = rng.randrange(1 << 55) * rng.randrange(p)
+            a = o.mod128(rid, v)
+            b = chk.chk_ref_barrett(v >> 64, v & (2**64 - 1), p, int(mu[rid]), nb)
+            assert a == b
+            assert a < p and (a == v % p or (a - v) % p == (-(1 << nb) + p) % p)
+
+
+def test_moddown_fast_term_matches_reference_formula():
+    """Outside [eps, delta_max*eps) the reference Barrett equals the exact
+    remainder; the product's Shoup fast path relies on it.  The deviation is
+    real at prec 48 and 52 and absent at 45/55."""
+    import ctypes
+    chk = native_chk.load()
+    for shape, expect_quirk in [((65536, 30, 31, 48), True), ((4096, 4, 5, 52), True),
+                                ((16384, 5, 5, 55), False), ((4096, 10, 11, 45), False)]:
+        out = (ctypes.c_int64 * 3)()
+        assert chk.chk_moddown_terms(*shape, 40000, 11, out) == 0
+        assert out[0] == 0, "fast ModDown term differs from the reference formula %s" % (shape,)
+        assert (out[1] > 0) == expect_quirk
+
+
+def test_product_exact_mults():
+    chk = native_chk.load()
+    for shape in [(65536, 30, 31, 48), (4096, 4, 5, 52), (4096, 5, 5, 55)]:
+        assert chk.chk_exact_mults(*shape, 3000, 5) == 0
+
+
+# ---------------------------------------------------------------------- NTT ---
+
+@pytest.mark.parametrize("N", [128, 256, 512, 1024])
+def test_ntt_closed_form_bruteforce(N):
+    o = Oracle(N, 3, 2, 55)
+    rng = np.random.default_rng(N)
+    lg = N.bit_length() - 1
+    for rid in (0, 2, 4):
+        p, g = o.primes[rid], o.roots[rid]
+        psi = pow(g, (p - 1) // (2 * N), p)
+        assert pow(psi, N, p) == p - 1
+        full = np.zeros(o.N * (o.k + o.kl), dtype=np.uint64)
+        x = rng.integers(0, p, size=N, dtype=np.uint64)
+        full[rid * N:(rid + 1) * N] = x
+        o.ntt(full, rid, 1, 0)
+        y = full[rid * N:(rid + 1) * N]
+        xs = [int(v) for v in x]
+        positions = range(N) if N <= 256 else rng.choice(N, 48, replace=False)
+        for pos in positions:
+            w = pow(psi, 2 * brev(int(pos), lg) + 1, p)
+            acc, t = 0, 1
+            for n in range(N):
+                acc += xs[n] * t
+                t = t * w % p
+            assert acc % p == int(y[pos])
+
+
+@pytest.mark.parametrize("shape", [(4096, 4, 5, 52), (32768, 3, 0, 48), (65536, 2, 0, 48)])
+def test_ntt_roundtrip_and_sampled_closed_form(shape):
+    o = Oracle(*shape)
+    rng = np.random.default_rng(7)
+    R = o.k + o.kl
+    x = o.random_poly(rng, R)
+    y = x.copy()
+    o.ntt(y, 0, R, 0)
+    z = y.copy()
+    o.ntt(z, 0, R, 1)
+    assert np.array_equal(z, x)
+    N = o.N
+    lg = N.bit_length() - 1
+    rid = R - 1
+    p, g = o.primes[rid], o.roots[rid]
+    psi = pow(g, (p - 1) // (2 * N), p)
+    xs = [int(v) for v in x[rid * N:(rid + 1) * N]]
+    for pos in (0, 1, N // 2 + 3, N - 1):
+        w = pow(psi, 2 * brev(pos, lg) + 1, p)
+        acc, t = 0, 1
+        for n in range(N):
+            acc += xs[n] * t
+            t = t * w % p
+        assert acc % p == int(y[rid * N + pos])
+
+
+def test_ntt_negacyclic_product():
+    N = 256
+    o = Oracle(N, 3, 2, 55)
+    rng = np.random.default_rng(3)
+    R = o.k + o.kl
+    a, b = o.random_poly(rng, R), o.random_poly(rng, R)
+    A, B = a.copy(), b.copy()
+    o.ntt(A, 0, R, 0)
+    o.ntt(B, 0, R, 0)
+    C = np.zeros_like(A)
+    o.eltwise(2, C, A, B, R)
+    o.ntt(C, 0, R, 1)
+    for rid in (0, R - 1):
+        p = o.primes[rid]
+        aa = [int(v) for v in a[rid * N:(rid + 1) * N]]
+        bb = [int(v) for v in b[rid * N:(rid + 1) * N]]
+        c = [0] * N
+        for i in range(N):
+            for j in range(N):
+                if i + j < N:
+                    c[i + j] += aa[i] * bb[j]
+                else:
+                    c[i + j - N] -= aa[i] * bb[j]
+        assert [v % p for v in c] == [int(v) for v in C[rid * N:(rid + 1) * N]]
+
+
+def test_product_twiddles_match_definition():
+    N, k, kl, prec = 4096, 3, 2, 55
+    o = Oracle(N, k, kl, prec)
+    tf = native_chk.table(N, k, kl, prec, 0).reshape(k + kl, N)
+    ti = native_chk.table(N, k, kl, prec, 1).reshape(k + kl, N)
+    for rid in range(k + kl):
+        p, g = o.primes[rid], o.roots[rid]
+        psi = pow(g, (p - 1) // (2 * N), p)
+        for i in (0, 1, 2, 3, 100, N - 1):
+            e = brev(i, 12)
+            assert int(tf[rid, i]) == pow(psi, e, p)
+            assert int(ti[rid, i]) == pow(psi, -e, p)
+
+
+# ------------------------------------------------------------ RNS constants ---
+
+@pytest.mark.parametrize("shape", [(4096, 4, 5, 52), (65536, 30, 31, 48), (4096, 5, 5, 55)])
+def test_rns_constants_bigint(shape):
+    N, k, kl, prec = shape
+    o = Oracle(*shape)
+    kl = o.kl
+    Q, B = o.primes[:k], o.primes[k:]
+    Bp = math.prod(B)
+    tabs = {w: (o.table(w), native_chk.table(N, k, kl, prec, w)) for w in range(2, 10)}
+    for w, (a, b) in tabs.items():
+        if w == 3:
+            continue
+        assert np.array_equal(a, b), "table %d differs between oracle and product" % w
+    inv_qhat, qhat_b, inv_bhat, bhat_q, inv_b, b_q, inv_ql = (tabs[w][0] for w in (2, 3, 4, 5, 6, 7, 8))
+    qhat_b_prod = tabs[3][1]
+    for l in {0, 1, k - 1}:
+        Ql = math.prod(Q[:l + 1])
+        for i in range(l + 1):
+            qh = Ql // Q[i]
+            assert int(inv_qhat[l * k + i]) == pow(qh % Q[i], -1, Q[i])
+            for j in range(kl):
+                assert int(qhat_b[(l * kl + j) * k + i]) == qh % B[j]
+                assert int(qhat_b_prod[(l * kl + j) * k + i]) == qh % B[j]
+        for i in range(k):
+            if i != l:
+                assert int(inv_ql[l * k + i]) == pow(Q[l], -1, Q[i])
+    for j in range(kl):
+        assert int(inv_bhat[j]) == pow((Bp // B[j]) % B[j], -1, B[j])
+    for i in range(k):
+        assert int(b_q[i]) == Bp % Q[i]
+        assert int(inv_b[i]) == pow(Bp, -1, Q[i])
+        for j in range(kl):
+            assert int(bhat_q[i * kl + j]) == (Bp // B[j]) % Q[i]
+
+
+# ----------------------------------------------------------- basis conversion ---
+
+def _crt(residues, mods):
+    M = math.prod(mods)
+    x = 0
+    for r, m in zip(residues, mods):
+        Mi = M // m
+        x += r * Mi * pow(Mi, -1, m)
+    return x % M
+
+
+@pytest.mark.parametrize("shape", [(4096, 4, 5, 52), (4096, 6, 0, 48), (4096, 4, 4, 55)])
+def test_modup_crt_identity(shape):
+    o = Oracle(*shape)
+    N, k, kl = o.N, o.k, o.kl
+    rng = np.random.default_rng(9)
+    Q, B = o.primes[:k], o.primes[k:]
+    for level in (0, 1, k - 1):
+        qb = o.random_poly(rng, k + kl)
+        o.modup(qb, level)
+        Ql = math.prod(Q[:level + 1])
+        for cid in rng.choice(N, 24, replace=False):
+            xt = _crt([int(qb[i * N + cid]) for i in range(level + 1)], Q[:level + 1])
+            for j in range(kl):
+                out = int(qb[(k + j) * N + cid])
+                assert any((xt + u * Ql - out) % B[j] == 0 for u in range(level + 2)), (level, cid, j)
+
+
+def test_moddown_identity_and_deviation():
+    """ModDown(z) = floor(z / B) - e (mod q_i), one integer e in [0, kl] for all
+    i, where the reference's Barrett is exact (prec 55).  At prec 48 the same
+    holds except for limbs whose products hit the deviation window, which move
+    by a multiple of (2^nb - q_i) B^-1."""
+    for shape, exact in [((4096, 4, 4, 55), True), ((4096, 6, 0, 48), False)]:
+        o = Oracle(*shape)
+        N, k, kl = o.N, o.k, o.kl
+        Q, B = o.primes[:k], o.primes[k:]
+        Bp = math.prod(B)
+        rng = np.random.default_rng(13)
+        level = k - 1
+        qb = o.random_poly(rng, k + kl)
+        out = np.zeros(k * N, dtype=np.uint64)
+        o.moddown(out, qb, level)
+        mods = Q[:level + 1] + B
+        bad = 0
+        for cid in rng.choice(N, 64, replace=False):
+            z = _crt([int(qb[i * N + cid]) for i in range(level + 1)] + [int(qb[(k + j) * N + cid]) for j in range(kl)],
+                     mods)
+            fz = z // Bp
+            es = []
+            for i in range(level + 1):
+                d = (fz - int(out[i * N + cid])) % Q[i]
+                es.append(d)
+            ok = all(e <= kl for e in es) and len(set(es)) == 1
+            if exact:
+                assert ok, (cid, es)
+            elif not ok:
+                bad += 1
+                for i, e in enumerate(es):
+                    if e > kl:
+                        eps = (1 << Q[i].bit_length()) - Q[i]
+                        binv = pow(Bp, -1, Q[i])
+                        assert any((e - t - m * eps * binv) % Q[i] == 0 for t in range(kl + 1) for m in range(1, 4))
+        assert bad < 64
+
+
+def test_rescale_identity():
+    o = Oracle(4096, 5, 0, 48)
+    N, k = o.N, o.k
+    rng = np.random.default_rng(4)
+    a, b = o.random_poly(rng, k), o.random_poly(rng, k)
+    a0, b0 = a.copy(), b.copy()
+    level = k - 1
+    o.rescale(a, b, level)
+    ql = o.primes[level]
+    for x, x0 in ((a, a0), (b, b0)):
+        for i in range(level):
+            qi = o.primes[i]
+            for cid in (0, 1, 77, N - 1):
+                assert (int(x[i * N + cid]) * ql - (int(x0[i * N + cid]) - int(x0[level * N + cid]))) % qi == 0
+
+
+def test_automorphism_definition():
+    o = Oracle(4096, 2, 1, 55)
+    N = o.N
+    rng = np.random.default_rng(6)
+    x = o.random_poly(rng, 2)
+    for rot in (1, 3, 1000):
+        out = np.zeros_like(x)
+        o.automorphism(out, x, 2, rot)
+        g = pow(5, rot, 2 * N)
+        for rid in range(2):
+            p = o.primes[rid]
+            ref = [0] * N
+            for c in range(N):
+                e = c * g % (2 * N)
+                v = int(x[rid * N + c])
+                if e >= N:
+                    ref[e - N] = (-v) % p if v else p  # the reference writes p for -0
+                else:
+                    ref[e] = v
+            got = [int(v) for v in out[rid * N:(rid + 1) * N]]
+            assert got == ref
