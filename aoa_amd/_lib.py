@@ -37,6 +37,32 @@ SIGNATURES = {
                                         ctypes.c_int, c_vp, ctypes.c_size_t]),
 }
 
+
+
+class MulItem(ctypes.Structure):
+    _fields_ = [("c1", c_vp * 2), ("c2", c_vp * 2), ("out", c_vp * 2), ("level", ctypes.c_int)]
+
+
+class RotItem(ctypes.Structure):
+    _fields_ = [("inp", c_vp * 2), ("out", c_vp * 2), ("rtk_b", c_vp), ("rtk_a", c_vp),
+                ("rot_slots", ctypes.c_int), ("level", ctypes.c_int), ("accumulate", ctypes.c_int)]
+
+
+SIGNATURES.update({
+    "aoa_params_set_ntt_split": (ctypes.c_int, [c_vp, ctypes.c_int]),
+    "aoa_params_get_ntt_split": (ctypes.c_int, [c_vp]),
+    "aoa_autotune_ntt": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "aoa_batch_create": (ctypes.c_int, [ctypes.POINTER(c_vp)]),
+    "aoa_batch_add_mul": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(MulItem), ctypes.c_int]),
+    "aoa_batch_add_rotate": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(RotItem), ctypes.c_int]),
+    "aoa_batch_barrier": (ctypes.c_int, [c_vp]),
+    "aoa_batch_build": (ctypes.c_int, [c_vp]),
+    "aoa_batch_workspace_bytes": (ctypes.c_size_t, [c_vp]),
+    "aoa_batch_launch": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int]),
+    "aoa_batch_destroy": (None, [c_vp]),
+})
+
 _lib = None
 
 

@@ -7,7 +7,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, load
+from ._lib import MulItem, RotItem, check, load
 
 FORWARD, INVERSE = 0, 1
 ADD, SUB, MUL, NEGATE, MULADD = 0, 1, 2, 3, 5
@@ -108,8 +108,81 @@ class Params:
             self.h, _stream(stream), _ptr(c3[0]), _ptr(c3[1]), _ptr(c1[0]), _ptr(c1[1]), _ptr(c2[0]),
             _ptr(c2[1]), _ptr(evk[0]), _ptr(evk[1]), level, _ptr(w), nb))
 
+    def set_ntt_split(self, n1):
+        check(self.lib.aoa_params_set_ntt_split(self.h, n1))
+
+    def ntt_split(self):
+        return self.lib.aoa_params_get_ntt_split(self.h)
+
+    def autotune_ntt(self, nlimbs=0, reps=10, stream=None):
+        """Times every NTT split on the device, keeps the fastest; returns
+        {n1: microseconds}."""
+        n1 = (ctypes.c_int * 8)()
+        us = (ctypes.c_double * 8)()
+        n = self.lib.aoa_autotune_ntt(self.h, _stream(stream), nlimbs, reps, n1, us, 8)
+        if n < 0:
+            check(-n)
+        return {int(n1[i]): float(us[i]) for i in range(min(n, 8))}
+
     def rotate_left(self, out, c, rtk, rot_slots, level, ws, stream=None):
         w, nb = ws
         check(self.lib.aoa_rotate_left(self.h, _stream(stream), _ptr(out[0]), _ptr(out[1]), _ptr(c[0]),
                                        _ptr(c[1]), _ptr(rtk[0]), _ptr(rtk[1]), rot_slots, level, _ptr(w),
                                        nb))
+
+
+class Batch:
+    """aoa_batch handle: stage-fused evaluation of many ciphertexts, replayed
+    as a hipGraph.  Keeps references to every tensor it was given."""
+
+    def __init__(self):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.aoa_batch_create(ctypes.byref(h)))
+        self.h = h
+        self._keep = []
+
+    def add_mul(self, params, evk, items):
+        """items: list of (c1, c2, out, level) with c1/c2/out pairs of tensors."""
+        arr = (MulItem * len(items))()
+        for i, (c1, c2, out, level) in enumerate(items):
+            arr[i].c1[0], arr[i].c1[1] = c1[0].data_ptr(), c1[1].data_ptr()
+            arr[i].c2[0], arr[i].c2[1] = c2[0].data_ptr(), c2[1].data_ptr()
+            arr[i].out[0], arr[i].out[1] = out[0].data_ptr(), out[1].data_ptr()
+            arr[i].level = level
+            self._keep += [c1, c2, out]
+        self._keep += [evk, params]
+        check(self.lib.aoa_batch_add_mul(self.h, params.h, _ptr(evk[0]), _ptr(evk[1]), arr, len(items)))
+
+    def add_rotate(self, params, items):
+        """items: list of (inp, out, rtk, rot_slots, level, accumulate)."""
+        arr = (RotItem * len(items))()
+        for i, (inp, out, rtk, rot, level, acc) in enumerate(items):
+            arr[i].inp[0], arr[i].inp[1] = inp[0].data_ptr(), inp[1].data_ptr()
+            arr[i].out[0], arr[i].out[1] = out[0].data_ptr(), out[1].data_ptr()
+            arr[i].rtk_b, arr[i].rtk_a = rtk[0].data_ptr(), rtk[1].data_ptr()
+            arr[i].rot_slots, arr[i].level, arr[i].accumulate = rot, level, int(acc)
+            self._keep += [inp, out, rtk]
+        self._keep.append(params)
+        check(self.lib.aoa_batch_add_rotate(self.h, params.h, arr, len(items)))
+
+    def barrier(self):
+        check(self.lib.aoa_batch_barrier(self.h))
+
+    def build(self):
+        check(self.lib.aoa_batch_build(self.h))
+        return self.lib.aoa_batch_workspace_bytes(self.h)
+
+    def launch(self, stream=None, graph=True):
+        check(self.lib.aoa_batch_launch(self.h, _stream(stream), 1 if graph else 0))
+
+    def close(self):
+        if self.h:
+            self.lib.aoa_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,34 +1,42 @@
 // C-ABI entry points (include/aoa.h): argument checks, workspace carving and
 // the kernel sequences of the plain (one ciphertext per call) form.
-#include <hip/hip_runtime.h>
-
+#include <algorithm>
 #include <cstring>
 #include <string>
 
-#include "../../include/aoa.h"
-#include "kernels.hpp"
-#include "params.hpp"
+#include "internal.hpp"
 
 using namespace aoa;
 
-struct aoa_params {
-  HostParams host;
-  DevParams dev;
-  void* dmem = nullptr;
-};
-
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string& msg) {
+namespace aoa {
+
+int set_error(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
 
-static int check_launch(const char* what) {
+int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(AOA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  if (e != hipSuccess) return set_error(AOA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
   return AOA_OK;
 }
+
+uint64_t galois_of(int rot_slots, int N) {
+  const uint64_t M = 2 * (uint64_t)N;
+  uint64_t r = 1, s = 5 % M, e = (uint64_t)rot_slots;
+  while (e) {
+    if (e & 1) r = (r * s) % M;
+    s = (s * s) % M;
+    e >>= 1;
+  }
+  return r;
+}
+
+}  // namespace aoa
+
+static int fail(int code, const std::string& msg) { return set_error(code, msg); }
 
 extern "C" const char* aoa_last_error(void) { return g_err.c_str(); }
 
@@ -223,17 +231,6 @@ extern "C" int aoa_moddown(const aoa_params* h, void* stream, uint64_t* d_q_out,
   a.rescale = 0;
   launch_moddown(h->dev, a, (hipStream_t)stream);
   return check_launch("aoa_moddown");
-}
-
-static uint64_t galois_of(int rot_slots, int N) {
-  const uint64_t M = 2 * (uint64_t)N;
-  uint64_t r = 1, s = 5 % M, e = (uint64_t)rot_slots;
-  while (e) {
-    if (e & 1) r = (r * s) % M;
-    s = (s * s) % M;
-    e >>= 1;
-  }
-  return r;
 }
 
 extern "C" int aoa_automorphism(const aoa_params* h, void* stream, uint64_t* out, const uint64_t* in, int nres,

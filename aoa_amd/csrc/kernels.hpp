@@ -11,6 +11,7 @@ namespace aoa {
 enum EltOp { ELT_ADD = 0, ELT_SUB = 1, ELT_MUL = 2, ELT_NEG = 3, ELT_MULADD = 5 };
 
 int ntt_default_n1(int N);
+bool ntt_valid_split(int N, int N1);
 // Forward / inverse NTT of every limb in `m`.  N1 = 0 picks the default split.
 // With key0/key1 set, the inverse multiplies limb li of m.src[0] by limb
 // prime_of(li) of key y before transforming into m.base[y] (y = 0, 1).
@@ -52,5 +53,15 @@ void launch_rescale(const DevParams& P, uint64_t* c0, uint64_t* c1, int level, h
 // Galois automorphism X -> X^g (g odd) in the coefficient domain, gather form.
 void launch_automorphism(const DevParams& P, uint64_t* out, const uint64_t* in, int nlimbs,
                          uint64_t galois, hipStream_t st);
+uint64_t galois_inverse(uint64_t galois, int N);
+
+// Descriptor-driven (stage-fused) forms used by batches: one launch covers
+// every descriptor of the table (device memory).
+int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st);
+int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st);
+void launch_tensor_desc(const DevParams& P, const TensorDesc* d, int ndesc, hipStream_t st);
+void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int max_level, hipStream_t st);
+void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, hipStream_t st);
+void launch_automorphism_desc(const DevParams& P, const AutoDesc* d, int ndesc, hipStream_t st);
 
 }  // namespace aoa

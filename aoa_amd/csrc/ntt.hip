@@ -9,11 +9,14 @@
 // The N-point transform is split as N = N1 x N2.  The column pass runs the
 // first log2(N1) stages on S = N1 elements with stride N2; the row pass runs
 // the remaining log2(N2) stages on contiguous rows of S = N2.  Both passes are
-// in place, so no ping-pong buffer is needed.  A 256-thread workgroup
-// transforms a 4096-element tile: TS = S/16 threads per sub-transform, 16
-// values per thread in registers, four stages per register round and one LDS
-// exchange between rounds.  Twiddles psi^brev(i) come from a per-prime table
-// with Shoup quotients.
+// in place (or read a separate source in the first pass), so no ping-pong
+// buffer is needed.  A 256-thread workgroup transforms a 4096-element tile:
+// TS = S/16 threads per sub-transform, 16 values per thread in registers, four
+// stages per register round and one LDS exchange between rounds.  Twiddles
+// psi^brev(i) come from a per-prime table with Shoup quotients.
+//
+// Every pass exists in two launch forms: limbs described by kernel arguments
+// (plain calls) and limbs listed in a device descriptor table (batches).
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -24,6 +27,8 @@ struct Geo {
   static constexpr int LOGS = (S == 64) ? 6 : (S == 128 ? 7 : 8);
   static constexpr int TS = S / 16;   // threads per sub-transform
   static constexpr int C = 256 / TS;  // sub-transforms per tile
+  static constexpr int COL_LDS = S * (C + 1);
+  static constexpr int ROW_LDS = C * (S + S / 16);
 };
 
 __device__ __forceinline__ void ct_bfly(uint64_t& a, uint64_t& b, const tw2 t, uint64_t p) {
@@ -41,24 +46,23 @@ __device__ __forceinline__ void gs_bfly(uint64_t& a, uint64_t& b, const tw2 t, u
   b = mul_shoup(u + (p - v), t.w, t.q, p);
 }
 
-// ---------------------------------------------------------------- forward --
+__device__ __forceinline__ int swz(int s) { return s + (s >> 4); }
+
+// ------------------------------------------------------------ pass bodies --
+// src/dst point at limb 0 of the limb; `pr` is its prime index; `tile` the
+// tile within the limb.
 
 // Column pass: global stages 0..log2(S)-1 on columns of an S x N2 matrix.
 template <int S>
-__global__ __launch_bounds__(256) void k_ntt_fwd_col(DevParams P, LimbMap m, int N2) {
+__device__ __forceinline__ void fwd_col_body(const DevParams& P, const uint64_t* src, uint64_t* dst, int pr,
+                                             int tile, int N2, uint64_t* lds) {
   using G = Geo<S>;
   constexpr int C = G::C, TS = G::TS, LOGS = G::LOGS, LDW = C + 1;
-  __shared__ uint64_t lds[S * LDW];
-  const int tiles = N2 / C;
-  const int li = blockIdx.x / tiles;
-  const int col0 = (blockIdx.x % tiles) * C;
-  const int y = blockIdx.y;
-  const int pr = limb_prime(m, P.k, li);
+  const int col0 = tile * C;
   const uint64_t p = P.primes[pr].p;
-  const size_t N = (size_t)P.N;
-  const uint64_t* src = m.src[y] + (size_t)li * N + col0;
-  uint64_t* dst = m.base[y] + (size_t)li * N + col0;
-  const uint64_t* T = P.tw_fwd + (size_t)pr * N * 2;
+  const uint64_t* T = P.tw_fwd + (size_t)pr * P.N * 2;
+  src += col0;
+  dst += col0;
   const int tid = threadIdx.x;
   const int c = tid % C;
   const int sl = tid / C;
@@ -95,29 +99,21 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(DevParams P, LimbMap m, int
   for (int k = 0; k < 16; ++k) dst[(size_t)(16 * sh + k) * N2 + c] = x[k];
 }
 
-__device__ __forceinline__ int swz(int s) { return s + (s >> 4); }
-
 // Row pass: global stages g0..g0+log2(S)-1 on contiguous rows of length S.
 template <int S>
-__global__ __launch_bounds__(256) void k_ntt_fwd_row(DevParams P, LimbMap m, int g0) {
+__device__ __forceinline__ void fwd_row_body(const DevParams& P, const uint64_t* src, uint64_t* dst, int pr,
+                                             int tile, int g0, uint64_t* lds) {
   using G = Geo<S>;
   constexpr int C = G::C, TS = G::TS, LOGS = G::LOGS, PS = S + S / 16;
-  __shared__ uint64_t lds[C * PS];
-  const size_t N = (size_t)P.N;
-  const int N1 = P.N / S;
-  const int tiles = N1 / C;
-  const int li = blockIdx.x / tiles;
-  const int row0 = (blockIdx.x % tiles) * C;
-  const int y = blockIdx.y;
-  const int pr = limb_prime(m, P.k, li);
+  const int row0 = tile * C;
   const uint64_t p = P.primes[pr].p;
-  const uint64_t* T = P.tw_fwd + (size_t)pr * N * 2;
+  const uint64_t* T = P.tw_fwd + (size_t)pr * P.N * 2;
   const int tid = threadIdx.x;
   const int rl = tid / TS;
   const int sl = tid % TS;
   const int row = row0 + rl;
-  const uint64_t* src = m.src[y] + (size_t)li * N + (size_t)row * S;
-  uint64_t* dst = m.base[y] + (size_t)li * N + (size_t)row * S;
+  src += (size_t)row * S;
+  dst += (size_t)row * S;
   uint64_t* L = lds + rl * PS;
   uint64_t x[16];
 #pragma unroll
@@ -157,39 +153,30 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(DevParams P, LimbMap m, int
   for (int k = 0; k < 16; ++k) dst[sl + TS * k] = L[swz(sl + TS * k)];
 }
 
-// ---------------------------------------------------------------- inverse --
-
-// Row pass of the inverse: global stages g0+log2(S)-1 .. g0.  With KEYMUL the
-// input limb is first multiplied by limb `prime` of key poly blockIdx.y
-// (key switching: INTT(x * key_b), INTT(x * key_a) from one input).
-template <int S, bool KEYMUL>
-__global__ __launch_bounds__(256) void k_ntt_inv_row(DevParams P, LimbMap m, int g0,
-                                                      const uint64_t* key0, const uint64_t* key1) {
+// Row pass of the inverse: global stages g0+log2(S)-1 .. g0.  With a key, the
+// input limb is first multiplied by the key limb (key switching:
+// INTT(x * key_b), INTT(x * key_a) from one input).
+template <int S>
+__device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t* src, uint64_t* dst,
+                                             const uint64_t* key, int pr, int tile, int g0, uint64_t* lds) {
   using G = Geo<S>;
   constexpr int C = G::C, TS = G::TS, LOGS = G::LOGS, PS = S + S / 16;
-  __shared__ uint64_t lds[C * PS];
-  const size_t N = (size_t)P.N;
-  const int N1 = P.N / S;
-  const int tiles = N1 / C;
-  const int li = blockIdx.x / tiles;
-  const int row0 = (blockIdx.x % tiles) * C;
-  const int y = blockIdx.y;
-  const int pr = limb_prime(m, P.k, li);
+  const int row0 = tile * C;
   const DevPrime dp = P.primes[pr];
   const uint64_t p = dp.p;
-  const uint64_t* T = P.tw_inv + (size_t)pr * N * 2;
+  const uint64_t* T = P.tw_inv + (size_t)pr * P.N * 2;
   const int tid = threadIdx.x;
   const int rl = tid / TS;
   const int sl = tid % TS;
   const int row = row0 + rl;
-  const uint64_t* src = m.src[KEYMUL ? 0 : y] + (size_t)li * N + (size_t)row * S;
-  uint64_t* dst = m.base[y] + (size_t)li * N + (size_t)row * S;
+  src += (size_t)row * S;
+  dst += (size_t)row * S;
   uint64_t* L = lds + rl * PS;
   uint64_t x[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) x[k] = src[sl + TS * k];
-  if (KEYMUL) {
-    const uint64_t* kp = (y == 0 ? key0 : key1) + (size_t)pr * N + (size_t)row * S;
+  if (key) {
+    const uint64_t* kp = key + (size_t)row * S;
 #pragma unroll
     for (int k = 0; k < 16; ++k) x[k] = mul_barrett(x[k], kp[sl + TS * k], p, dp.mu, (int)dp.nb);
   }
@@ -233,21 +220,16 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row(DevParams P, LimbMap m, int
 // Column pass of the inverse: global stages log2(S)-1 .. 0, N^-1 folded into
 // stage 0.
 template <int S>
-__global__ __launch_bounds__(256) void k_ntt_inv_col(DevParams P, LimbMap m, int N2) {
+__device__ __forceinline__ void inv_col_body(const DevParams& P, const uint64_t* src, uint64_t* dst, int pr,
+                                             int tile, int N2, uint64_t* lds) {
   using G = Geo<S>;
   constexpr int C = G::C, TS = G::TS, LOGS = G::LOGS, LDW = C + 1;
-  __shared__ uint64_t lds[S * LDW];
-  const int tiles = N2 / C;
-  const int li = blockIdx.x / tiles;
-  const int col0 = (blockIdx.x % tiles) * C;
-  const int y = blockIdx.y;
-  const int pr = limb_prime(m, P.k, li);
+  const int col0 = tile * C;
   const DevPrime dp = P.primes[pr];
   const uint64_t p = dp.p;
-  const size_t N = (size_t)P.N;
-  const uint64_t* src = m.src[y] + (size_t)li * N + col0;
-  uint64_t* dst = m.base[y] + (size_t)li * N + col0;
-  const uint64_t* T = P.tw_inv + (size_t)pr * N * 2;
+  const uint64_t* T = P.tw_inv + (size_t)pr * P.N * 2;
+  src += col0;
+  dst += col0;
   const int tid = threadIdx.x;
   const int c = tid % C;
   const int sh = tid / C;
@@ -291,10 +273,91 @@ __global__ __launch_bounds__(256) void k_ntt_inv_col(DevParams P, LimbMap m, int
   for (int k = 0; k < 16; ++k) dst[(size_t)(sl + TS * k) * N2 + c] = x[k];
 }
 
+// ------------------------------------------------- argument-driven kernels --
+
+template <int S>
+__global__ __launch_bounds__(256) void k_ntt_fwd_col(DevParams P, LimbMap m, int N2) {
+  __shared__ uint64_t lds[Geo<S>::COL_LDS];
+  const int tiles = N2 / Geo<S>::C;
+  const int li = blockIdx.x / tiles;
+  const int y = blockIdx.y;
+  const size_t off = (size_t)li * P.N;
+  fwd_col_body<S>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, N2, lds);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_ntt_fwd_row(DevParams P, LimbMap m, int g0) {
+  __shared__ uint64_t lds[Geo<S>::ROW_LDS];
+  const int tiles = (P.N / S) / Geo<S>::C;
+  const int li = blockIdx.x / tiles;
+  const int y = blockIdx.y;
+  const size_t off = (size_t)li * P.N;
+  fwd_row_body<S>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, g0, lds);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_ntt_inv_row(DevParams P, LimbMap m, int g0, const uint64_t* key0,
+                                                      const uint64_t* key1) {
+  __shared__ uint64_t lds[Geo<S>::ROW_LDS];
+  const int tiles = (P.N / S) / Geo<S>::C;
+  const int li = blockIdx.x / tiles;
+  const int y = blockIdx.y;
+  const size_t off = (size_t)li * P.N;
+  const int pr = limb_prime(m, P.k, li);
+  const uint64_t* key = key0 ? (y == 0 ? key0 : key1) + (size_t)pr * P.N : nullptr;
+  const uint64_t* src = (key0 ? m.src[0] : m.src[y]) + off;
+  inv_row_body<S>(P, src, m.base[y] + off, key, pr, blockIdx.x % tiles, g0, lds);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_ntt_inv_col(DevParams P, LimbMap m, int N2) {
+  __shared__ uint64_t lds[Geo<S>::COL_LDS];
+  const int tiles = N2 / Geo<S>::C;
+  const int li = blockIdx.x / tiles;
+  const int y = blockIdx.y;
+  const size_t off = (size_t)li * P.N;
+  inv_col_body<S>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, N2, lds);
+}
+
+// ----------------------------------------------- descriptor-driven kernels --
+
+template <int S>
+__global__ __launch_bounds__(256) void k_ntt_fwd_col_d(DevParams P, const NttDesc* d, int N2) {
+  __shared__ uint64_t lds[Geo<S>::COL_LDS];
+  const int tiles = N2 / Geo<S>::C;
+  const NttDesc e = d[blockIdx.x / tiles];
+  fwd_col_body<S>(P, e.src, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_ntt_fwd_row_d(DevParams P, const NttDesc* d, int g0) {
+  __shared__ uint64_t lds[Geo<S>::ROW_LDS];
+  const int tiles = (P.N / S) / Geo<S>::C;
+  const NttDesc e = d[blockIdx.x / tiles];
+  fwd_row_body<S>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, g0, lds);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_ntt_inv_row_d(DevParams P, const NttDesc* d, int g0) {
+  __shared__ uint64_t lds[Geo<S>::ROW_LDS];
+  const int tiles = (P.N / S) / Geo<S>::C;
+  const NttDesc e = d[blockIdx.x / tiles];
+  inv_row_body<S>(P, e.src, e.dst, e.key, e.prime, blockIdx.x % tiles, g0, lds);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_ntt_inv_col_d(DevParams P, const NttDesc* d, int N2) {
+  __shared__ uint64_t lds[Geo<S>::COL_LDS];
+  const int tiles = N2 / Geo<S>::C;
+  const NttDesc e = d[blockIdx.x / tiles];
+  inv_col_body<S>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
+}
+
 // ---------------------------------------------------------------- launch ---
 
-static bool valid_split(int N, int N1) {
+bool ntt_valid_split(int N, int N1) {
   if (N1 != 64 && N1 != 128 && N1 != 256) return false;
+  if (N % N1) return false;
   int N2 = N / N1;
   return N2 == 64 || N2 == 128 || N2 == 256;
 }
@@ -310,33 +373,10 @@ int ntt_default_n1(int N) {
   }
 }
 
-template <int S>
-static void launch_fwd_col(const DevParams& P, const LimbMap& m, int N2, hipStream_t st) {
-  const int tiles = N2 / Geo<S>::C;
-  dim3 grid(tiles * m.nlimbs, m.npolys);
-  k_ntt_fwd_col<S><<<grid, 256, 0, st>>>(P, m, N2);
-}
-template <int S>
-static void launch_fwd_row(const DevParams& P, const LimbMap& m, int g0, hipStream_t st) {
-  const int tiles = (P.N / S) / Geo<S>::C;
-  dim3 grid(tiles * m.nlimbs, m.npolys);
-  k_ntt_fwd_row<S><<<grid, 256, 0, st>>>(P, m, g0);
-}
-template <int S>
-static void launch_inv_row(const DevParams& P, const LimbMap& m, int g0, const uint64_t* k0,
-                           const uint64_t* k1, hipStream_t st) {
-  const int tiles = (P.N / S) / Geo<S>::C;
-  dim3 grid(tiles * m.nlimbs, m.npolys);
-  if (k0)
-    k_ntt_inv_row<S, true><<<grid, 256, 0, st>>>(P, m, g0, k0, k1);
-  else
-    k_ntt_inv_row<S, false><<<grid, 256, 0, st>>>(P, m, g0, nullptr, nullptr);
-}
-template <int S>
-static void launch_inv_col(const DevParams& P, const LimbMap& m, int N2, hipStream_t st) {
-  const int tiles = N2 / Geo<S>::C;
-  dim3 grid(tiles * m.nlimbs, m.npolys);
-  k_ntt_inv_col<S><<<grid, 256, 0, st>>>(P, m, N2);
+static int log2i(int n) {
+  int g = 0;
+  while ((1 << g) < n) ++g;
+  return g;
 }
 
 #define AOA_DISPATCH_S(S_, CALL) \
@@ -347,42 +387,97 @@ static void launch_inv_col(const DevParams& P, const LimbMap& m, int N2, hipStre
     default: return -1;          \
   }
 
+static int pick_n1(const DevParams& P, int N1) {
+  if (N1 == 0) N1 = P.ntt_n1 ? P.ntt_n1 : ntt_default_n1(P.N);
+  return ntt_valid_split(P.N, N1) ? N1 : 0;
+}
+
 int ntt_forward(const DevParams& P, LimbMap m, int N1, hipStream_t st) {
-  if (N1 == 0) N1 = ntt_default_n1(P.N);
-  if (!valid_split(P.N, N1)) return -1;
-  const int N2 = P.N / N1;
-  int g0 = 0;
-  while ((1 << g0) < N1) ++g0;
+  N1 = pick_n1(P, N1);
+  if (!N1) return -1;
+  const int N2 = P.N / N1, g0 = log2i(N1);
   for (int y = 0; y < m.npolys; ++y)
     if (!m.src[y]) m.src[y] = m.base[y];
-#define CALLC(S) launch_fwd_col<S>(P, m, N2, st)
-  AOA_DISPATCH_S(N1, CALLC)
+  {
+    const int tiles = N2 / (4096 / N1);
+    dim3 grid(tiles * m.nlimbs, m.npolys);
+#define CALLC(S) k_ntt_fwd_col<S><<<grid, 256, 0, st>>>(P, m, N2)
+    AOA_DISPATCH_S(N1, CALLC)
 #undef CALLC
+  }
   LimbMap m2 = m;
   for (int y = 0; y < m2.npolys; ++y) m2.src[y] = m2.base[y];
-#define CALLR(S) launch_fwd_row<S>(P, m2, g0, st)
-  AOA_DISPATCH_S(N2, CALLR)
+  {
+    const int tiles = N1 / (4096 / N2);
+    dim3 grid(tiles * m.nlimbs, m.npolys);
+#define CALLR(S) k_ntt_fwd_row<S><<<grid, 256, 0, st>>>(P, m2, g0)
+    AOA_DISPATCH_S(N2, CALLR)
 #undef CALLR
+  }
   return 0;
 }
 
 int ntt_inverse(const DevParams& P, LimbMap m, int N1, const uint64_t* key0, const uint64_t* key1,
                 hipStream_t st) {
-  if (N1 == 0) N1 = ntt_default_n1(P.N);
-  if (!valid_split(P.N, N1)) return -1;
-  const int N2 = P.N / N1;
-  int g0 = 0;
-  while ((1 << g0) < N1) ++g0;
+  N1 = pick_n1(P, N1);
+  if (!N1) return -1;
+  const int N2 = P.N / N1, g0 = log2i(N1);
   for (int y = 0; y < m.npolys; ++y)
     if (!m.src[y]) m.src[y] = m.base[y];
-#define CALLR(S) launch_inv_row<S>(P, m, g0, key0, key1, st)
-  AOA_DISPATCH_S(N2, CALLR)
+  {
+    const int tiles = N1 / (4096 / N2);
+    dim3 grid(tiles * m.nlimbs, m.npolys);
+#define CALLR(S) k_ntt_inv_row<S><<<grid, 256, 0, st>>>(P, m, g0, key0, key1)
+    AOA_DISPATCH_S(N2, CALLR)
 #undef CALLR
+  }
   LimbMap m2 = m;
   for (int y = 0; y < m2.npolys; ++y) m2.src[y] = m2.base[y];
-#define CALLC(S) launch_inv_col<S>(P, m2, N2, st)
-  AOA_DISPATCH_S(N1, CALLC)
+  {
+    const int tiles = N2 / (4096 / N1);
+    dim3 grid(tiles * m.nlimbs, m.npolys);
+#define CALLC(S) k_ntt_inv_col<S><<<grid, 256, 0, st>>>(P, m2, N2)
+    AOA_DISPATCH_S(N1, CALLC)
 #undef CALLC
+  }
+  return 0;
+}
+
+int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st) {
+  N1 = pick_n1(P, N1);
+  if (!N1 || ndesc <= 0) return -1;
+  const int N2 = P.N / N1, g0 = log2i(N1);
+  {
+    const int tiles = N2 / (4096 / N1);
+#define CALLC(S) k_ntt_fwd_col_d<S><<<dim3(tiles * ndesc), 256, 0, st>>>(P, d, N2)
+    AOA_DISPATCH_S(N1, CALLC)
+#undef CALLC
+  }
+  {
+    const int tiles = N1 / (4096 / N2);
+#define CALLR(S) k_ntt_fwd_row_d<S><<<dim3(tiles * ndesc), 256, 0, st>>>(P, d, g0)
+    AOA_DISPATCH_S(N2, CALLR)
+#undef CALLR
+  }
+  return 0;
+}
+
+int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st) {
+  N1 = pick_n1(P, N1);
+  if (!N1 || ndesc <= 0) return -1;
+  const int N2 = P.N / N1, g0 = log2i(N1);
+  {
+    const int tiles = N1 / (4096 / N2);
+#define CALLR(S) k_ntt_inv_row_d<S><<<dim3(tiles * ndesc), 256, 0, st>>>(P, d, g0)
+    AOA_DISPATCH_S(N2, CALLR)
+#undef CALLR
+  }
+  {
+    const int tiles = N2 / (4096 / N1);
+#define CALLC(S) k_ntt_inv_col_d<S><<<dim3(tiles * ndesc), 256, 0, st>>>(P, d, N2)
+    AOA_DISPATCH_S(N1, CALLC)
+#undef CALLC
+  }
   return 0;
 }
 

@@ -43,6 +43,8 @@ struct DevPrime {
 // Passed to kernels by value.
 struct DevParams {
   int N, logN, k, K;
+  int ntt_n1;  // NTT split N = N1 x (N/N1); 0 = built-in default (autotuner sets it)
+  int pad;
   const DevPrime* primes;        // [k+K]: Q primes then B primes
   const uint64_t* tw_fwd;        // [k+K][N][2]: psi^brev(i) and Shoup quotient
   const uint64_t* tw_inv;        // [k+K][N][2]: psi^-brev(i) and Shoup quotient

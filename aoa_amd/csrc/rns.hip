@@ -48,14 +48,11 @@ void launch_eltwise(const DevParams& P, int op, uint64_t* out, const uint64_t* a
   k_eltwise<<<dim3((unsigned)(total / 512)), 256, 0, st>>>(P, op, out, a, b, c, qlimbs, prime0);
 }
 
-// d0 = a0 b0, d2 = a1 b1, d1 = (a0 + a1)(b0 + b1) - d0 - d2 (= a0 b1 + a1 b0).
-__global__ __launch_bounds__(256) void k_tensor(DevParams P, uint64_t* d0, uint64_t* d1, uint64_t* d2,
-                                                const uint64_t* a0, const uint64_t* a1,
-                                                const uint64_t* b0, const uint64_t* b1) {
-  const size_t e0 = (size_t)blockIdx.x * 512;
-  const int li = (int)(e0 >> P.logN);
-  const DevPrime d = P.primes[li];
-  const size_t e = e0 + (size_t)threadIdx.x * 2;
+// d0 = a0 b0, d2 = a1 b1, d1 = (a0 + a1)(b0 + b1) - d0 - d2 (= a0 b1 + a1 b0),
+// two elements per thread of one limb (pointers at element e of the limb).
+__device__ __forceinline__ void tensor_body(const DevPrime& d, uint64_t* d0, uint64_t* d1, uint64_t* d2,
+                                            const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
+                                            const uint64_t* b1, size_t e) {
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const size_t i = e + t;
@@ -69,10 +66,29 @@ __global__ __launch_bounds__(256) void k_tensor(DevParams P, uint64_t* d0, uint6
   }
 }
 
+__global__ __launch_bounds__(256) void k_tensor(DevParams P, uint64_t* d0, uint64_t* d1, uint64_t* d2,
+                                                const uint64_t* a0, const uint64_t* a1,
+                                                const uint64_t* b0, const uint64_t* b1) {
+  const size_t e0 = (size_t)blockIdx.x * 512;
+  const int li = (int)(e0 >> P.logN);
+  tensor_body(P.primes[li], d0, d1, d2, a0, a1, b0, b1, e0 + (size_t)threadIdx.x * 2);
+}
+
 void launch_tensor(const DevParams& P, uint64_t* d0, uint64_t* d1, uint64_t* d2, const uint64_t* a0,
                    const uint64_t* a1, const uint64_t* b0, const uint64_t* b1, int L, hipStream_t st) {
   const size_t total = (size_t)L * P.N;
   k_tensor<<<dim3((unsigned)(total / 512)), 256, 0, st>>>(P, d0, d1, d2, a0, a1, b0, b1);
+}
+
+__global__ __launch_bounds__(256) void k_tensor_d(DevParams P, const TensorDesc* dd) {
+  const int per = P.N / 512;
+  const TensorDesc t = dd[blockIdx.x / per];
+  const size_t e = (size_t)(blockIdx.x % per) * 512 + (size_t)threadIdx.x * 2;
+  tensor_body(P.primes[t.prime], t.d0, t.d1, t.d2, t.a0, t.a1, t.b0, t.b1, e);
+}
+
+void launch_tensor_desc(const DevParams& P, const TensorDesc* d, int ndesc, hipStream_t st) {
+  k_tensor_d<<<dim3((unsigned)(ndesc * (P.N / 512))), 256, 0, st>>>(P, d);
 }
 
 // Basis conversion workgroups: 64 consecutive coefficients x 4 limb groups
@@ -86,14 +102,14 @@ constexpr int BC_GROUPS = 4;
 // mod b_j (aoa-ntt/src/cuda/manager.cu:180-223).  The per-i values are shared
 // through LDS; the sum over i is accumulated exactly in 128 bits and reduced
 // once, which equals the reference's term-by-term modular sum.
-__global__ __launch_bounds__(256) void k_modup(DevParams P, const uint64_t* x, uint64_t* outB, int level) {
-  extern __shared__ uint64_t tl[];  // [L][64]
+__device__ __forceinline__ void modup_body(const DevParams& P, const uint64_t* x, uint64_t* outB, int level,
+                                           int blk, uint64_t* tl) {
   const int L = level + 1;
   const size_t N = (size_t)P.N;
   const int c = threadIdx.x % BC_COEF;
   // wave-uniform limb group: keeps limb indices and constant loads scalar
   const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / BC_COEF));
-  const size_t cid = (size_t)blockIdx.x * BC_COEF + c;
+  const size_t cid = (size_t)blk * BC_COEF + c;
   const tw2* iq = reinterpret_cast<const tw2*>(P.inv_qhat) + (size_t)level * P.k;
   for (int i = g; i < L; i += BC_GROUPS) {
     const uint64_t p = P.primes[i].p;
@@ -118,9 +134,26 @@ __global__ __launch_bounds__(256) void k_modup(DevParams P, const uint64_t* x, u
   }
 }
 
+__global__ __launch_bounds__(256) void k_modup(DevParams P, const uint64_t* x, uint64_t* outB, int level) {
+  extern __shared__ uint64_t tl[];  // [L][64]
+  modup_body(P, x, outB, level, blockIdx.x, tl);
+}
+
 void launch_modup(const DevParams& P, const uint64_t* x, uint64_t* outB, int level, hipStream_t st) {
   const size_t lds = (size_t)(level + 1) * BC_COEF * sizeof(uint64_t);
   k_modup<<<dim3(P.N / BC_COEF), 256, lds, st>>>(P, x, outB, level);
+}
+
+__global__ __launch_bounds__(256) void k_modup_d(DevParams P, const ModUpDesc* dd) {
+  extern __shared__ uint64_t tl[];
+  const int per = P.N / BC_COEF;
+  const ModUpDesc u = dd[blockIdx.x / per];
+  modup_body(P, u.x, u.outB, u.level, blockIdx.x % per, tl);
+}
+
+void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int max_level, hipStream_t st) {
+  const size_t lds = (size_t)(max_level + 1) * BC_COEF * sizeof(uint64_t);
+  k_modup_d<<<dim3((unsigned)(ndesc * (P.N / BC_COEF))), 256, lds, st>>>(P, d);
 }
 
 // --------------------------------------------------------------- ModDown ----
@@ -139,19 +172,15 @@ __device__ __forceinline__ uint64_t moddown_term(uint64_t a, const tw2 w, const 
   return ref_mulmod_shoup(a, w.w, w.q, dq.p, dq.mu, (int)dq.nb, dq.quirk_lo, dq.quirk_thr);
 }
 
-__global__ __launch_bounds__(256) void k_moddown(DevParams P, ModDownArgs A) {
-  extern __shared__ uint64_t aux[];  // [K][64] then [4][64] partial sums
+__device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t* inQ, const uint64_t* inB,
+                                             const uint64_t* add, uint64_t* out, int level, int rescale,
+                                             int blk, uint64_t* aux) {
   const size_t N = (size_t)P.N;
   const int c = threadIdx.x % BC_COEF;
   // wave-uniform limb group: keeps limb indices and constant loads scalar
   const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / BC_COEF));
-  const int y = blockIdx.y;
-  const size_t cid = (size_t)blockIdx.x * BC_COEF + c;
-  const int K = P.K, k = P.k, level = A.level;
-  const uint64_t* inB = A.inB[y];
-  const uint64_t* inQ = A.inQ[y];
-  const uint64_t* add = A.add[y];
-  uint64_t* out = A.out[y];
+  const size_t cid = (size_t)blk * BC_COEF + c;
+  const int K = P.K, k = P.k;
   const tw2* ib = reinterpret_cast<const tw2*>(P.inv_bhat);
   for (int j = g; j < K; j += BC_GROUPS) {
     const uint64_t bp = P.primes[k + j].p;
@@ -175,7 +204,7 @@ __global__ __launch_bounds__(256) void k_moddown(DevParams P, ModDownArgs A) {
     if (add) v = add_mod(v, add[(size_t)i * N + cid], p);
     return v;
   };
-  if (!A.rescale) {
+  if (!rescale) {
     for (int i = g; i <= level; i += BC_GROUPS) out[(size_t)i * N + cid] = finish(i, conv(i, 0, K));
     return;
   }
@@ -200,9 +229,27 @@ __global__ __launch_bounds__(256) void k_moddown(DevParams P, ModDownArgs A) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_moddown(DevParams P, ModDownArgs A) {
+  extern __shared__ uint64_t aux[];  // [K][64] then [4][64] partial sums
+  const int y = blockIdx.y;
+  moddown_body(P, A.inQ[y], A.inB[y], A.add[y], A.out[y], A.level, A.rescale, blockIdx.x, aux);
+}
+
 void launch_moddown(const DevParams& P, const ModDownArgs& a, hipStream_t st) {
   const size_t lds = (size_t)(P.K + BC_GROUPS) * BC_COEF * sizeof(uint64_t);
   k_moddown<<<dim3(P.N / BC_COEF, a.npolys), 256, lds, st>>>(P, a);
+}
+
+__global__ __launch_bounds__(256) void k_moddown_d(DevParams P, const ModDownDesc* dd) {
+  extern __shared__ uint64_t aux[];
+  const int per = P.N / BC_COEF;
+  const ModDownDesc m = dd[blockIdx.x / per];
+  moddown_body(P, m.inQ, m.inB, m.add, m.out, m.level, m.rescale, blockIdx.x % per, aux);
+}
+
+void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, hipStream_t st) {
+  const size_t lds = (size_t)(P.K + BC_GROUPS) * BC_COEF * sizeof(uint64_t);
+  k_moddown_d<<<dim3((unsigned)(ndesc * (P.N / BC_COEF))), 256, lds, st>>>(P, d);
 }
 
 // --------------------------------------------------------------- rescale ----
@@ -240,18 +287,39 @@ __global__ __launch_bounds__(256) void k_automorphism(DevParams P, uint64_t* out
   out[e] = v;
 }
 
-void launch_automorphism(const DevParams& P, uint64_t* out, const uint64_t* in, int nlimbs,
-                         uint64_t galois, hipStream_t st) {
+uint64_t galois_inverse(uint64_t galois, int N) {
   // g^-1 mod 2N: the group (Z/2NZ)* has exponent N/2, so g^-1 = g^(N/2 - 1).
-  const uint64_t M = 2 * (uint64_t)P.N;
-  uint64_t r = 1, s = galois % M, e = (uint64_t)P.N / 2 - 1;
+  const uint64_t M = 2 * (uint64_t)N;
+  uint64_t r = 1, s = galois % M, e = (uint64_t)N / 2 - 1;
   while (e) {
     if (e & 1) r = (r * s) % M;
     s = (s * s) % M;
     e >>= 1;
   }
+  return r;
+}
+
+void launch_automorphism(const DevParams& P, uint64_t* out, const uint64_t* in, int nlimbs,
+                         uint64_t galois, hipStream_t st) {
   const size_t total = (size_t)nlimbs * P.N;
-  k_automorphism<<<dim3((unsigned)(total / 256)), 256, 0, st>>>(P, out, in, r);
+  k_automorphism<<<dim3((unsigned)(total / 256)), 256, 0, st>>>(P, out, in, galois_inverse(galois, P.N));
+}
+
+// Descriptor form, one entry per limb: out = sigma(in) (+ addsrc when set).
+__global__ __launch_bounds__(256) void k_automorphism_d(DevParams P, const AutoDesc* dd) {
+  const int per = P.N / 256;
+  const AutoDesc a = dd[blockIdx.x / per];
+  const uint64_t N = (uint64_t)P.N;
+  const uint64_t j = (uint64_t)(blockIdx.x % per) * 256 + threadIdx.x;
+  const uint64_t p = P.primes[a.prime].p;
+  const uint64_t t = (j * a.ginv) & (2 * N - 1);
+  uint64_t v = t < N ? a.in[t] : neg_mod(a.in[t - N], p);
+  if (a.addsrc) v = add_mod(v, a.addsrc[j], p);
+  a.out[j] = v;
+}
+
+void launch_automorphism_desc(const DevParams& P, const AutoDesc* d, int ndesc, hipStream_t st) {
+  k_automorphism_d<<<dim3((unsigned)(ndesc * (P.N / 256))), 256, 0, st>>>(P, d);
 }
 
 }  // namespace aoa

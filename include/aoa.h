@@ -133,6 +133,65 @@ int aoa_rotate_left(const aoa_params* params, void* stream, uint64_t* out0, uint
                     const uint64_t* c0, const uint64_t* c1, const uint64_t* rtk_b, const uint64_t* rtk_a,
                     int rot_slots, int level, void* workspace, size_t ws_bytes);
 
+/* ---------------------------------------------------------------- tuning --
+ * NTT split N = N1 x (N/N1) of a parameter set (N1 in {64, 128, 256}, N/N1 in
+ * the same set).  No counterpart in the reference, whose split is fixed per N
+ * at compile time (aoa-ntt/include/newckks/cuda/htrans/ntt.h:24-44; every
+ * split gives the same residues).  Set before issuing work with the handle. */
+int aoa_params_set_ntt_split(aoa_params* params, int n1);
+int aoa_params_get_ntt_split(const aoa_params* params);
+
+/* Autotuner: times forward+inverse NTTs of `nlimbs` limbs for every valid
+ * split (reps repetitions each, on the given stream, allocating a scratch
+ * buffer), stores the fastest split in the handle and writes up to `max`
+ * (n1, microseconds) pairs to out_n1/out_us.  Returns the number of
+ * candidates, or a negative status. */
+int aoa_autotune_ntt(aoa_params* params, void* stream, int nlimbs, int reps, int* out_n1, double* out_us, int max);
+
+/* ---------------------------------------------------------------- batches --
+ * Many ciphertexts (mixed levels and parameter sets) evaluated with one
+ * stage-fused launch per pipeline stage and per parameter set; the sequence
+ * is captured as a hipGraph at the first launch and replayed afterwards.
+ * No counterpart in the reference, which evaluates one ciphertext per call
+ * (aoa-ntt/src/ckks/ckks.cpp:273-358, :982-1009). */
+typedef struct aoa_batch aoa_batch;
+
+/* One HomMul+relin+rescale: inputs in the NTT domain at `level` (>= 1),
+ * outputs (coefficient domain, limbs 0..level-1) must not alias any input of
+ * the same layer. */
+typedef struct {
+  const uint64_t* c1[2];
+  const uint64_t* c2[2];
+  uint64_t* out[2];
+  int level;
+} aoa_mul_item;
+
+/* One left rotation by rot_slots with its rotation key (NTT domain QB):
+ * out = rot(in) (+ in when accumulate != 0, one rotate-and-sum step).
+ * Input and output in the coefficient domain; out may equal in. */
+typedef struct {
+  const uint64_t* in[2];
+  uint64_t* out[2];
+  const uint64_t* rtk_b;
+  const uint64_t* rtk_a;
+  int rot_slots;
+  int level;
+  int accumulate;
+} aoa_rot_item;
+
+int aoa_batch_create(aoa_batch** out);
+int aoa_batch_add_mul(aoa_batch* batch, const aoa_params* params, const uint64_t* evk_b, const uint64_t* evk_a,
+                      const aoa_mul_item* items, int n);
+int aoa_batch_add_rotate(aoa_batch* batch, const aoa_params* params, const aoa_rot_item* items, int n);
+/* Operations added after a barrier start after all earlier ones finish. */
+int aoa_batch_barrier(aoa_batch* batch);
+/* Allocates the batch's descriptor tables and workspace (once). */
+int aoa_batch_build(aoa_batch* batch);
+size_t aoa_batch_workspace_bytes(const aoa_batch* batch);
+/* Runs the batch on `stream`; use_graph != 0 captures/replays a hipGraph. */
+int aoa_batch_launch(aoa_batch* batch, void* stream, int use_graph);
+void aoa_batch_destroy(aoa_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif
