@@ -138,6 +138,18 @@ AOA_HD uint64_t ref_mulmod_shoup(uint64_t a, uint64_t w, uint64_t wq, uint64_t p
   return r;
 }
 
+// Lazy form of ref_mulmod_shoup: returns a value in [0, 2p) congruent to
+// ref_barrett(a * w) mod p.  r = Shoup remainder in [0, 2p); its canonical
+// value lies in the deviation window only if r < thr or r - p < thr
+// (unsigned), a superset of [lo, thr) that also covers [0, lo), where the
+// literal formula is just as exact.  thr must be < p.
+AOA_HD uint64_t ref_mulmod_shoup_lazy(uint64_t a, uint64_t w, uint64_t wq, uint64_t p, uint64_t mu, int nb,
+                                      uint64_t thr) {
+  uint64_t r = mul_shoup_lazy(a, w, wq, p);
+  if (r < thr || (r - p) < thr) r = ref_barrett(mulhi64(a, w), a * w, p, mu, nb);
+  return r;
+}
+
 // Bit length of a nonzero 64-bit value.
 AOA_HD int bitlen64(uint64_t x) {
   int n = 0;

@@ -117,20 +117,32 @@ __device__ __forceinline__ void modup_body(const DevParams& P, const uint64_t* x
     tl[i * BC_COEF + c] = mul_shoup(x[(size_t)i * N + cid], w.w, w.q, p);
   }
   __syncthreads();
-  for (int j = g; j < P.K; j += BC_GROUPS) {
-    const DevPrime db = P.primes[P.k + j];
-    const uint64_t* cj = P.qhat_mod_b + ((size_t)level * P.K + j) * P.k;
-    uint64_t hi = 0, lo = 0;
-#pragma unroll 4
+  // Two special limbs per pass over i (shared LDS reads, two independent
+  // 128-bit accumulation chains).
+  for (int j = g; j < P.K; j += 2 * BC_GROUPS) {
+    const int j2 = j + BC_GROUPS;
+    const bool two = j2 < P.K;
+    const DevPrime d1 = P.primes[P.k + j];
+    const DevPrime d2 = P.primes[P.k + (two ? j2 : j)];
+    const uint64_t* c1 = P.qhat_mod_b + ((size_t)level * P.K + j) * P.k;
+    const uint64_t* c2 = P.qhat_mod_b + ((size_t)level * P.K + (two ? j2 : j)) * P.k;
+    uint64_t h1 = 0, l1 = 0, h2 = 0, l2 = 0;
+#pragma unroll 2
     for (int i = 0; i < L; ++i) {
-      uint64_t v = tl[i * BC_COEF + c];
-      if (P.primes[i].p > db.p) v = barrett_reduce(0, v, db.p, db.mu, (int)db.nb);
-      const uint64_t w = cj[i];
-      const uint64_t ph = mulhi64(v, w), pl = v * w;
-      lo += pl;
-      hi += ph + (lo < pl ? 1u : 0u);
+      const uint64_t v = tl[i * BC_COEF + c];
+      const uint64_t qi = P.primes[i].p;
+      const uint64_t v1 = qi > d1.p ? barrett_reduce(0, v, d1.p, d1.mu, (int)d1.nb) : v;
+      const uint64_t v2 = qi > d2.p ? barrett_reduce(0, v, d2.p, d2.mu, (int)d2.nb) : v;
+      const uint64_t w1 = c1[i], w2 = c2[i];
+      const uint64_t ph1 = mulhi64(v1, w1), pl1 = v1 * w1;
+      const uint64_t ph2 = mulhi64(v2, w2), pl2 = v2 * w2;
+      l1 += pl1;
+      h1 += ph1 + (l1 < pl1 ? 1u : 0u);
+      l2 += pl2;
+      h2 += ph2 + (l2 < pl2 ? 1u : 0u);
     }
-    outB[(size_t)j * N + cid] = reduce128(hi, lo, db);
+    outB[(size_t)j * N + cid] = reduce128(h1, l1, d1);
+    if (two) outB[(size_t)j2 * N + cid] = reduce128(h2, l2, d2);
   }
 }
 
@@ -189,13 +201,48 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   }
   __syncthreads();
   const tw2* ibq = reinterpret_cast<const tw2*>(P.inv_b_mod_q);
+  // Sum over j of the reference's products for limb i, reduced mod q_i.
+  // Terms below 2^58 bits stay lazy in [0, 2p) and are summed without
+  // reduction (K <= 64 terms); the 63-bit prime reduces every term.
   auto conv = [&](int i, int j0, int j1) -> uint64_t {
     const DevPrime dq = P.primes[i];
     const tw2* ci = reinterpret_cast<const tw2*>(P.bhat_mod_q) + (size_t)i * K;
     uint64_t acc = 0;
+    if (dq.nb <= 56) {
+      const uint64_t thr = dq.quirk_thr < dq.p ? dq.quirk_thr : dq.p;
+#pragma unroll 4
+      for (int j = j0; j < j1; ++j) {
+        const tw2 w = ci[j];
+        acc += ref_mulmod_shoup_lazy(aux[j * BC_COEF + c], w.w, w.q, dq.p, dq.mu, (int)dq.nb, thr);
+      }
+      return barrett_reduce(0, acc, dq.p, dq.mu, (int)dq.nb);
+    }
 #pragma unroll 4
     for (int j = j0; j < j1; ++j) acc = add_mod(acc, moddown_term(aux[j * BC_COEF + c], ci[j], dq), dq.p);
     return acc;
+  };
+  // Two limbs at once: shared aux reads, two independent accumulation chains.
+  auto conv2 = [&](int i, int i2, uint64_t* r1, uint64_t* r2) {
+    const DevPrime d1 = P.primes[i], d2 = P.primes[i2];
+    if (d1.nb > 56 || d2.nb > 56) {
+      *r1 = conv(i, 0, K);
+      *r2 = conv(i2, 0, K);
+      return;
+    }
+    const tw2* c1 = reinterpret_cast<const tw2*>(P.bhat_mod_q) + (size_t)i * K;
+    const tw2* c2 = reinterpret_cast<const tw2*>(P.bhat_mod_q) + (size_t)i2 * K;
+    const uint64_t t1 = d1.quirk_thr < d1.p ? d1.quirk_thr : d1.p;
+    const uint64_t t2 = d2.quirk_thr < d2.p ? d2.quirk_thr : d2.p;
+    uint64_t a1 = 0, a2 = 0;
+#pragma unroll 2
+    for (int j = 0; j < K; ++j) {
+      const uint64_t a = aux[j * BC_COEF + c];
+      const tw2 w1 = c1[j], w2 = c2[j];
+      a1 += ref_mulmod_shoup_lazy(a, w1.w, w1.q, d1.p, d1.mu, (int)d1.nb, t1);
+      a2 += ref_mulmod_shoup_lazy(a, w2.w, w2.q, d2.p, d2.mu, (int)d2.nb, t2);
+    }
+    *r1 = barrett_reduce(0, a1, d1.p, d1.mu, (int)d1.nb);
+    *r2 = barrett_reduce(0, a2, d2.p, d2.mu, (int)d2.nb);
   };
   auto finish = [&](int i, uint64_t acc) -> uint64_t {
     const uint64_t p = P.primes[i].p;
@@ -205,7 +252,14 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
     return v;
   };
   if (!rescale) {
-    for (int i = g; i <= level; i += BC_GROUPS) out[(size_t)i * N + cid] = finish(i, conv(i, 0, K));
+    int i = g;
+    for (; i + BC_GROUPS <= level; i += 2 * BC_GROUPS) {
+      uint64_t r1, r2;
+      conv2(i, i + BC_GROUPS, &r1, &r2);
+      out[(size_t)i * N + cid] = finish(i, r1);
+      out[(size_t)(i + BC_GROUPS) * N + cid] = finish(i + BC_GROUPS, r2);
+    }
+    if (i <= level) out[(size_t)i * N + cid] = finish(i, conv(i, 0, K));
     return;
   }
   // Rescale needs limb `level` everywhere: its sum over j is split across the
@@ -221,12 +275,20 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   for (int t = 0; t < BC_GROUPS; ++t) acc = add_mod(acc, part[t * BC_COEF + c], pl);
   const uint64_t top = finish(level, acc);
   const tw2* iql = reinterpret_cast<const tw2*>(P.inv_ql_mod_qi) + (size_t)level * k;
-  for (int i = g; i < level; i += BC_GROUPS) {
+  auto emit = [&](int i, uint64_t acc) {
     const uint64_t p = P.primes[i].p;
-    const uint64_t v = finish(i, conv(i, 0, K));
+    const uint64_t v = finish(i, acc);
     const tw2 s = iql[i];
     out[(size_t)i * N + cid] = mul_shoup(sub_mod(v, top, p), s.w, s.q, p);
+  };
+  int i = g;
+  for (; i + BC_GROUPS < level; i += 2 * BC_GROUPS) {
+    uint64_t r1, r2;
+    conv2(i, i + BC_GROUPS, &r1, &r2);
+    emit(i, r1);
+    emit(i + BC_GROUPS, r2);
   }
+  if (i < level) emit(i, conv(i, 0, K));
 }
 
 __global__ __launch_bounds__(256) void k_moddown(DevParams P, ModDownArgs A) {

@@ -84,8 +84,11 @@ int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t s
       }
       const uint64_t ref = ref_barrett(mulhi64(a, c), a * c, p, d.mu, (int)d.nb);
       const uint64_t fast = ref_mulmod_shoup(a, c, cq, p, d.mu, (int)d.nb, d.quirk_lo, d.quirk_thr);
+      const uint64_t thr = d.quirk_thr < p ? d.quirk_thr : p;
+      const uint64_t lazy = ref_mulmod_shoup_lazy(a, c, cq, p, d.mu, (int)d.nb, thr);
       const uint64_t exact = (uint64_t)(((unsigned __int128)a * c) % p);
       if (ref != fast) ++bad;
+      if (lazy >= 2 * p || lazy % p != ref) ++bad;
       if (ref != exact) ++quirks;
       if (exact >= d.quirk_lo && exact < d.quirk_thr) ++slow;
     }
