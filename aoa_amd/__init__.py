@@ -4,4 +4,4 @@ The Python layer is plumbing for tests and benchmarks: device buffers come
 from torch, every computation runs in libaoa.so.
 """
 from ._lib import AoaError, check, load  # noqa: F401
-from .core import Batch, Params  # noqa: F401
+from .core import Batch, Params, Server  # noqa: F401

@@ -48,6 +48,33 @@ class RotItem(ctypes.Structure):
                 ("rot_slots", ctypes.c_int), ("level", ctypes.c_int), ("accumulate", ctypes.c_int)]
 
 
+class NttItem(ctypes.Structure):
+    _fields_ = [("poly", c_vp * 2), ("level", ctypes.c_int), ("dir", ctypes.c_int)]
+
+
+class Op(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("rot_slots", ctypes.c_int)]
+
+
+SIGNATURES.update({
+    "aoa_batch_add_ntt": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(NttItem), ctypes.c_int]),
+    "aoa_server_create": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "aoa_server_register": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
+    "aoa_server_submit": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp,
+                                          ctypes.POINTER(Op), ctypes.c_int, c_vp, c_vp,
+                                          ctypes.POINTER(ctypes.c_int64)]),
+    "aoa_server_step": (ctypes.c_int, [c_vp, c_vp]),
+    "aoa_server_start": (ctypes.c_int, [c_vp]),
+    "aoa_server_stop": (ctypes.c_int, [c_vp]),
+    "aoa_server_result": (ctypes.c_int, [c_vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
+    "aoa_server_stats": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "aoa_server_destroy": (None, [c_vp]),
+})
+
 SIGNATURES.update({
     "aoa_params_set_ntt_split": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "aoa_params_get_ntt_split": (ctypes.c_int, [c_vp]),

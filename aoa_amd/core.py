@@ -7,7 +7,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import MulItem, RotItem, check, load
+from ._lib import MulItem, Op, RotItem, check, load
 
 FORWARD, INVERSE = 0, 1
 ADD, SUB, MUL, NEGATE, MULADD = 0, 1, 2, 3, 5
@@ -179,6 +179,82 @@ class Batch:
     def close(self):
         if self.h:
             self.lib.aoa_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """aoa_server handle: continuous batching of encrypted-op requests."""
+
+    MUL, ROTATE = 0, 1
+    PENDING, ACTIVE, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, pool_bytes, step_ws_bytes, max_active=64):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.aoa_server_create(pool_bytes, step_ws_bytes, max_active, ctypes.byref(h)))
+        self.h = h
+        self._keep = []
+
+    def register(self, params, evk, rtks=None):
+        rtks = rtks or {}
+        steps = sorted(rtks)
+        n = len(steps)
+        st = (ctypes.c_int * max(1, n))(*steps)
+        kb = (ctypes.c_void_p * max(1, n))(*[rtks[s][0].data_ptr() for s in steps])
+        ka = (ctypes.c_void_p * max(1, n))(*[rtks[s][1].data_ptr() for s in steps])
+        self._keep += [params, evk, rtks]
+        rc = self.lib.aoa_server_register(self.h, params.h, _ptr(evk[0]), _ptr(evk[1]), n, st, kb, ka)
+        if rc < 0:
+            check(-rc)
+        return rc
+
+    def submit(self, keyset, c, level, in_ntt, m, ops, out):
+        """ops: list of ("mul",) / ("rot", slots)."""
+        arr = (Op * max(1, len(ops)))()
+        for i, op in enumerate(ops):
+            arr[i].kind = self.MUL if op[0] == "mul" else self.ROTATE
+            arr[i].rot_slots = op[1] if op[0] == "rot" else 0
+        rid = ctypes.c_int64()
+        check(self.lib.aoa_server_submit(self.h, keyset, _ptr(c[0]), _ptr(c[1]), level, int(in_ntt),
+                                         _ptr(m[0]) if m else None, _ptr(m[1]) if m else None, arr, len(ops),
+                                         _ptr(out[0]), _ptr(out[1]), ctypes.byref(rid)))
+        self._keep += [c, m, out]
+        return rid.value
+
+    def step(self, stream=None):
+        rc = self.lib.aoa_server_step(self.h, _stream(stream) if stream is not None else None)
+        if rc < 0:
+            check(-rc)
+        return rc
+
+    def start(self):
+        check(self.lib.aoa_server_start(self.h))
+
+    def stop(self):
+        check(self.lib.aoa_server_stop(self.h))
+
+    def result(self, rid):
+        st, lv, lat = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(self.lib.aoa_server_result(self.h, rid, ctypes.byref(st), ctypes.byref(lv), ctypes.byref(lat)))
+        return st.value, lv.value, lat.value
+
+    def stats(self):
+        c, o, s = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        span, used, keys = ctypes.c_double(), ctypes.c_size_t(), ctypes.c_size_t()
+        check(self.lib.aoa_server_stats(self.h, ctypes.byref(c), ctypes.byref(o), ctypes.byref(s),
+                                        ctypes.byref(span), ctypes.byref(used), ctypes.byref(keys)))
+        return {"completed": c.value, "ops": o.value, "steps": s.value, "span_ms": span.value,
+                "pool_used": used.value, "key_bytes": keys.value}
+
+    def close(self):
+        if self.h:
+            self.lib.aoa_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -3,65 +3,63 @@
 // sequence captured once as a hipGraph and replayed.
 //
 // A batch is a list of layers; a layer holds independent operations
-// (HomMul+relin+rescale or rotate[-and-accumulate]) grouped by parameter set.
-// Layers run in order (each may consume the previous one's outputs, as the
-// rotate-and-sum steps of ckks_sumslots do, aoa-ntt/src/ckks/ckks.cpp:479-495).
-// Every stage of a group is one descriptor-driven kernel launch covering all
-// its ciphertexts, so a launch carries thousands of workgroups even when each
-// ciphertext is small, and evaluation keys are shared between them.
+// (HomMul+relin+rescale, rotate[-and-accumulate], forward/inverse NTT) grouped
+// by parameter set.  Layers run in order (each may consume the previous one's
+// outputs, as the rotate-and-sum steps of ckks_sumslots do,
+// aoa-ntt/src/ckks/ckks.cpp:479-495).  Every stage of a group is one
+// descriptor-driven kernel launch covering all its ciphertexts, so a launch
+// carries thousands of workgroups even when each ciphertext is small, and
+// evaluation keys are shared between them.
 #include <algorithm>
 #include <cstring>
-#include <map>
 #include <vector>
 
 #include "internal.hpp"
 
 using namespace aoa;
 
-namespace {
+namespace aoa {
 
-struct MulOp {
-  aoa_mul_item it;
-};
-struct RotOp {
-  aoa_rot_item it;
-};
-
-struct Group {
+struct BatchGroup {
   const aoa_params* params = nullptr;
   const uint64_t* evk_b = nullptr;
   const uint64_t* evk_a = nullptr;
   std::vector<aoa_mul_item> muls;
   std::vector<aoa_rot_item> rots;
-  // built: descriptor tables (device) and counts
-  NttDesc* d_intt3 = nullptr;  // mul: INTT of d0, d1, d2
-  int n_intt3 = 0;
+  std::vector<aoa_ntt_item> ntts;
+  // descriptor tables (device) and counts, set by batch_fill
+  NttDesc* d_pre_fwd = nullptr;
+  int n_pre_fwd = 0;
+  NttDesc* d_pre_inv = nullptr;
+  int n_pre_inv = 0;
   TensorDesc* d_tensor = nullptr;
   int n_tensor = 0;
   AutoDesc* d_auto = nullptr;
   int n_auto = 0;
-  NttDesc* d_fwd = nullptr;  // NTT of ModUp output (and of the rotated c1)
-  int n_fwd = 0;
+  NttDesc* d_intt3 = nullptr;  // mul: INTT of d0, d1, d2
+  int n_intt3 = 0;
   ModUpDesc* d_modup = nullptr;
   int n_modup = 0;
   int max_level = 0;
+  NttDesc* d_fwd = nullptr;  // NTT of ModUp output (and of the rotated c1)
+  int n_fwd = 0;
   NttDesc* d_key = nullptr;  // key product + INTT
   int n_key = 0;
   ModDownDesc* d_moddown = nullptr;
   int n_moddown = 0;
 };
 
-struct Layer {
-  std::vector<Group> groups;
+struct BatchLayer {
+  std::vector<BatchGroup> groups;
 };
 
 size_t mul_ws_limbs(int L, int K) { return 6 * (size_t)L + 3 * (size_t)K; }
 size_t rot_ws_limbs(int L, int K) { return 5 * (size_t)L + 3 * (size_t)K; }
 
-}  // namespace
+}  // namespace aoa
 
 struct aoa_batch {
-  std::vector<Layer> layers;
+  std::vector<BatchLayer> layers;
   bool built = false;
   void* dmem = nullptr;  // descriptors
   void* ws = nullptr;    // workspace
@@ -72,120 +70,83 @@ struct aoa_batch {
   hipGraphExec_t exec = nullptr;
 };
 
-static Group& group_for(aoa_batch* b, const aoa_params* p, const uint64_t* kb, const uint64_t* ka) {
+namespace aoa {
+
+static BatchGroup& group_for(aoa_batch* b, const aoa_params* p, const uint64_t* kb, const uint64_t* ka) {
   if (b->layers.empty()) b->layers.emplace_back();
-  Layer& L = b->layers.back();
+  BatchLayer& L = b->layers.back();
   for (auto& g : L.groups)
     if (g.params == p && g.evk_b == kb && g.evk_a == ka) return g;
   L.groups.emplace_back();
-  Group& g = L.groups.back();
+  BatchGroup& g = L.groups.back();
   g.params = p;
   g.evk_b = kb;
   g.evk_a = ka;
   return g;
 }
 
-extern "C" int aoa_batch_create(aoa_batch** out) {
-  if (!out) return set_error(AOA_ERR_ARG, "null output handle");
-  *out = new aoa_batch();
-  (*out)->layers.emplace_back();
-  return AOA_OK;
-}
-
-extern "C" void aoa_batch_destroy(aoa_batch* b) {
-  if (!b) return;
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->cap) (void)hipStreamDestroy(b->cap);
-  if (b->dmem) (void)hipFree(b->dmem);
-  if (b->ws) (void)hipFree(b->ws);
-  delete b;
-}
-
-extern "C" int aoa_batch_add_mul(aoa_batch* b, const aoa_params* p, const uint64_t* evk_b, const uint64_t* evk_a,
-                                 const aoa_mul_item* items, int n) {
-  if (!b || !p || !evk_b || !evk_a || (!items && n > 0)) return set_error(AOA_ERR_ARG, "null argument");
-  if (b->built) return set_error(AOA_ERR_ARG, "batch already built");
-  for (int i = 0; i < n; ++i) {
-    const aoa_mul_item& it = items[i];
-    if (!it.c1[0] || !it.c1[1] || !it.c2[0] || !it.c2[1] || !it.out[0] || !it.out[1])
-      return set_error(AOA_ERR_ARG, "null pointer in item");
-    if (it.level == 0) return set_error(AOA_ERR_LEVEL, "The ciphertext level is 0. No more rescaling is possible.");
-    if (it.level < 0 || it.level >= p->host.k) return set_error(AOA_ERR_ARG, "level out of range");
-  }
-  Group& g = group_for(b, p, evk_b, evk_a);
-  g.muls.insert(g.muls.end(), items, items + n);
-  return AOA_OK;
-}
-
-extern "C" int aoa_batch_add_rotate(aoa_batch* b, const aoa_params* p, const aoa_rot_item* items, int n) {
-  if (!b || !p || (!items && n > 0)) return set_error(AOA_ERR_ARG, "null argument");
-  if (b->built) return set_error(AOA_ERR_ARG, "batch already built");
-  for (int i = 0; i < n; ++i) {
-    const aoa_rot_item& it = items[i];
-    if (!it.in[0] || !it.in[1] || !it.out[0] || !it.out[1] || !it.rtk_b || !it.rtk_a)
-      return set_error(AOA_ERR_ARG, "null pointer in item");
-    if (it.level < 0 || it.level >= p->host.k || it.rot_slots < 0) return set_error(AOA_ERR_ARG, "bad item");
-  }
-  // rotation groups carry their keys per item
-  Group& g = group_for(b, p, nullptr, nullptr);
-  g.rots.insert(g.rots.end(), items, items + n);
-  return AOA_OK;
-}
-
-extern "C" int aoa_batch_barrier(aoa_batch* b) {
-  if (!b) return set_error(AOA_ERR_ARG, "null batch");
-  if (b->built) return set_error(AOA_ERR_ARG, "batch already built");
-  if (!b->layers.back().groups.empty()) b->layers.emplace_back();
-  return AOA_OK;
-}
-
-// Builds every descriptor table on the host, then copies them in one go.
-extern "C" int aoa_batch_build(aoa_batch* b) {
-  if (!b) return set_error(AOA_ERR_ARG, "null batch");
-  if (b->built) return AOA_OK;
-  // workspace: per layer, sum of per-item needs; the batch keeps the max
-  size_t ws_limb_bytes = 0;
+// Workspace bytes (max over layers of the layer's sum) and descriptor bytes.
+void batch_sizes(const aoa_batch* b, size_t* ws_bytes, size_t* desc_bytes) {
+  size_t ws = 0, desc = 0;
   for (auto& L : b->layers) {
     size_t s = 0;
     for (auto& g : L.groups) {
       const size_t N = g.params->host.N, K = g.params->host.K;
-      for (auto& it : g.muls) s += mul_ws_limbs(it.level + 1, (int)K) * N * 8;
-      for (auto& it : g.rots) s += rot_ws_limbs(it.level + 1, (int)K) * N * 8;
+      for (auto& it : g.muls) {
+        const size_t Lq = it.level + 1;
+        s += mul_ws_limbs((int)Lq, (int)K) * N * 8;
+        desc += Lq * (sizeof(TensorDesc) + 3 * sizeof(NttDesc)) + sizeof(ModUpDesc) + K * sizeof(NttDesc) +
+                2 * (Lq + K) * sizeof(NttDesc) + 2 * sizeof(ModDownDesc);
+      }
+      for (auto& it : g.rots) {
+        const size_t Lq = it.level + 1;
+        s += rot_ws_limbs((int)Lq, (int)K) * N * 8;
+        desc += Lq * (2 * sizeof(AutoDesc) + sizeof(NttDesc)) + sizeof(ModUpDesc) + K * sizeof(NttDesc) +
+                2 * (Lq + K) * sizeof(NttDesc) + 2 * sizeof(ModDownDesc);
+      }
+      for (auto& it : g.ntts) desc += 2 * (size_t)(it.level + 1) * sizeof(NttDesc);
+      desc += 16 * 256;  // per-table alignment slack
     }
-    ws_limb_bytes = std::max(ws_limb_bytes, s);
+    ws = std::max(ws, s);
   }
-  b->ws_bytes = ws_limb_bytes;
-  if (b->ws_bytes && hipMalloc(&b->ws, b->ws_bytes) != hipSuccess)
-    return set_error(AOA_ERR_HIP, "hipMalloc of batch workspace failed");
+  *ws_bytes = ws;
+  *desc_bytes = desc;
+}
 
-  // host descriptor staging, one byte vector; device pointers patched after
-  std::vector<char> host;
-  struct Patch {
-    Group* g;
-    int which;
-    size_t off;
+// Fills every descriptor table into `host` (desc_cap bytes) using workspace
+// `ws` and device descriptor base `dev`; device table pointers are set to
+// dev + offset.  Returns the bytes used, or 0 if desc_cap is too small.
+size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc_cap) {
+  size_t used = 0;
+  bool overflow = false;
+  auto put = [&](const void* p, size_t bytes) -> char* {
+    size_t off = (used + 255) & ~(size_t)255;
+    if (off + bytes > desc_cap) {
+      overflow = true;
+      return nullptr;
+    }
+    std::memcpy(host + off, p, bytes);
+    used = off + bytes;
+    return dev + off;
   };
-  std::vector<Patch> patches;
-  auto put = [&](const void* p, size_t bytes) {
-    size_t off = (host.size() + 255) & ~(size_t)255;
-    host.resize(off + bytes);
-    std::memcpy(host.data() + off, p, bytes);
-    return off;
-  };
-  uint64_t* wsb = (uint64_t*)b->ws;
   for (auto& L : b->layers) {
-    uint64_t* w = wsb;  // layers reuse the workspace
+    uint64_t* w = ws;  // layers reuse the workspace
     for (auto& g : L.groups) {
       const DevParams& P = g.params->dev;
       const size_t N = (size_t)P.N;
       const int k = P.k, K = P.K;
       std::vector<TensorDesc> tens;
-      std::vector<NttDesc> intt3, fwd, key;
+      std::vector<NttDesc> prefwd, preinv, intt3, fwd, key;
       std::vector<ModUpDesc> mu;
       std::vector<ModDownDesc> md;
       std::vector<AutoDesc> au;
       g.max_level = 0;
+      for (auto& it : g.ntts)
+        for (int y = 0; y < 2; ++y)
+          for (int i = 0; i <= it.level; ++i) {
+            uint64_t* l = it.poly[y] + (size_t)i * N;
+            (it.dir == AOA_FORWARD ? prefwd : preinv).push_back({l, l, nullptr, i, 0});
+          }
       for (auto& it : g.muls) {
         const int lv = it.level, Lq = lv + 1;
         g.max_level = std::max(g.max_level, lv);
@@ -214,7 +175,8 @@ extern "C" int aoa_batch_build(aoa_batch* b) {
           intt3.push_back({X + o, D2 + o, nullptr, i, 0});
         }
         mu.push_back({D2, X + (size_t)Lq * N, lv, 0});
-        for (int j = 0; j < K; ++j) fwd.push_back({X + (size_t)(Lq + j) * N, X + (size_t)(Lq + j) * N, nullptr, k + j, 0});
+        for (int j = 0; j < K; ++j)
+          fwd.push_back({X + (size_t)(Lq + j) * N, X + (size_t)(Lq + j) * N, nullptr, k + j, 0});
         for (int y = 0; y < 2; ++y) {
           uint64_t* Py = y ? P1 : P0;
           const uint64_t* ky = y ? g.evk_a : g.evk_b;
@@ -237,14 +199,13 @@ extern "C" int aoa_batch_build(aoa_batch* b) {
         const uint64_t ginv = galois_inverse(galois_of(it.rot_slots, P.N), P.N);
         for (int i = 0; i < Lq; ++i) {
           const size_t o = (size_t)i * N;
-          AutoDesc a0 = {it.in[0] + o, R0 + o, it.accumulate ? it.in[0] + o : nullptr, ginv, i, 0};
-          AutoDesc a1 = {it.in[1] + o, R1 + o, nullptr, ginv, i, 0};
-          au.push_back(a0);
-          au.push_back(a1);
+          au.push_back({it.in[0] + o, R0 + o, it.accumulate ? it.in[0] + o : nullptr, ginv, i, 0});
+          au.push_back({it.in[1] + o, R1 + o, nullptr, ginv, i, 0});
           fwd.push_back({R1 + o, X + o, nullptr, i, 0});
         }
         mu.push_back({R1, X + (size_t)Lq * N, lv, 0});
-        for (int j = 0; j < K; ++j) fwd.push_back({X + (size_t)(Lq + j) * N, X + (size_t)(Lq + j) * N, nullptr, k + j, 0});
+        for (int j = 0; j < K; ++j)
+          fwd.push_back({X + (size_t)(Lq + j) * N, X + (size_t)(Lq + j) * N, nullptr, k + j, 0});
         for (int y = 0; y < 2; ++y) {
           uint64_t* Py = y ? P1 : P0;
           const uint64_t* ky = y ? it.rtk_a : it.rtk_b;
@@ -256,6 +217,8 @@ extern "C" int aoa_batch_build(aoa_batch* b) {
           md.push_back({Py, Py + (size_t)Lq * N, add, it.out[y], lv, 0});
         }
       }
+      g.n_pre_fwd = (int)prefwd.size();
+      g.n_pre_inv = (int)preinv.size();
       g.n_tensor = (int)tens.size();
       g.n_intt3 = (int)intt3.size();
       g.n_auto = (int)au.size();
@@ -263,42 +226,30 @@ extern "C" int aoa_batch_build(aoa_batch* b) {
       g.n_modup = (int)mu.size();
       g.n_key = (int)key.size();
       g.n_moddown = (int)md.size();
-      if (!tens.empty()) patches.push_back({&g, 0, put(tens.data(), tens.size() * sizeof(TensorDesc))});
-      if (!intt3.empty()) patches.push_back({&g, 1, put(intt3.data(), intt3.size() * sizeof(NttDesc))});
-      if (!au.empty()) patches.push_back({&g, 2, put(au.data(), au.size() * sizeof(AutoDesc))});
-      if (!fwd.empty()) patches.push_back({&g, 3, put(fwd.data(), fwd.size() * sizeof(NttDesc))});
-      if (!mu.empty()) patches.push_back({&g, 4, put(mu.data(), mu.size() * sizeof(ModUpDesc))});
-      if (!key.empty()) patches.push_back({&g, 5, put(key.data(), key.size() * sizeof(NttDesc))});
-      if (!md.empty()) patches.push_back({&g, 6, put(md.data(), md.size() * sizeof(ModDownDesc))});
+      g.d_pre_fwd = (NttDesc*)(prefwd.empty() ? nullptr : put(prefwd.data(), prefwd.size() * sizeof(NttDesc)));
+      g.d_pre_inv = (NttDesc*)(preinv.empty() ? nullptr : put(preinv.data(), preinv.size() * sizeof(NttDesc)));
+      g.d_tensor = (TensorDesc*)(tens.empty() ? nullptr : put(tens.data(), tens.size() * sizeof(TensorDesc)));
+      g.d_intt3 = (NttDesc*)(intt3.empty() ? nullptr : put(intt3.data(), intt3.size() * sizeof(NttDesc)));
+      g.d_auto = (AutoDesc*)(au.empty() ? nullptr : put(au.data(), au.size() * sizeof(AutoDesc)));
+      g.d_fwd = (NttDesc*)(fwd.empty() ? nullptr : put(fwd.data(), fwd.size() * sizeof(NttDesc)));
+      g.d_modup = (ModUpDesc*)(mu.empty() ? nullptr : put(mu.data(), mu.size() * sizeof(ModUpDesc)));
+      g.d_key = (NttDesc*)(key.empty() ? nullptr : put(key.data(), key.size() * sizeof(NttDesc)));
+      g.d_moddown = (ModDownDesc*)(md.empty() ? nullptr : put(md.data(), md.size() * sizeof(ModDownDesc)));
     }
   }
-  b->desc_bytes = host.size();
-  if (b->desc_bytes) {
-    if (hipMalloc(&b->dmem, b->desc_bytes) != hipSuccess) return set_error(AOA_ERR_HIP, "hipMalloc of descriptors failed");
-    if (hipMemcpy(b->dmem, host.data(), b->desc_bytes, hipMemcpyHostToDevice) != hipSuccess)
-      return set_error(AOA_ERR_HIP, "hipMemcpy of descriptors failed");
-  }
-  char* base = (char*)b->dmem;
-  for (auto& pt : patches) {
-    void* p = base + pt.off;
-    switch (pt.which) {
-      case 0: pt.g->d_tensor = (TensorDesc*)p; break;
-      case 1: pt.g->d_intt3 = (NttDesc*)p; break;
-      case 2: pt.g->d_auto = (AutoDesc*)p; break;
-      case 3: pt.g->d_fwd = (NttDesc*)p; break;
-      case 4: pt.g->d_modup = (ModUpDesc*)p; break;
-      case 5: pt.g->d_key = (NttDesc*)p; break;
-      case 6: pt.g->d_moddown = (ModDownDesc*)p; break;
-    }
-  }
-  b->built = true;
-  return AOA_OK;
+  return overflow ? 0 : used;
 }
 
 // Issues every stage launch of the batch on stream st.
-static int enqueue(aoa_batch* b, hipStream_t st) {
+int batch_enqueue(aoa_batch* b, hipStream_t st) {
   for (auto& L : b->layers) {
     // stage order inside a layer; groups of the same stage back to back
+    for (auto& g : L.groups)
+      if (g.n_pre_fwd && ntt_forward_desc(g.params->dev, g.d_pre_fwd, g.n_pre_fwd, 0, st))
+        return set_error(AOA_ERR_UNSUPPORTED, "no NTT split");
+    for (auto& g : L.groups)
+      if (g.n_pre_inv && ntt_inverse_desc(g.params->dev, g.d_pre_inv, g.n_pre_inv, 0, st))
+        return set_error(AOA_ERR_UNSUPPORTED, "no NTT split");
     for (auto& g : L.groups)
       if (g.n_tensor) launch_tensor_desc(g.params->dev, g.d_tensor, g.n_tensor, st);
     for (auto& g : L.groups)
@@ -320,6 +271,125 @@ static int enqueue(aoa_batch* b, hipStream_t st) {
   return check_launch("aoa_batch");
 }
 
+int batch_add_mul(aoa_batch* b, const aoa_params* p, const uint64_t* evk_b, const uint64_t* evk_a,
+                  const aoa_mul_item* items, int n) {
+  for (int i = 0; i < n; ++i) {
+    const aoa_mul_item& it = items[i];
+    if (!it.c1[0] || !it.c1[1] || !it.c2[0] || !it.c2[1] || !it.out[0] || !it.out[1])
+      return set_error(AOA_ERR_ARG, "null pointer in item");
+    if (it.level == 0) return set_error(AOA_ERR_LEVEL, "The ciphertext level is 0. No more rescaling is possible.");
+    if (it.level < 0 || it.level >= p->host.k) return set_error(AOA_ERR_ARG, "level out of range");
+  }
+  BatchGroup& g = group_for(b, p, evk_b, evk_a);
+  g.muls.insert(g.muls.end(), items, items + n);
+  return AOA_OK;
+}
+
+int batch_add_rotate(aoa_batch* b, const aoa_params* p, const aoa_rot_item* items, int n) {
+  for (int i = 0; i < n; ++i) {
+    const aoa_rot_item& it = items[i];
+    if (!it.in[0] || !it.in[1] || !it.out[0] || !it.out[1] || !it.rtk_b || !it.rtk_a)
+      return set_error(AOA_ERR_ARG, "null pointer in item");
+    if (it.level < 0 || it.level >= p->host.k || it.rot_slots < 0) return set_error(AOA_ERR_ARG, "bad item");
+  }
+  BatchGroup& g = group_for(b, p, nullptr, nullptr);
+  g.rots.insert(g.rots.end(), items, items + n);
+  return AOA_OK;
+}
+
+int batch_add_ntt(aoa_batch* b, const aoa_params* p, const aoa_ntt_item* items, int n) {
+  for (int i = 0; i < n; ++i) {
+    const aoa_ntt_item& it = items[i];
+    if (!it.poly[0] || !it.poly[1]) return set_error(AOA_ERR_ARG, "null pointer in item");
+    if (it.level < 0 || it.level >= p->host.k) return set_error(AOA_ERR_ARG, "level out of range");
+    if (it.dir != AOA_FORWARD && it.dir != AOA_INVERSE) return set_error(AOA_ERR_ARG, "bad direction");
+  }
+  BatchGroup& g = group_for(b, p, nullptr, nullptr);
+  g.ntts.insert(g.ntts.end(), items, items + n);
+  return AOA_OK;
+}
+
+void batch_clear(aoa_batch* b) {
+  b->layers.clear();
+  b->layers.emplace_back();
+}
+
+bool batch_empty(const aoa_batch* b) {
+  for (auto& L : b->layers)
+    for (auto& g : L.groups)
+      if (!g.muls.empty() || !g.rots.empty() || !g.ntts.empty()) return false;
+  return true;
+}
+
+void batch_barrier(aoa_batch* b) {
+  if (!b->layers.back().groups.empty()) b->layers.emplace_back();
+}
+
+}  // namespace aoa
+
+// ------------------------------------------------------------------ C-ABI --
+
+extern "C" int aoa_batch_create(aoa_batch** out) {
+  if (!out) return set_error(AOA_ERR_ARG, "null output handle");
+  *out = new aoa_batch();
+  (*out)->layers.emplace_back();
+  return AOA_OK;
+}
+
+extern "C" void aoa_batch_destroy(aoa_batch* b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  if (b->dmem) (void)hipFree(b->dmem);
+  if (b->ws) (void)hipFree(b->ws);
+  delete b;
+}
+
+extern "C" int aoa_batch_add_mul(aoa_batch* b, const aoa_params* p, const uint64_t* evk_b, const uint64_t* evk_a,
+                                 const aoa_mul_item* items, int n) {
+  if (!b || !p || !evk_b || !evk_a || (!items && n > 0)) return set_error(AOA_ERR_ARG, "null argument");
+  if (b->built) return set_error(AOA_ERR_ARG, "batch already built");
+  return batch_add_mul(b, p, evk_b, evk_a, items, n);
+}
+
+extern "C" int aoa_batch_add_rotate(aoa_batch* b, const aoa_params* p, const aoa_rot_item* items, int n) {
+  if (!b || !p || (!items && n > 0)) return set_error(AOA_ERR_ARG, "null argument");
+  if (b->built) return set_error(AOA_ERR_ARG, "batch already built");
+  return batch_add_rotate(b, p, items, n);
+}
+
+extern "C" int aoa_batch_add_ntt(aoa_batch* b, const aoa_params* p, const aoa_ntt_item* items, int n) {
+  if (!b || !p || (!items && n > 0)) return set_error(AOA_ERR_ARG, "null argument");
+  if (b->built) return set_error(AOA_ERR_ARG, "batch already built");
+  return batch_add_ntt(b, p, items, n);
+}
+
+extern "C" int aoa_batch_barrier(aoa_batch* b) {
+  if (!b) return set_error(AOA_ERR_ARG, "null batch");
+  if (b->built) return set_error(AOA_ERR_ARG, "batch already built");
+  batch_barrier(b);
+  return AOA_OK;
+}
+
+extern "C" int aoa_batch_build(aoa_batch* b) {
+  if (!b) return set_error(AOA_ERR_ARG, "null batch");
+  if (b->built) return AOA_OK;
+  size_t ws = 0, desc = 0;
+  batch_sizes(b, &ws, &desc);
+  b->ws_bytes = ws;
+  if (ws && hipMalloc(&b->ws, ws) != hipSuccess) return set_error(AOA_ERR_HIP, "hipMalloc of batch workspace failed");
+  if (desc && hipMalloc(&b->dmem, desc) != hipSuccess) return set_error(AOA_ERR_HIP, "hipMalloc of descriptors failed");
+  std::vector<char> host(desc);
+  size_t used = batch_fill(b, (uint64_t*)b->ws, host.data(), (char*)b->dmem, desc);
+  if (desc && !used) return set_error(AOA_ERR_ARG, "descriptor sizing failed");
+  b->desc_bytes = used;
+  if (used && hipMemcpy(b->dmem, host.data(), used, hipMemcpyHostToDevice) != hipSuccess)
+    return set_error(AOA_ERR_HIP, "hipMemcpy of descriptors failed");
+  b->built = true;
+  return AOA_OK;
+}
+
 extern "C" size_t aoa_batch_workspace_bytes(const aoa_batch* b) { return b ? b->ws_bytes : 0; }
 
 extern "C" int aoa_batch_launch(aoa_batch* b, void* stream, int use_graph) {
@@ -327,13 +397,13 @@ extern "C" int aoa_batch_launch(aoa_batch* b, void* stream, int use_graph) {
   int rc = aoa_batch_build(b);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
-  if (!use_graph) return enqueue(b, st);
+  if (!use_graph) return batch_enqueue(b, st);
   if (!b->exec) {
     if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess)
       return set_error(AOA_ERR_HIP, "hipStreamCreate failed");
     if (hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return set_error(AOA_ERR_HIP, "hipStreamBeginCapture failed");
-    int erc = enqueue(b, b->cap);
+    int erc = batch_enqueue(b, b->cap);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->cap, &g);
     if (erc) {

@@ -179,7 +179,17 @@ typedef struct {
   int accumulate;
 } aoa_rot_item;
 
+/* In-place forward (dir = AOA_FORWARD) or inverse NTT of limbs 0..level of
+ * both polynomials of a ciphertext: the domain changes of poly_t's state
+ * machine (aoa-ntt/src/arithmetic/poly_t.cpp:201-220) inside a batch. */
+typedef struct {
+  uint64_t* poly[2];
+  int level;
+  int dir;
+} aoa_ntt_item;
+
 int aoa_batch_create(aoa_batch** out);
+int aoa_batch_add_ntt(aoa_batch* batch, const aoa_params* params, const aoa_ntt_item* items, int n);
 int aoa_batch_add_mul(aoa_batch* batch, const aoa_params* params, const uint64_t* evk_b, const uint64_t* evk_a,
                       const aoa_mul_item* items, int n);
 int aoa_batch_add_rotate(aoa_batch* batch, const aoa_params* params, const aoa_rot_item* items, int n);
@@ -191,6 +201,46 @@ size_t aoa_batch_workspace_bytes(const aoa_batch* batch);
 /* Runs the batch on `stream`; use_graph != 0 captures/replays a hipGraph. */
 int aoa_batch_launch(aoa_batch* batch, void* stream, int use_graph);
 void aoa_batch_destroy(aoa_batch* batch);
+
+/* ---------------------------------------------------------------- serving --
+ * Continuous batching of encrypted-op requests.  No counterpart in the
+ * reference (one call at a time on one stream,
+ * aoa-ntt/include/newckks/tool/context.h:16-49).  A request is a ciphertext
+ * and a chain of ops; each HomMul+relin+rescale multiplies the running
+ * ciphertext by the request's operand ciphertext (NTT domain, top level),
+ * each rotation uses a registered rotation key.  The server owns a device
+ * pool (working ciphertexts, admission budget) and a step workspace; keys
+ * stay resident (caller memory, registered once). */
+typedef struct aoa_server aoa_server;
+enum { AOA_OP_MUL = 0, AOA_OP_ROTATE = 1 };
+typedef struct {
+  int kind;      /* AOA_OP_MUL or AOA_OP_ROTATE */
+  int rot_slots; /* AOA_OP_ROTATE: left rotation by this many slots */
+} aoa_op;
+/* Request states reported by aoa_server_result. */
+enum { AOA_REQ_PENDING = 0, AOA_REQ_ACTIVE = 1, AOA_REQ_DONE = 2, AOA_REQ_FAILED = 3 };
+
+int aoa_server_create(size_t pool_bytes, size_t step_ws_bytes, int max_active, aoa_server** out);
+/* Registers a parameter set with its evaluation key and rotation keys (NTT
+ * domain QB polynomials).  Returns the key-set id (>= 0) or -status. */
+int aoa_server_register(aoa_server* server, const aoa_params* params, const uint64_t* evk_b, const uint64_t* evk_a,
+                        int nrot, const int* rot_steps, const uint64_t* const* rtk_b, const uint64_t* const* rtk_a);
+/* Queues a request; input (c0, c1) at `level` (in_ntt: NTT domain), result
+ * copied to (out0, out1) in the coefficient domain at the final level.
+ * Inputs and outputs must stay valid until the request is done. */
+int aoa_server_submit(aoa_server* server, int keyset, const uint64_t* c0, const uint64_t* c1, int level, int in_ntt,
+                      const uint64_t* m0, const uint64_t* m1, const aoa_op* ops, int nops, uint64_t* out0,
+                      uint64_t* out1, int64_t* req_id);
+/* Runs one step on `stream` (NULL: the server's own stream) when no server
+ * thread is running; returns requests still queued or active, or -status. */
+int aoa_server_step(aoa_server* server, void* stream);
+/* Background serving thread (own stream). */
+int aoa_server_start(aoa_server* server);
+int aoa_server_stop(aoa_server* server);
+int aoa_server_result(aoa_server* server, int64_t req_id, int* state, int* level, double* latency_ms);
+int aoa_server_stats(aoa_server* server, int64_t* completed, int64_t* ops, int64_t* steps, double* span_ms,
+                     size_t* pool_used, size_t* key_bytes);
+void aoa_server_destroy(aoa_server* server);
 
 #ifdef __cplusplus
 }
