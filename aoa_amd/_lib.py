@@ -78,6 +78,11 @@ SIGNATURES.update({
 SIGNATURES.update({
     "aoa_params_set_ntt_split": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "aoa_params_get_ntt_split": (ctypes.c_int, [c_vp]),
+    "aoa_params_set_moddown_exact": (ctypes.c_int, [c_vp, ctypes.c_int]),
+    "aoa_params_get_moddown_exact": (ctypes.c_int, [c_vp]),
+    "aoa_hommul_relin": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                         ctypes.c_int, c_vp, ctypes.c_size_t]),
+    "aoa_rho": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "aoa_autotune_ntt": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "aoa_batch_create": (ctypes.c_int, [ctypes.POINTER(c_vp)]),

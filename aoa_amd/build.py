@@ -43,7 +43,38 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_newckks(verbose)
     return LIB
+
+
+NEWCKKS_LIB = os.path.join(HERE, "libnewckks.so")
+CPP_TEST = os.path.join(ROOT, "tests", "cpp", "_build", "test_newckks")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+
+
+def build_newckks(verbose=False):
+    """libnewckks.so: the newckks-compatible C++ API, a host library over the
+    C-ABI of libaoa.so; plus the C++ test program written against it."""
+    inc = ["-I" + os.path.join(ROOT, "include")]
+    obj = os.path.join(OBJ, "newckks.o")
+    _run([HIPCC, "-O2", "-fPIC", "-std=c++17", "-Wall"] + inc +
+         ["-c", os.path.join(CSRC, "newckks", "newckks.cpp"), "-o", obj])
+    tmp = NEWCKKS_LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "-o", tmp, obj, "-L" + HERE, "-laoa", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, NEWCKKS_LIB)
+    os.makedirs(os.path.dirname(CPP_TEST), exist_ok=True)
+    _run([HIPCC, "-O2", "-std=c++17"] + inc + [os.path.join(ROOT, "tests", "cpp", "test_newckks.cpp"), "-o",
+                                               CPP_TEST + ".tmp", "-L" + HERE, "-lnewckks", "-laoa",
+                                               "-Wl,-rpath," + os.path.relpath(HERE, os.path.dirname(CPP_TEST))
+                                               .join(["$ORIGIN/", ""])])
+    os.replace(CPP_TEST + ".tmp", CPP_TEST)
+    if verbose:
+        print("built", NEWCKKS_LIB, CPP_TEST)
 
 
 if __name__ == "__main__":

@@ -108,6 +108,20 @@ class Params:
             self.h, _stream(stream), _ptr(c3[0]), _ptr(c3[1]), _ptr(c1[0]), _ptr(c1[1]), _ptr(c2[0]),
             _ptr(c2[1]), _ptr(evk[0]), _ptr(evk[1]), level, _ptr(w), nb))
 
+    def set_moddown_exact(self, exact):
+        """False (default): the reference's ModDown reduction, bit-exact residues;
+        True: exact reduction."""
+        check(self.lib.aoa_params_set_moddown_exact(self.h, int(bool(exact))))
+
+    def hommul_norescale(self, c3, c1, c2, evk, level, ws, stream=None):
+        w, nb = ws
+        check(self.lib.aoa_hommul_relin(
+            self.h, _stream(stream), _ptr(c3[0]), _ptr(c3[1]), _ptr(c1[0]), _ptr(c1[1]), _ptr(c2[0]),
+            _ptr(c2[1]), _ptr(evk[0]), _ptr(evk[1]), level, _ptr(w), nb))
+
+    def rho(self, out, inp, stream=None):
+        check(self.lib.aoa_rho(self.h, _stream(stream), _ptr(out), _ptr(inp)))
+
     def set_ntt_split(self, n1):
         check(self.lib.aoa_params_set_ntt_split(self.h, n1))
 

@@ -314,10 +314,9 @@ extern "C" int aoa_keyswitch(const aoa_params* h, void* stream, uint64_t* out0, 
   return check_launch("aoa_keyswitch");
 }
 
-extern "C" int aoa_hommul_relin_rescale(const aoa_params* h, void* stream, uint64_t* c3_0, uint64_t* c3_1,
-                                        const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
-                                        const uint64_t* b1, const uint64_t* evk_b, const uint64_t* evk_a,
-                                        int level, void* ws, size_t ws_bytes) {
+static int hommul_core(const aoa_params* h, void* stream, uint64_t* c3_0, uint64_t* c3_1, const uint64_t* a0,
+                       const uint64_t* a1, const uint64_t* b0, const uint64_t* b1, const uint64_t* evk_b,
+                       const uint64_t* evk_a, int level, void* ws, size_t ws_bytes, int rescale) {
   if (!h || !c3_0 || !c3_1 || !a0 || !a1 || !b0 || !b1 || !evk_b || !evk_a || !ws)
     return fail(AOA_ERR_ARG, "null argument");
   if (level == 0) return fail(AOA_ERR_LEVEL, "The ciphertext level is 0. No more rescaling is possible.");
@@ -375,9 +374,31 @@ extern "C" int aoa_hommul_relin_rescale(const aoa_params* h, void* stream, uint6
   a.out[1] = c3_1;
   a.npolys = 2;
   a.level = level;
-  a.rescale = 1;
+  a.rescale = rescale;
   launch_moddown(P, a, st);
-  return check_launch("aoa_hommul_relin_rescale");
+  return check_launch(rescale ? "aoa_hommul_relin_rescale" : "aoa_hommul_relin");
+}
+
+extern "C" int aoa_hommul_relin_rescale(const aoa_params* h, void* stream, uint64_t* c3_0, uint64_t* c3_1,
+                                        const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
+                                        const uint64_t* b1, const uint64_t* evk_b, const uint64_t* evk_a,
+                                        int level, void* ws, size_t ws_bytes) {
+  return hommul_core(h, stream, c3_0, c3_1, a0, a1, b0, b1, evk_b, evk_a, level, ws, ws_bytes, 1);
+}
+
+extern "C" int aoa_hommul_relin(const aoa_params* h, void* stream, uint64_t* c3_0, uint64_t* c3_1,
+                                const uint64_t* a0, const uint64_t* a1, const uint64_t* b0, const uint64_t* b1,
+                                const uint64_t* evk_b, const uint64_t* evk_a, int level, void* ws,
+                                size_t ws_bytes) {
+  return hommul_core(h, stream, c3_0, c3_1, a0, a1, b0, b1, evk_b, evk_a, level, ws, ws_bytes, 0);
+}
+
+// rho (aoa-ntt/src/cuda/manager.cu:149-176): Q limbs times B mod q_i, special
+// limbs zero.  Any domain (the factor is a scalar per limb).
+extern "C" int aoa_rho(const aoa_params* h, void* stream, uint64_t* out, const uint64_t* in) {
+  if (!h || !out || !in) return fail(AOA_ERR_ARG, "null argument");
+  launch_rho(h->dev, out, in, (hipStream_t)stream);
+  return check_launch("aoa_rho");
 }
 
 extern "C" int aoa_rotate_left(const aoa_params* h, void* stream, uint64_t* out0, uint64_t* out1,

@@ -55,6 +55,9 @@ void launch_automorphism(const DevParams& P, uint64_t* out, const uint64_t* in, 
                          uint64_t galois, hipStream_t st);
 uint64_t galois_inverse(uint64_t galois, int N);
 
+// rho of a QB polynomial: Q limbs times (B mod q_i), special limbs zero.
+void launch_rho(const DevParams& P, uint64_t* out, const uint64_t* in, hipStream_t st);
+
 // Descriptor-driven (stage-fused) forms used by batches: one launch covers
 // every descriptor of the table (device memory).
 int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st);

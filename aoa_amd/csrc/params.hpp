@@ -44,7 +44,9 @@ struct DevPrime {
 struct DevParams {
   int N, logN, k, K;
   int ntt_n1;  // NTT split N = N1 x (N/N1); 0 = built-in default (autotuner sets it)
-  int pad;
+  // 0: ModDown reproduces the reference's Barrett reduction of its
+  // cross-base products (bit-exact residues); 1: exact reduction.
+  int moddown_exact;
   const DevPrime* primes;        // [k+K]: Q primes then B primes
   const uint64_t* tw_fwd;        // [k+K][N][2]: psi^brev(i) and Shoup quotient
   const uint64_t* tw_inv;        // [k+K][N][2]: psi^-brev(i) and Shoup quotient

@@ -180,8 +180,13 @@ void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int ma
 // remainder is used unless r falls in [eps, delta_max*eps), a window of
 // probability about 2^-17 per term at 48-bit q_i, where the literal formula
 // is evaluated.
-__device__ __forceinline__ uint64_t moddown_term(uint64_t a, const tw2 w, const DevPrime& dq) {
-  return ref_mulmod_shoup(a, w.w, w.q, dq.p, dq.mu, (int)dq.nb, dq.quirk_lo, dq.quirk_thr);
+// exact != 0 selects the exact remainder for every term (an empty window).
+__device__ __forceinline__ uint64_t moddown_term(uint64_t a, const tw2 w, const DevPrime& dq, int exact) {
+  return ref_mulmod_shoup(a, w.w, w.q, dq.p, dq.mu, (int)dq.nb, exact ? 0 : dq.quirk_lo, exact ? 0 : dq.quirk_thr);
+}
+
+__device__ __forceinline__ uint64_t lazy_window(const DevPrime& d, int exact) {
+  return exact ? 0 : (d.quirk_thr < d.p ? d.quirk_thr : d.p);
 }
 
 __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t* inQ, const uint64_t* inB,
@@ -209,7 +214,7 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
     const tw2* ci = reinterpret_cast<const tw2*>(P.bhat_mod_q) + (size_t)i * K;
     uint64_t acc = 0;
     if (dq.nb <= 56) {
-      const uint64_t thr = dq.quirk_thr < dq.p ? dq.quirk_thr : dq.p;
+      const uint64_t thr = lazy_window(dq, P.moddown_exact);
 #pragma unroll 4
       for (int j = j0; j < j1; ++j) {
         const tw2 w = ci[j];
@@ -218,7 +223,8 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
       return barrett_reduce(0, acc, dq.p, dq.mu, (int)dq.nb);
     }
 #pragma unroll 4
-    for (int j = j0; j < j1; ++j) acc = add_mod(acc, moddown_term(aux[j * BC_COEF + c], ci[j], dq), dq.p);
+    for (int j = j0; j < j1; ++j)
+      acc = add_mod(acc, moddown_term(aux[j * BC_COEF + c], ci[j], dq, P.moddown_exact), dq.p);
     return acc;
   };
   // Two limbs at once: shared aux reads, two independent accumulation chains.
@@ -231,8 +237,8 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
     }
     const tw2* c1 = reinterpret_cast<const tw2*>(P.bhat_mod_q) + (size_t)i * K;
     const tw2* c2 = reinterpret_cast<const tw2*>(P.bhat_mod_q) + (size_t)i2 * K;
-    const uint64_t t1 = d1.quirk_thr < d1.p ? d1.quirk_thr : d1.p;
-    const uint64_t t2 = d2.quirk_thr < d2.p ? d2.quirk_thr : d2.p;
+    const uint64_t t1 = lazy_window(d1, P.moddown_exact);
+    const uint64_t t2 = lazy_window(d2, P.moddown_exact);
     uint64_t a1 = 0, a2 = 0;
 #pragma unroll 2
     for (int j = 0; j < K; ++j) {
@@ -331,6 +337,25 @@ void launch_rescale(const DevParams& P, uint64_t* c0, uint64_t* c1, int level, h
   if (level <= 0) return;
   const size_t total = (size_t)level * P.N;
   k_rescale<<<dim3((unsigned)(total / 256), 2), 256, 0, st>>>(P, c0, c1, level);
+}
+
+// ------------------------------------------------------------------- rho ----
+// Q limbs times (B mod q_i), special limbs zero (aoa-ntt/src/cuda/manager.cu:149-160,
+// where d_RNSBqi is zero past the Q limbs).
+__global__ __launch_bounds__(256) void k_rho(DevParams P, uint64_t* out, const uint64_t* in) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int li = (int)(((size_t)blockIdx.x * 256) >> P.logN);
+  if (li >= P.k) {
+    out[e] = 0;
+    return;
+  }
+  const tw2 b = reinterpret_cast<const tw2*>(P.b_mod_q)[li];
+  out[e] = mul_shoup(in[e], b.w, b.q, P.primes[li].p);
+}
+
+void launch_rho(const DevParams& P, uint64_t* out, const uint64_t* in, hipStream_t st) {
+  const size_t total = (size_t)(P.k + P.K) * P.N;
+  k_rho<<<dim3((unsigned)(total / 256)), 256, 0, st>>>(P, out, in);
 }
 
 // ---------------------------------------------------------- automorphism ----

@@ -15,6 +15,17 @@ extern "C" int aoa_params_set_ntt_split(aoa_params* h, int n1) {
   return AOA_OK;
 }
 
+extern "C" int aoa_params_set_moddown_exact(aoa_params* h, int exact) {
+  if (!h) return set_error(AOA_ERR_ARG, "null params");
+  h->dev.moddown_exact = exact ? 1 : 0;
+  return AOA_OK;
+}
+
+extern "C" int aoa_params_get_moddown_exact(const aoa_params* h) {
+  if (!h) return -set_error(AOA_ERR_ARG, "null params");
+  return h->dev.moddown_exact;
+}
+
 extern "C" int aoa_params_get_ntt_split(const aoa_params* h) {
   if (!h) return -set_error(AOA_ERR_ARG, "null params");
   return h->dev.ntt_n1 ? h->dev.ntt_n1 : ntt_default_n1(h->host.N);

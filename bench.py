@@ -92,6 +92,73 @@ def plain_fn(P, evk, items, ws_cache):
     return run
 
 
+def serving_bench(args, rng):
+    """Config 5: Poisson arrivals; ring dimension uniform in {2^13..2^16} with
+    {4, 8, 16, 30} limbs; each request is 1-5 HomMul+relin+rescale (at most
+    its level) and 0-4 rotations by power-of-two steps, in random order.
+    Keys stay resident; admission by an HBM pool budget."""
+    import torch
+    from aoa_amd import Params, Server
+    shapes = [(8192, 4), (16384, 8), (32768, 16), (65536, 30)]
+    steps = [1, 2, 4, 8, 16]
+    srv = Server(pool_bytes=24 << 30, step_ws_bytes=24 << 30, max_active=256)
+    sets = []
+    for N_, k_ in shapes:
+        P_ = Params(N_, k_, 0, PREC)
+        evk_ = rand_polys(P_, 2, P_.k + P_.kl)
+        rk = rand_polys(P_, 2 * len(steps), P_.k + P_.kl)
+        rtk_ = {s: (rk[2 * i], rk[2 * i + 1]) for i, s in enumerate(steps)}
+        ks = srv.register(P_, evk_, rtk_)
+        ins = rand_polys(P_, 8, P_.k)  # 4 input ciphertexts and 4 operands per shape
+        sets.append((P_, ks, ins, evk_, rtk_))
+    reqs = []
+    for _ in range(args.requests):
+        s = int(rng.integers(0, len(shapes)))
+        P_ = sets[s][0]
+        nmul = int(rng.integers(1, 6))
+        nmul = min(nmul, P_.k - 1)
+        nrot = int(rng.integers(0, 5))
+        ops = [("mul",)] * nmul + [("rot", int(rng.choice(steps)))] * nrot
+        order = rng.permutation(len(ops))
+        reqs.append((s, [ops[i] for i in order]))
+    gaps = rng.exponential(1.0 / args.rate, size=args.requests)
+    outs = [[sets[s][0].zeros(sets[s][0].k), sets[s][0].zeros(sets[s][0].k)] for s, _ in reqs]
+    torch.cuda.synchronize()
+    srv.start()
+    ids = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for (s, ops), out, gap in zip(reqs, outs, gaps):
+        t_next += gap
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        P_, ks, ins, _, _ = sets[s]
+        j = int(rng.integers(0, 4))
+        ids.append(srv.submit(ks, (ins[2 * j], ins[2 * j + 1]), P_.k - 1, True, (ins[2 * j + 1], ins[2 * j]), ops,
+                              out))
+    deadline = time.perf_counter() + 300
+    while time.perf_counter() < deadline:
+        if all(srv.result(i)[0] >= Server.DONE for i in ids):
+            break
+        time.sleep(0.005)
+    srv.stop()
+    res = [srv.result(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r[2] for r in res if r[0] == Server.DONE])
+    nops = sum(len(o) for _, o in reqs)
+    return {"requests": args.requests, "completed": int(st["completed"]), "failed": int(sum(r[0] == Server.FAILED
+                                                                                         for r in res)),
+            "arrival_rate_per_s": args.rate, "ops": int(st["ops"]), "steps": int(st["steps"]),
+            "ops_per_s": round(st["ops"] / (st["span_ms"] / 1000.0), 1) if st["span_ms"] > 0 else None,
+            "requests_per_s": round(st["completed"] / (st["span_ms"] / 1000.0), 1) if st["span_ms"] > 0 else None,
+            "p50_ms": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
+            "p99_ms": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
+            "mean_ops_per_request": round(nops / args.requests, 2),
+            "resident_key_bytes": int(st["key_bytes"])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +167,9 @@ def main():
     ap.add_argument("--batch", type=int, default=32, help="ciphertext pairs per batched step")
     ap.add_argument("--cpu-sample", type=int, default=1, help="HomMuls timed on the CPU oracle (0: skip)")
     ap.add_argument("--no-c3", action="store_true", help="skip the config-3 comparison")
+    ap.add_argument("--no-serving", action="store_true", help="skip the config-5 serving run")
+    ap.add_argument("--requests", type=int, default=600, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=300.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU baseline, no C3)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -207,6 +277,10 @@ def main():
                      "batched_over_plain": round(p3_ms / b3_ms, 3),
                      "batched_roofline_frac": round(mean_bytes * 256 / (b3_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
         del b3, it3
+        torch.cuda.empty_cache()
+
+    if not args.no_serving and not args.profile:
+        res["serving"] = serving_bench(args, rng)
         torch.cuda.empty_cache()
 
     if args.cpu_sample > 0 and not args.profile:

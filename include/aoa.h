@@ -125,6 +125,18 @@ int aoa_hommul_relin_rescale(const aoa_params* params, void* stream, uint64_t* c
                              const uint64_t* c2_1, const uint64_t* evk_b, const uint64_t* evk_a, int level,
                              void* workspace, size_t ws_bytes);
 
+/* HomMul + relinearisation without the rescale: outputs in the coefficient
+ * domain, limbs 0..level (level unchanged).  Replaces ckks_mul_without_rescale
+ * (aoa-ntt/src/ckks/ckks.cpp:273-332). */
+int aoa_hommul_relin(const aoa_params* params, void* stream, uint64_t* c3_0, uint64_t* c3_1, const uint64_t* c1_0,
+                     const uint64_t* c1_1, const uint64_t* c2_0, const uint64_t* c2_1, const uint64_t* evk_b,
+                     const uint64_t* evk_a, int level, void* workspace, size_t ws_bytes);
+
+/* rho of a QB polynomial (k+kl limbs): Q limbs multiplied by B mod q_i,
+ * special limbs set to zero, any domain.  Replaces CUDAManager::execute_rho
+ * (aoa-ntt/src/cuda/manager.cu:149-176), used by switching-key generation. */
+int aoa_rho(const aoa_params* params, void* stream, uint64_t* out, const uint64_t* in);
+
 /* Rotation by rot_slots slots to the left: automorphism of both polys
  * (coefficient domain input) + key switch of c1 with rtk (NTT domain QB key).
  * Output coefficient domain, limbs 0..level.  Replaces ckks_rotate_left
@@ -140,6 +152,17 @@ int aoa_rotate_left(const aoa_params* params, void* stream, uint64_t* out0, uint
  * split gives the same residues).  Set before issuing work with the handle. */
 int aoa_params_set_ntt_split(aoa_params* params, int n1);
 int aoa_params_get_ntt_split(const aoa_params* params);
+
+/* ModDown reduction mode.  0 (default): reproduce the reference's Barrett
+ * reduction of the cross-base products (aoa-ntt/src/cuda/manager.cu:331-334 with
+ * aoa-ntt/src/cuda/htrans/common.cu:257-291), so residues are bit-identical to
+ * the reference's; that reduction returns exact - (2^nb - q_i) for a small
+ * fraction of products when the special primes are wider than q_i (prec 48 and
+ * 52 layouts) and can make decryption fail, as it does in the reference.
+ * 1: exact reduction (no counterpart in the reference; residues then differ
+ * from the reference's exactly where its reduction deviates). */
+int aoa_params_set_moddown_exact(aoa_params* params, int exact);
+int aoa_params_get_moddown_exact(const aoa_params* params);
 
 /* Autotuner: times forward+inverse NTTs of `nlimbs` limbs for every valid
  * split (reps repetitions each, on the given stream, allocating a scratch

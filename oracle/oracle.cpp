@@ -64,6 +64,9 @@ struct Oracle {
   std::map<int, std::vector<uint64_t>> gjk, invgjk;  // [R][log S][S]
   std::vector<uint64_t> gN, ginvN;                   // [R][N]
   std::vector<uint64_t> InvQqi, Qqibi, Bbiqi, InvBbi, InvModBqi, Bqi, Invqlqi;
+  // Not in the reference: when set, ModDown reduces its cross-base products
+  // exactly instead of with the reference's Barrett (the product's exact mode).
+  bool exact_moddown = false;
   int R() const { return k + kl; }
 };
 
@@ -511,7 +514,9 @@ static void moddown_kernel(const Oracle& o, uint64_t* out, const uint64_t* qb, i
       uint64_t val1 = 0;
       for (int rid_B = 0; rid_B < kl; rid_B++) {
         const uint64_t aux1 = mulmod(o, qb[cid + (size_t)(rid_B + k) * N], o.InvBbi[rid_B], rid_B + k);
-        const uint64_t tmp = mulmod(o, aux1, o.Bbiqi[(size_t)rid_Q * kl + rid_B], rid_Q);
+        const uint64_t c = o.Bbiqi[(size_t)rid_Q * kl + rid_B];
+        const uint64_t tmp = o.exact_moddown ? (uint64_t)(((unsigned __int128)aux1 * c) % o.P[rid_Q])
+                                             : mulmod(o, aux1, c, rid_Q);
         val1 = addmod(o, val1, tmp, rid_Q);
       }
       const uint64_t tmp = submod(o, qb[cid + (size_t)rid_Q * N], val1, rid_Q);
@@ -604,6 +609,7 @@ int orc_primes(void* h, uint64_t* out) {
   for (int r = 0; r < o->R(); ++r) out[r] = o->P[r];
   return o->R();
 }
+void orc_set_moddown_exact(void* h, int exact) { ((Oracle*)h)->exact_moddown = exact != 0; }
 int orc_roots(void* h, uint64_t* out) {
   Oracle* o = (Oracle*)h;
   for (int r = 0; r < o->R(); ++r) out[r] = o->G[r];

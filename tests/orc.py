@@ -34,6 +34,7 @@ def load():
         "orc_set_threads": (ctypes.c_int, [ctypes.c_int]),
         "orc_primes": (ctypes.c_int, [_vp, _u64p]),
         "orc_roots": (ctypes.c_int, [_vp, _u64p]),
+        "orc_set_moddown_exact": (None, [_vp, ctypes.c_int]),
         "orc_factor": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "orc_table": (ctypes.c_int64, [_vp, ctypes.c_int, _u64p]),
         "orc_mod128": (ctypes.c_uint64, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]),
@@ -81,6 +82,9 @@ class Oracle:
     @staticmethod
     def threads(n=0):
         return load().orc_set_threads(n)
+
+    def set_moddown_exact(self, exact):
+        self.lib.orc_set_moddown_exact(self.h, int(bool(exact)))
 
     def table(self, which):
         n = self.lib.orc_table(self.h, which, None)

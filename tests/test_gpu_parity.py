@@ -143,3 +143,30 @@ def test_hommul_c2_parity(gpu):
     n = level * N
     assert np.array_equal(P.to_host(out[0])[:n], ref0[:n])
     assert np.array_equal(P.to_host(out[1])[:n], ref1[:n])
+
+
+@pytest.mark.parametrize("shape", [(8192, 6, 0, 48), (65536, 30, 0, 48)])
+def test_hommul_exact_moddown_parity(gpu, shape):
+    """Exact ModDown mode (not in the reference) against the oracle's exact mode;
+    the default mode stays bit-exact with the reference's reduction."""
+    P, O = _params(*shape)
+    P.set_moddown_exact(True)
+    O.set_moddown_exact(True)
+    rng = np.random.default_rng(71)
+    k, kl, N = O.k, O.kl, O.N
+    level = k - 1
+    c1 = [O.random_poly(rng, k) for _ in range(2)]
+    c2 = [O.random_poly(rng, k) for _ in range(2)]
+    evk = [O.random_poly(rng, k + kl) for _ in range(2)]
+    ref0, ref1 = O.hommul(c1, c2, evk, level)
+    d = [P.to_device(x) for x in c1 + c2 + evk]
+    out = [P.zeros(k), P.zeros(k)]
+    P.hommul(out, (d[0], d[1]), (d[2], d[3]), (d[4], d[5]), level, P.workspace(level))
+    n = level * N
+    assert np.array_equal(P.to_host(out[0])[:n], ref0[:n])
+    assert np.array_equal(P.to_host(out[1])[:n], ref1[:n])
+    # the two modes differ somewhere at this shape (the reference's reduction deviates)
+    O.set_moddown_exact(False)
+    r0, _ = O.hommul(c1, c2, evk, level)
+    if shape[1] >= 30:
+        assert not np.array_equal(r0[:n], ref0[:n])
