@@ -9,6 +9,7 @@
 // the bookkeeping (levels, scales, domains).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "../../../include/newckks/newckks.h"
+#include "fft.hpp"
 
 // ------------------------------------------------------------ helpers ------
 
@@ -376,61 +378,10 @@ void cipher_copy(Context* ctx, cipher_t* b, cipher_t* a) {
 }
 
 // ------------------------------------------------------------ encoder ------
-// Canonical embedding of a slot vector of `size` complex values:
-// decode z_j = sum_{s<size} u_s w^(s 5^j) with w = exp(2 pi i / (4 size)); encode
-// is its exact inverse u_s = (1/size) sum_j z_j w^(-s 5^j).  Both are computed
-// with one radix-2 FFT of length M = 4 size over the full circle of M-th
-// roots, reading or writing only the points w^(5^j mod M).
 
-static void fft_radix2(std::vector<std::complex<double>>& a, int sign) {
-  const size_t n = a.size();
-  for (size_t i = 1, j = 0; i < n; ++i) {
-    size_t bit = n >> 1;
-    for (; j & bit; bit >>= 1) j ^= bit;
-    j ^= bit;
-    if (i < j) std::swap(a[i], a[j]);
-  }
-  for (size_t len = 2; len <= n; len <<= 1) {
-    const double ang = sign * 2.0 * M_PI / (double)len;
-    std::vector<std::complex<double>> w(len / 2);
-    for (size_t k = 0; k < len / 2; ++k) w[k] = std::complex<double>(std::cos(ang * k), std::sin(ang * k));
-    for (size_t i = 0; i < n; i += len)
-      for (size_t k = 0; k < len / 2; ++k) {
-        std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w[k];
-        a[i + k] = u + v;
-        a[i + k + len / 2] = u - v;
-      }
-  }
-}
+void fftSpecial(std::complex<double>* vals, const long size) { aoa_fft::special_fft(vals, size); }
 
-static std::vector<size_t> rot_points(long size) {
-  const size_t M = 4 * (size_t)size;
-  std::vector<size_t> pts(size);
-  size_t g = 1;
-  for (long j = 0; j < size; ++j) {
-    pts[j] = g;
-    g = (g * 5) % M;
-  }
-  return pts;
-}
-
-void fftSpecial(std::complex<double>* vals, const long size) {
-  const size_t M = 4 * (size_t)size;
-  std::vector<std::complex<double>> a(M, 0.0);
-  for (long s = 0; s < size; ++s) a[s] = vals[s];
-  fft_radix2(a, +1);  // a[t] = sum_s u_s w^(s t)
-  const std::vector<size_t> pts = rot_points(size);
-  for (long j = 0; j < size; ++j) vals[j] = a[pts[j]];
-}
-
-void fftSpecialInv(std::complex<double>* vals, const long size) {
-  const size_t M = 4 * (size_t)size;
-  std::vector<std::complex<double>> a(M, 0.0);
-  const std::vector<size_t> pts = rot_points(size);
-  for (long j = 0; j < size; ++j) a[pts[j]] = vals[j];
-  fft_radix2(a, -1);  // a[s] = sum_t W_t w^(-s t)
-  for (long s = 0; s < size; ++s) vals[s] = a[s] / (double)size;
-}
+void fftSpecialInv(std::complex<double>* vals, const long size) { aoa_fft::special_fft_inv(vals, size); }
 
 // encoder.cu:143-221: value * scale truncated to int64, reduced per residue,
 // real parts at s*gap and imaginary parts at N/2 + s*gap.

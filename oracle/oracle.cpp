@@ -571,6 +571,72 @@ static long powmod_small(long base, long e, long m) {
   return r;
 }
 
+// ------------------------------------------------ encoder FFT (encoder.cu:10-124)
+static void array_bit_reverse(std::complex<double>* vals, long size) {  // encoder.cu:12-23
+  for (long i = 1, j = 0; i < size; ++i) {
+    long bit = size >> 1;
+    for (; j >= bit; bit >>= 1) j -= bit;
+    j += bit;
+    if (i < j) std::swap(vals[i], vals[j]);
+  }
+}
+// fftSpecial (encoder.cu:33-73): Nh = ring dimension N, M = 4N.
+static void fft_special(std::complex<double>* vals, long size, long ringN) {
+  const long Nh = ringN, M = 4 * ringN;
+  std::vector<long> rotGroup(Nh);
+  long g = 1;
+  for (long i = 0; i < Nh; ++i) {
+    rotGroup[i] = g;
+    g = (g * 5) % M;
+  }
+  std::vector<std::complex<double>> ksi(M + 1);
+  for (long j = 0; j < M; ++j) {
+    const double angle = 2.0 * M_PI * j / M;
+    ksi[j] = std::complex<double>(std::cos(angle), std::sin(angle));
+  }
+  ksi[M] = ksi[0];
+  array_bit_reverse(vals, size);
+  for (long len = 2; len <= size; len <<= 1)
+    for (long i = 0; i < size; i += len) {
+      const long lenh = len >> 1, lenq = len << 2;
+      for (long j = 0; j < lenh; ++j) {
+        const long idx = (rotGroup[j] % lenq) * M / lenq;
+        std::complex<double> u = vals[i + j], v = vals[i + j + lenh] * ksi[idx];
+        vals[i + j] = u + v;
+        vals[i + j + lenh] = u - v;
+      }
+    }
+}
+// fftSpecialInv (encoder.cu:75-124): N = ring dimension, M = 2N, scaled by 1/size.
+static void fft_special_inv(std::complex<double>* vals, long size, long ringN) {
+  const long Nh = ringN / 2, M = 2 * ringN;
+  std::vector<long> rotGroup(Nh);
+  long g = 1;
+  for (long i = 0; i < Nh; ++i) {
+    rotGroup[i] = g;
+    g = (g * 5) % M;
+  }
+  std::vector<std::complex<double>> ksi(M + 1);
+  for (long j = 0; j < M; ++j) {
+    const double angle = 2.0 * M_PI * j / M;
+    ksi[j] = std::complex<double>(std::cos(angle), std::sin(angle));
+  }
+  ksi[M] = ksi[0];
+  for (long len = size; len >= 1; len >>= 1)
+    for (long i = 0; i < size; i += len) {
+      const long lenh = len >> 1, lenq = len << 2;
+      for (long j = 0; j < lenh; ++j) {
+        const long idx = (lenq - (rotGroup[j] % lenq)) * M / lenq;
+        std::complex<double> u = vals[i + j] + vals[i + j + lenh];
+        std::complex<double> v = (vals[i + j] - vals[i + j + lenh]) * ksi[idx];
+        vals[i + j] = u;
+        vals[i + j + lenh] = v;
+      }
+    }
+  array_bit_reverse(vals, size);
+  for (long i = 0; i < size; ++i) vals[i] /= (double)size;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -610,6 +676,14 @@ int orc_primes(void* h, uint64_t* out) {
   return o->R();
 }
 void orc_set_moddown_exact(void* h, int exact) { ((Oracle*)h)->exact_moddown = exact != 0; }
+// Encoder FFT on interleaved (re, im) doubles; inverse = encoding direction.
+void orc_fft_special(double* v, long size, long ringN, int inverse) {
+  auto* c = reinterpret_cast<std::complex<double>*>(v);
+  if (inverse)
+    fft_special_inv(c, size, ringN);
+  else
+    fft_special(c, size, ringN);
+}
 int orc_roots(void* h, uint64_t* out) {
   Oracle* o = (Oracle*)h;
   for (int r = 0; r < o->R(); ++r) out[r] = o->G[r];

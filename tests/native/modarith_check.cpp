@@ -5,6 +5,7 @@
 #include <random>
 
 #include "modarith.hpp"
+#include "newckks/fft.hpp"
 #include "params.hpp"
 
 using namespace aoa;
@@ -97,6 +98,15 @@ int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t s
   out[1] = quirks;
   out[2] = slow;
   return 0;
+}
+
+// The product's encoder FFT on interleaved (re, im) doubles.
+void chk_fft_special(double* v, long size, int inverse) {
+  auto* c = reinterpret_cast<std::complex<double>*>(v);
+  if (inverse)
+    aoa_fft::special_fft_inv(c, size);
+  else
+    aoa_fft::special_fft(c, size);
 }
 
 // The product's host tables (same numbering as aoa_params_table), built

@@ -35,6 +35,7 @@ def load():
         "orc_primes": (ctypes.c_int, [_vp, _u64p]),
         "orc_roots": (ctypes.c_int, [_vp, _u64p]),
         "orc_set_moddown_exact": (None, [_vp, ctypes.c_int]),
+        "orc_fft_special": (None, [ctypes.POINTER(ctypes.c_double), ctypes.c_long, ctypes.c_long, ctypes.c_int]),
         "orc_factor": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "orc_table": (ctypes.c_int64, [_vp, ctypes.c_int, _u64p]),
         "orc_mod128": (ctypes.c_uint64, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]),
