@@ -93,6 +93,8 @@ SIGNATURES.update({
     "aoa_batch_workspace_bytes": (ctypes.c_size_t, [c_vp]),
     "aoa_batch_launch": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int]),
     "aoa_batch_destroy": (None, [c_vp]),
+    "aoa_batch_stage_times": (ctypes.c_int, [c_vp, c_vp, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
 })
 
 _lib = None

@@ -190,6 +190,17 @@ class Batch:
     def launch(self, stream=None, graph=True):
         check(self.lib.aoa_batch_launch(self.h, _stream(stream), 1 if graph else 0))
 
+    STAGES = ("pre_fwd", "pre_inv", "tensor", "automorphism", "intt", "modup", "ntt", "key_intt", "moddown")
+
+    def stage_times(self, stream=None):
+        """One eager run with HIP events around every stage launch; returns
+        {stage: (milliseconds summed over launches, launches)}."""
+        n = len(self.STAGES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        check(self.lib.aoa_batch_stage_times(self.h, _stream(stream), ms, cnt, n))
+        return {s: (ms[i], cnt[i]) for i, s in enumerate(self.STAGES) if cnt[i]}
+
     def close(self):
         if self.h:
             self.lib.aoa_batch_destroy(self.h)

@@ -68,6 +68,7 @@ struct aoa_batch {
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  BatchLayer* layers_cur = nullptr;  // layer being enqueued
 };
 
 namespace aoa {
@@ -240,34 +241,74 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
   return overflow ? 0 : used;
 }
 
-// Issues every stage launch of the batch on stream st.
-int batch_enqueue(aoa_batch* b, hipStream_t st) {
-  for (auto& L : b->layers) {
-    // stage order inside a layer; groups of the same stage back to back
-    for (auto& g : L.groups)
-      if (g.n_pre_fwd && ntt_forward_desc(g.params->dev, g.d_pre_fwd, g.n_pre_fwd, 0, st))
-        return set_error(AOA_ERR_UNSUPPORTED, "no NTT split");
-    for (auto& g : L.groups)
-      if (g.n_pre_inv && ntt_inverse_desc(g.params->dev, g.d_pre_inv, g.n_pre_inv, 0, st))
-        return set_error(AOA_ERR_UNSUPPORTED, "no NTT split");
-    for (auto& g : L.groups)
-      if (g.n_tensor) launch_tensor_desc(g.params->dev, g.d_tensor, g.n_tensor, st);
-    for (auto& g : L.groups)
-      if (g.n_auto) launch_automorphism_desc(g.params->dev, g.d_auto, g.n_auto, st);
-    for (auto& g : L.groups)
-      if (g.n_intt3 && ntt_inverse_desc(g.params->dev, g.d_intt3, g.n_intt3, 0, st))
-        return set_error(AOA_ERR_UNSUPPORTED, "no NTT split");
-    for (auto& g : L.groups)
-      if (g.n_modup) launch_modup_desc(g.params->dev, g.d_modup, g.n_modup, g.max_level, st);
-    for (auto& g : L.groups)
-      if (g.n_fwd && ntt_forward_desc(g.params->dev, g.d_fwd, g.n_fwd, 0, st))
-        return set_error(AOA_ERR_UNSUPPORTED, "no NTT split");
-    for (auto& g : L.groups)
-      if (g.n_key && ntt_inverse_desc(g.params->dev, g.d_key, g.n_key, 0, st))
-        return set_error(AOA_ERR_UNSUPPORTED, "no NTT split");
-    for (auto& g : L.groups)
-      if (g.n_moddown) launch_moddown_desc(g.params->dev, g.d_moddown, g.n_moddown, st);
+// Records a HIP event pair around each stage launch of batch_enqueue (on the
+// stream the stage is launched on); used only by aoa_batch_stage_times.
+struct StageTimer {
+  std::vector<int> stage;
+  std::vector<hipEvent_t> ev;  // begin, end pairs
+  bool ok = true;
+  void begin(int s, hipStream_t st) {
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) { ok = false; return; }
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); ok = false; return; }
+    stage.push_back(s);
+    ev.push_back(e0);
+    ev.push_back(e1);
+    (void)hipEventRecord(e0, st);
   }
+  void end(hipStream_t st) {
+    if (ok) (void)hipEventRecord(ev.back(), st);
+  }
+  ~StageTimer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+// Issues every stage launch of the batch on stream st (stage order
+// AOA_STAGE_*); with a timer, each (layer, stage) launch run is bracketed by
+// an event pair.
+int batch_enqueue(aoa_batch* b, hipStream_t st, StageTimer* tm) {
+  int rc = AOA_OK;
+  auto run = [&](int s, auto&& has, auto&& go) {
+    bool any = false;
+    for (auto& g : b->layers_cur->groups) any = any || has(g);
+    if (!any || rc) return;
+    if (tm) tm->begin(s, st);
+    for (auto& g : b->layers_cur->groups)
+      if (has(g) && go(g)) rc = set_error(AOA_ERR_UNSUPPORTED, "no NTT split");
+    if (tm) tm->end(st);
+  };
+  for (auto& L : b->layers) {
+    b->layers_cur = &L;
+    run(AOA_STAGE_PRE_FWD, [](BatchGroup& g) { return g.n_pre_fwd > 0; },
+        [&](BatchGroup& g) { return ntt_forward_desc(g.params->dev, g.d_pre_fwd, g.n_pre_fwd, 0, st); });
+    run(AOA_STAGE_PRE_INV, [](BatchGroup& g) { return g.n_pre_inv > 0; },
+        [&](BatchGroup& g) { return ntt_inverse_desc(g.params->dev, g.d_pre_inv, g.n_pre_inv, 0, st); });
+    run(AOA_STAGE_TENSOR, [](BatchGroup& g) { return g.n_tensor > 0; }, [&](BatchGroup& g) {
+      launch_tensor_desc(g.params->dev, g.d_tensor, g.n_tensor, st);
+      return 0;
+    });
+    run(AOA_STAGE_AUTOMORPHISM, [](BatchGroup& g) { return g.n_auto > 0; }, [&](BatchGroup& g) {
+      launch_automorphism_desc(g.params->dev, g.d_auto, g.n_auto, st);
+      return 0;
+    });
+    run(AOA_STAGE_INTT, [](BatchGroup& g) { return g.n_intt3 > 0; },
+        [&](BatchGroup& g) { return ntt_inverse_desc(g.params->dev, g.d_intt3, g.n_intt3, 0, st); });
+    run(AOA_STAGE_MODUP, [](BatchGroup& g) { return g.n_modup > 0; }, [&](BatchGroup& g) {
+      launch_modup_desc(g.params->dev, g.d_modup, g.n_modup, g.max_level, st);
+      return 0;
+    });
+    run(AOA_STAGE_NTT, [](BatchGroup& g) { return g.n_fwd > 0; },
+        [&](BatchGroup& g) { return ntt_forward_desc(g.params->dev, g.d_fwd, g.n_fwd, 0, st); });
+    run(AOA_STAGE_KEY_INTT, [](BatchGroup& g) { return g.n_key > 0; },
+        [&](BatchGroup& g) { return ntt_inverse_desc(g.params->dev, g.d_key, g.n_key, 0, st); });
+    run(AOA_STAGE_MODDOWN, [](BatchGroup& g) { return g.n_moddown > 0; }, [&](BatchGroup& g) {
+      launch_moddown_desc(g.params->dev, g.d_moddown, g.n_moddown, st);
+      return 0;
+    });
+  }
+  b->layers_cur = nullptr;
+  if (rc) return rc;
   return check_launch("aoa_batch");
 }
 
@@ -397,13 +438,13 @@ extern "C" int aoa_batch_launch(aoa_batch* b, void* stream, int use_graph) {
   int rc = aoa_batch_build(b);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
-  if (!use_graph) return batch_enqueue(b, st);
+  if (!use_graph) return batch_enqueue(b, st, nullptr);
   if (!b->exec) {
     if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess)
       return set_error(AOA_ERR_HIP, "hipStreamCreate failed");
     if (hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return set_error(AOA_ERR_HIP, "hipStreamBeginCapture failed");
-    int erc = batch_enqueue(b, b->cap);
+    int erc = batch_enqueue(b, b->cap, nullptr);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->cap, &g);
     if (erc) {
@@ -416,5 +457,29 @@ extern "C" int aoa_batch_launch(aoa_batch* b, void* stream, int use_graph) {
       return set_error(AOA_ERR_HIP, "hipGraphInstantiate failed");
   }
   if (hipGraphLaunch(b->exec, st) != hipSuccess) return set_error(AOA_ERR_HIP, "hipGraphLaunch failed");
+  return AOA_OK;
+}
+
+extern "C" int aoa_batch_stage_times(aoa_batch* b, void* stream, double* ms, int* launches, int nstages) {
+  if (!b || !ms || !launches || nstages < AOA_NSTAGES) return set_error(AOA_ERR_ARG, "bad argument");
+  int rc = aoa_batch_build(b);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  StageTimer tm;
+  rc = batch_enqueue(b, st, &tm);
+  if (!tm.ok) return set_error(AOA_ERR_HIP, "hipEventCreate failed");
+  if (rc) return rc;
+  if (hipStreamSynchronize(st) != hipSuccess) return set_error(AOA_ERR_HIP, "hipStreamSynchronize failed");
+  for (int s = 0; s < nstages; ++s) {
+    ms[s] = 0.0;
+    launches[s] = 0;
+  }
+  for (size_t i = 0; i < tm.stage.size(); ++i) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, tm.ev[2 * i], tm.ev[2 * i + 1]) != hipSuccess)
+      return set_error(AOA_ERR_HIP, "hipEventElapsedTime failed");
+    ms[tm.stage[i]] += t;
+    launches[tm.stage[i]] += 1;
+  }
   return AOA_OK;
 }

@@ -27,7 +27,8 @@ uint64_t galois_of(int rot_slots, int N);
 // batch every step from preallocated workspace and descriptor buffers).
 void batch_sizes(const aoa_batch* b, size_t* ws_bytes, size_t* desc_bytes);
 size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc_cap);
-int batch_enqueue(aoa_batch* b, hipStream_t st);
+struct StageTimer;
+int batch_enqueue(aoa_batch* b, hipStream_t st, StageTimer* tm = nullptr);
 int batch_add_mul(aoa_batch* b, const aoa_params* p, const uint64_t* evk_b, const uint64_t* evk_a,
                   const aoa_mul_item* items, int n);
 int batch_add_rotate(aoa_batch* b, const aoa_params* p, const aoa_rot_item* items, int n);

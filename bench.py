@@ -1,20 +1,26 @@
 """Benchmark: HomMul+relin+rescale ops/s at N=2^16 on one MI355X, batched vs
 plain, with p50/p99 latency.
 
-Headline workload (configs[1] shape): N = 2^16, k = 30 Q limbs (63-bit q0 +
-29 x 48-bit, prec 48), K = 31 special limbs (55-bit), from the top level 29,
-uniform random residues resident in HBM (synthetic ciphertexts and
-evaluation key of that shape).  A step of the batched form is one hipGraph
-replay of B HomMul+relin+rescale; a step of the plain form is one
-aoa_hommul_relin_rescale call.  `value` is the batched throughput; the plain
-throughput, their ratio, mixed-level and config-3 comparisons, the NTT
-roofline and the CPU oracle baseline are reported beside it.
+Headline workload: N = 2^16, k = 30 Q limbs (63-bit q0 + 29 x 48-bit, prec
+48), K = 31 special limbs (55-bit), from the top level 29, uniform random
+residues resident in HBM (synthetic ciphertexts and evaluation key of that
+shape).  A step of the batched form is one hipGraph replay of B
+HomMul+relin+rescale; a step of the plain form is one aoa_hommul_relin_rescale
+call.  `value` is the batched throughput.  Reported beside it: the plain
+throughput and latency, the ratio, the per-stage time split, the roofline of
+the dominant kernel (ModDown), the forward-NTT bandwidth, mixed levels at
+N=2^16, 256 mixed-level ciphertexts at N=2^15, rotate-and-sum over 2^15
+slots for 64 ciphertexts at N=2^16, the serving loop, and the CPU oracle.
 
-Prints one JSON line.
+Prints one JSON line.  --profile runs only the headline shape (plain +
+batched + stage timing) and a 1 GiB device copy used to calibrate the
+FETCH_SIZE / WRITE_SIZE counters, so that a profiler's per-kernel averages
+cover exactly the launches the roofline figure is computed from.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -25,6 +31,11 @@ sys.path.insert(0, ROOT)
 
 N, K_Q, K_B, PREC = 65536, 30, 31, 48
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_moddown_d launch at 32 pairs, from two rocprofv3 --pmc runs of
+# `bench.py --profile` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0; the factors make a
+# 1 GiB device copy read 1 GiB and write 1 GiB); the run is summarised in
+# profiles/moddown_pmc_traffic_n16_k30.md.
+MODDOWN_TRAFFIC_BYTES = {32: 4027514232}
 
 
 def op_bytes(N, L, K):
@@ -32,6 +43,19 @@ def op_bytes(N, L, K):
     two input ciphertexts (4L limbs), the evaluation key on the active Q limbs
     and all special limbs (2(L+K)), the rescaled output (2(L-1))."""
     return 8 * N * (8 * L + 2 * K - 2)
+
+
+def rot_bytes(N, L, K):
+    """One rotation (key switch) at L limbs: input ct (2L), rotation key on
+    the active Q limbs and the special limbs (2(L+K)), output ct (2L)."""
+    return 8 * N * (6 * L + 2 * K)
+
+
+def moddown_bytes(N, L, K, rescale):
+    """One ModDown of a QB polynomial with the HomMul epilogue fused: reads
+    L + K input limbs and the L-limb addend, writes L (or L-1 after the fused
+    rescale) limbs."""
+    return 8 * N * (L + K + L + (L - 1 if rescale else L))
 
 
 def ntt_bytes(N, R):
@@ -56,6 +80,9 @@ def make_items(P, levels):
 
 
 def time_steps(fn, steps, warmup, stream):
+    """Warmup untimed calls, then `steps` calls bracketed by a device
+    synchronisation on both sides; returns (ms per call from HIP events on
+    `stream`, wall seconds)."""
     import torch
     for _ in range(warmup):
         fn()
@@ -92,8 +119,100 @@ def plain_fn(P, evk, items, ws_cache):
     return run
 
 
+def stage_split(bt, stream, reps):
+    """Per-stage milliseconds per batch launch and launches per batch, from
+    `reps` eager runs timed with HIP events around each stage launch."""
+    tot = {}
+    for _ in range(reps):
+        for s, (ms, n) in bt.stage_times(stream).items():
+            a = tot.setdefault(s, [0.0, 0])
+            a[0] += ms
+            a[1] += n
+    return {s: (v[0] / reps, v[1] // reps) for s, v in tot.items()}
+
+
+def mixed_n15(args, rng, stream):
+    """256 ciphertext pairs at N=2^15, k=20 (prec 48), K=21, levels uniform in
+    1..19 (2..20 active limbs): plain calls vs one batched graph replay."""
+    import torch
+    from aoa_amd import Batch, Params
+    P3 = Params(32768, 20, 0, PREC)
+    evk3 = rand_polys(P3, 2, P3.k + P3.kl)
+    lv3 = [int(x) for x in rng.integers(1, 20, size=256)]
+    it3 = make_items(P3, lv3)
+    ws3 = {}
+    p3_ms, _ = time_steps(plain_fn(P3, evk3, it3, ws3), 3, 1, stream)
+    b3 = Batch()
+    b3.add_mul(P3, evk3, it3)
+    b3.build()
+    b3_ms, _ = time_steps(lambda: b3.launch(graph=True), 5, 2, stream)
+    mean_bytes = float(np.mean([op_bytes(32768, lv + 1, P3.kl) for lv in lv3]))
+    del b3, it3
+    torch.cuda.empty_cache()
+    return {"workload": "256 HomMul+relin+rescale, N=2^15, k=20, K=21, levels uniform in 1..19",
+            "plain_ops_s": round(256 * 1000.0 / p3_ms, 2), "batched_ops_s": round(256 * 1000.0 / b3_ms, 2),
+            "batched_over_plain": round(p3_ms / b3_ms, 3),
+            "batched_hbm_frac": round(mean_bytes * 256 / (b3_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+
+
+def sumslots_n16(args, stream):
+    """Rotate-and-sum over 2^15 slots (15 left rotations by 2^0..2^14, each
+    followed by an add, as ckks_sumslots) for 64 ciphertexts at N=2^16, k=24
+    (prec 48), K=25, top level: 15 layers of 64 fused rotate+add items in one
+    graph, against plain aoa_rotate_left + add calls on 4 ciphertexts."""
+    import torch
+    from aoa_amd import ADD, Batch, Params
+    P4 = Params(65536, 24, 0, PREC)
+    lv = P4.k - 1
+    L = lv + 1
+    nct = 64
+    steps = [1 << i for i in range(15)]
+    rk = rand_polys(P4, 2 * len(steps), P4.k + P4.kl)
+    rtk = {s: (rk[2 * i], rk[2 * i + 1]) for i, s in enumerate(steps)}
+    cts = rand_polys(P4, 2 * nct, P4.k)
+    acc = [P4.zeros(P4.k) for _ in range(2 * nct)]
+    bt = Batch()
+    for si, s in enumerate(steps):
+        items = []
+        for c in range(nct):
+            src = (cts[2 * c], cts[2 * c + 1]) if si == 0 else (acc[2 * c], acc[2 * c + 1])
+            items.append((src, (acc[2 * c], acc[2 * c + 1]), rtk[s], s, lv, True))
+        bt.add_rotate(P4, items)
+        bt.barrier()
+    bt.build()
+    b_ms, _ = time_steps(lambda: bt.launch(graph=True), 3, 1, stream)
+    del bt
+    torch.cuda.empty_cache()
+    ws = P4.workspace(lv)
+    tmp = [P4.zeros(P4.k), P4.zeros(P4.k)]
+    nplain = 4
+
+    def plain():
+        for c in range(nplain):
+            a = (acc[2 * c], acc[2 * c + 1])
+            a[0].copy_(cts[2 * c])
+            a[1].copy_(cts[2 * c + 1])
+            for s in steps:
+                P4.rotate_left(tmp, a, rtk[s], s, lv, ws)
+                P4.eltwise(ADD, a[0], a[0], tmp[0], nres=L)
+                P4.eltwise(ADD, a[1], a[1], tmp[1], nres=L)
+
+    p_ms, _ = time_steps(plain, 2, 1, stream)
+    nrot_b, nrot_p = nct * len(steps), nplain * len(steps)
+    r = {"workload": "rotate-and-sum over 2^15 slots (15 rotations+adds) for 64 ciphertexts, N=2^16, k=24, "
+                     "K=25, level 23",
+         "batched_rotations_s": round(nrot_b * 1000.0 / b_ms, 1),
+         "plain_rotations_s": round(nrot_p * 1000.0 / p_ms, 1),
+         "batched_over_plain": round((nrot_b / b_ms) / (nrot_p / p_ms), 3),
+         "batch_ms": round(b_ms, 3),
+         "batched_hbm_frac": round(rot_bytes(65536, L, P4.kl) * nrot_b / (b_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+    del cts, acc, rk, rtk, tmp, ws
+    torch.cuda.empty_cache()
+    return r
+
+
 def serving_bench(args, rng):
-    """Config 5: Poisson arrivals; ring dimension uniform in {2^13..2^16} with
+    """Poisson arrivals; ring dimension uniform in {2^13..2^16} with
     {4, 8, 16, 30} limbs; each request is 1-5 HomMul+relin+rescale (at most
     its level) and 0-4 rotations by power-of-two steps, in random order.
     Keys stay resident; admission by an HBM pool budget."""
@@ -159,18 +278,66 @@ def serving_bench(args, rng):
             "resident_key_bytes": int(st["key_bytes"])}
 
 
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                return ln.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return None
+
+
+def cpu_baseline(args, P, plain_items, evk, level):
+    """The oracle (the reference's algorithm restated in C++ with OpenMP over
+    limbs and coefficients) on the same HomMul+relin+rescale input, on the
+    host cores: median of `cpu_sample` runs on all threads (at most 16) and of
+    `cpu_sample_1` runs on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from orc import Oracle
+    threads = min(16, os.cpu_count() or 1)
+    O = Oracle(N, K_Q, K_B, PREC)
+    c1, c2, _, _ = plain_items[0]
+    h = [P.to_host(t) for t in list(c1) + list(c2) + evk]
+
+    def run(nthreads, reps):
+        Oracle.threads(nthreads)
+        O.hommul((h[0], h[1]), (h[2], h[3]), (h[4], h[5]), level)  # warm-up
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            O.hommul((h[0], h[1]), (h[2], h[3]), (h[4], h[5]), level)
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
+
+    t_all = run(threads, args.cpu_sample)
+    res = {"value": round(1.0 / t_all, 4), "unit": "ops/s", "cores": threads, "kind": "port",
+           "sample": "%d HomMul+relin+rescale (N=2^16, k=30, K=31, level 29) on the oracle, median; %d threads"
+                     % (args.cpu_sample, threads),
+           "cpu": cpu_model(), "nproc": os.cpu_count()}
+    if args.cpu_sample_1 > 0:
+        t1 = run(1, args.cpu_sample_1)
+        res["single_thread_ops_s"] = round(1.0 / t1, 4)
+        res["single_thread_sample"] = "%d runs, median" % args.cpu_sample_1
+    Oracle.threads(threads)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=32, help="ciphertext pairs per batched step")
-    ap.add_argument("--cpu-sample", type=int, default=1, help="HomMuls timed on the CPU oracle (0: skip)")
-    ap.add_argument("--no-c3", action="store_true", help="skip the config-3 comparison")
-    ap.add_argument("--no-serving", action="store_true", help="skip the config-5 serving run")
+    ap.add_argument("--cpu-sample", type=int, default=10, help="HomMuls timed on the CPU oracle, all threads (0: skip)")
+    ap.add_argument("--cpu-sample-1", type=int, default=2, help="HomMuls timed on the CPU oracle, one thread")
+    ap.add_argument("--no-extra", action="store_true", help="skip the N=2^15 and rotate-and-sum workloads")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
     ap.add_argument("--requests", type=int, default=600, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=300.0, help="serving: Poisson arrival rate (requests/s)")
-    ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU baseline, no C3)")
+    ap.add_argument("--profile", action="store_true",
+                    help="headline shape only (plus a 1 GiB copy for counter calibration), for rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every GPU box here has one GPU and this build has no "
@@ -189,14 +356,14 @@ def main():
     evk = rand_polys(P, 2, K_Q + K_B)
     ws_cache = {}
 
-    # ---- plain form at the C2 shape (one ciphertext per call) -------------
+    # ---- plain form (one ciphertext pair per call) -------------------------
     plain_items = make_items(P, [level])
     plain = plain_fn(P, evk, plain_items, ws_cache)
     plain_ms, _ = time_steps(plain, args.steps * 4, args.warmup, stream)
     plain_ops = 1000.0 / plain_ms
     lat = latencies(plain, min(100, args.steps * 2), stream)
 
-    # ---- batched form at the C2 shape --------------------------------------
+    # ---- batched form: the timed region ------------------------------------
     B = args.batch
     items = make_items(P, [level] * B)
     bt = Batch()
@@ -205,26 +372,15 @@ def main():
     batch_ms, wall = time_steps(lambda: bt.launch(graph=True), args.steps, args.warmup, stream)
     batch_ops = B * 1000.0 / batch_ms
     blat = latencies(lambda: bt.launch(graph=True), min(20, args.steps), stream)
+
+    # ---- dominant kernel: ModDown, timed per launch with HIP events --------
+    split = stage_split(bt, stream, reps=5)
+    md_ms, md_n = split["moddown"]
+    md_launch_ms = md_ms / md_n
+    md_bytes = B * 2 * moddown_bytes(N, L, K_B, rescale=True)  # one launch covers all B items, 2 polys each
+    md_gbs = md_bytes / (md_launch_ms * 1e-3) / 1e9
     del bt
-
-    # ---- mixed levels at N=2^16: batched vs plain on the same items --------
-    rng = np.random.default_rng(3)
-    mixed_levels = [int(x) for x in rng.integers(1, K_Q, size=B)]
-    mitems = make_items(P, mixed_levels)
-    mplain_ms, _ = time_steps(plain_fn(P, evk, mitems, ws_cache), max(3, args.steps // 10), 1, stream)
-    mb = Batch()
-    mb.add_mul(P, evk, mitems)
-    mb.build()
-    mbatch_ms, _ = time_steps(lambda: mb.launch(graph=True), max(3, args.steps // 5), 2, stream)
-    del mb, mitems, items
     torch.cuda.empty_cache()
-
-    # ---- dominant kernel roofline: forward NTT of the k+K limbs ------------
-    R = K_Q + K_B
-    x = rand_polys(P, 1, R)[0]
-    nrep = max(20, args.steps)
-    ntt_ms, _ = time_steps(lambda: P.ntt(x, 0, R, 0), nrep, 3, stream)
-    ntt_gbs = ntt_bytes(N, R) / (ntt_ms * 1e-3) / 1e9
 
     res = {
         "metric": "HomMul+relin+rescale ops/s at N=2^16 (batched vs plain); p50/p99 latency",
@@ -238,7 +394,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
-        "data": "synthetic: uniform random residues (ciphertexts and evaluation key of the C2 shape), on device",
+        "data": "synthetic: uniform random residues (ciphertexts and evaluation key, N=2^16, k=30, K=31), on device",
         "config": {"workload": "batched HomMul+relin+rescale, %d ciphertext pairs per step, N=2^16, k=30 "
                                "(prec 48), K=31, level 29" % B,
                    "N": N, "k": K_Q, "K": K_B, "prec": PREC, "level": level, "batch": B},
@@ -248,57 +404,61 @@ def main():
         "p99_ms": round(float(np.percentile(lat, 99)), 4),
         "batch_p50_ms": round(float(np.percentile(blat, 50)), 4),
         "batch_p99_ms": round(float(np.percentile(blat, 99)), 4),
-        "mixed_levels_n16": {"ops": B, "plain_ops_s": round(B * 1000.0 / mplain_ms, 2),
-                             "batched_ops_s": round(B * 1000.0 / mbatch_ms, 2),
-                             "batched_over_plain": round(mplain_ms / mbatch_ms, 3)},
-        "op_roofline_frac": round(op_bytes(N, L, K_B) * batch_ops / (HBM_PEAK_GBS * 1e9), 4),
+        "op_hbm_frac": round(op_bytes(N, L, K_B) * batch_ops / (HBM_PEAK_GBS * 1e9), 4),
+        "stages_ms_per_step": {s: round(v[0], 4) for s, v in split.items()},
         "wall_s": round(wall, 3),
-        "roofline": {"bound": "hbm", "achieved": round(ntt_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ntt_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "forward NTT (column + row pass) of 61 limbs, N=2^16",
-                     "launch_ms": round(ntt_ms, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(md_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(md_gbs / HBM_PEAK_GBS, 4), "traffic": MODDOWN_TRAFFIC_BYTES.get(B),
+                     "kernel": "aoa::k_moddown_d (ModDown + rescale + HomMul additions, %d items x 2 polys)" % B,
+                     "launch_ms": round(md_launch_ms, 4), "algorithmic_bytes": md_bytes},
     }
 
-    # ---- config 3: 256 ciphertexts, N=2^15, 20 limbs, L uniform in [2, 20] --
-    if not args.no_c3 and not args.profile:
-        P3 = Params(32768, 20, 0, 48)
-        evk3 = rand_polys(P3, 2, P3.k + P3.kl)
-        lv3 = [int(x) for x in rng.integers(1, 20, size=256)]
-        it3 = make_items(P3, lv3)
-        ws3 = {}
-        p3_ms, _ = time_steps(plain_fn(P3, evk3, it3, ws3), 3, 1, stream)
-        b3 = Batch()
-        b3.add_mul(P3, evk3, it3)
-        b3.build()
-        b3_ms, _ = time_steps(lambda: b3.launch(graph=True), 5, 2, stream)
-        mean_bytes = float(np.mean([op_bytes(32768, lv + 1, P3.kl) for lv in lv3]))
-        res["c3"] = {"ops": 256, "plain_ops_s": round(256 * 1000.0 / p3_ms, 2),
-                     "batched_ops_s": round(256 * 1000.0 / b3_ms, 2),
-                     "batched_over_plain": round(p3_ms / b3_ms, 3),
-                     "batched_roofline_frac": round(mean_bytes * 256 / (b3_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
-        del b3, it3
-        torch.cuda.empty_cache()
+    if args.profile:
+        # 1 GiB device-to-device copy: the counter calibration kernel
+        a = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        res["calibration_copy_bytes"] = 1 << 30
+        print(json.dumps(res))
+        return
 
-    if not args.no_serving and not args.profile:
+    # ---- mixed levels at N=2^16: batched vs plain on the same items --------
+    rng = np.random.default_rng(3)
+    mixed_levels = [int(x) for x in rng.integers(1, K_Q, size=B)]
+    mitems = make_items(P, mixed_levels)
+    mplain_ms, _ = time_steps(plain_fn(P, evk, mitems, ws_cache), max(3, args.steps // 10), 1, stream)
+    mb = Batch()
+    mb.add_mul(P, evk, mitems)
+    mb.build()
+    mbatch_ms, _ = time_steps(lambda: mb.launch(graph=True), max(3, args.steps // 5), 2, stream)
+    del mb, mitems, items
+    torch.cuda.empty_cache()
+    res["mixed_levels_n16"] = {"ops": B, "plain_ops_s": round(B * 1000.0 / mplain_ms, 2),
+                               "batched_ops_s": round(B * 1000.0 / mbatch_ms, 2),
+                               "batched_over_plain": round(mplain_ms / mbatch_ms, 3)}
+
+    # ---- forward NTT of the k+K limbs (two kernels) ------------------------
+    R = K_Q + K_B
+    x = rand_polys(P, 1, R)[0]
+    ntt_ms, _ = time_steps(lambda: P.ntt(x, 0, R, 0), max(20, args.steps), 3, stream)
+    ntt_gbs = ntt_bytes(N, R) / (ntt_ms * 1e-3) / 1e9
+    res["ntt_fwd_61_limbs"] = {"ms": round(ntt_ms, 4), "achieved_GBs": round(ntt_gbs, 1),
+                               "frac": round(ntt_gbs / HBM_PEAK_GBS, 4)}
+    del x
+
+    if not args.no_extra:
+        res["mixed_levels_n15"] = mixed_n15(args, rng, stream)
+        res["sumslots_n16"] = sumslots_n16(args, stream)
+
+    if not args.no_serving:
         res["serving"] = serving_bench(args, rng)
         torch.cuda.empty_cache()
 
-    if args.cpu_sample > 0 and not args.profile:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from orc import Oracle
-        threads = min(16, os.cpu_count() or 1)
-        Oracle.threads(threads)
-        O = Oracle(N, K_Q, K_B, PREC)
-        c1, c2, _, _ = plain_items[0]
-        h = [P.to_host(t) for t in list(c1) + list(c2) + evk]
-        O.hommul((h[0], h[1]), (h[2], h[3]), (h[4], h[5]), level)  # warm-up
-        t = time.perf_counter()
-        for _ in range(args.cpu_sample):
-            O.hommul((h[0], h[1]), (h[2], h[3]), (h[4], h[5]), level)
-        cpu_s = (time.perf_counter() - t) / args.cpu_sample
-        res["cpu_baseline"] = {"value": round(1.0 / cpu_s, 4), "unit": "ops/s", "cores": threads, "kind": "port",
-                               "sample": "%d HomMul+relin+rescale at the C2 shape on the oracle (the reference's "
-                                         "algorithm restated in C++, OpenMP over limbs/coefficients)" % args.cpu_sample}
+    if args.cpu_sample > 0:
+        res["cpu_baseline"] = cpu_baseline(args, P, plain_items, evk, level)
     print(json.dumps(res))
 
 

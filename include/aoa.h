@@ -225,6 +225,28 @@ size_t aoa_batch_workspace_bytes(const aoa_batch* batch);
 int aoa_batch_launch(aoa_batch* batch, void* stream, int use_graph);
 void aoa_batch_destroy(aoa_batch* batch);
 
+/* Pipeline stages of a batch, in launch order inside each layer.  NTT stages
+ * are two kernels (column and row pass); the others one kernel. */
+enum {
+  AOA_STAGE_PRE_FWD = 0,      /* forward NTT of inputs given in the coefficient domain */
+  AOA_STAGE_PRE_INV = 1,      /* inverse NTT of inputs given in the NTT domain */
+  AOA_STAGE_TENSOR = 2,       /* d0, d1, d2 of HomMul */
+  AOA_STAGE_AUTOMORPHISM = 3, /* rotation's Galois map */
+  AOA_STAGE_INTT = 4,         /* inverse NTT of d0, d1, d2 */
+  AOA_STAGE_MODUP = 5,
+  AOA_STAGE_NTT = 6,          /* forward NTT of the ModUp output */
+  AOA_STAGE_KEY_INTT = 7,     /* key product fused into the inverse NTT */
+  AOA_STAGE_MODDOWN = 8,      /* ModDown (+ rescale, + additions) */
+  AOA_NSTAGES = 9
+};
+
+/* Measurement helper (no counterpart in the reference): runs the batch once
+ * on `stream` without a graph, with a HIP event pair around each stage
+ * launch of each layer, waits for it, and returns per stage the summed
+ * milliseconds (ms[s]) and the number of timed launches (launches[s]).
+ * nstages must be >= AOA_NSTAGES. */
+int aoa_batch_stage_times(aoa_batch* batch, void* stream, double* ms, int* launches, int nstages);
+
 /* ---------------------------------------------------------------- serving --
  * Continuous batching of encrypted-op requests.  No counterpart in the
  * reference (one call at a time on one stream,

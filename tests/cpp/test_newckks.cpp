@@ -240,10 +240,10 @@ static void reference_mode_prec48() {
   CUDAManager::destroy();
 }
 
-// Config 1: N=4096, 4 limbs (prec 52), K=5; 2048 slots uniform in [-1,1];
+// N=4096, 4 limbs (prec 52), K=5; 2048 slots uniform in [-1,1];
 // encode -> encrypt -> HomMul + relin + rescale -> decrypt -> decode, against
 // the FP64 product, at the reference's scale 2^52 and at the explicit 2^40.
-static void config1() {
+static void mul_n4096_k4() {
   CUDAManager::init(4, 5, 4096, 52);
   CKKSContext* ctx = new CKKSContext();
   SecretKey* sk = ckks_new_sk(ctx);
@@ -265,7 +265,7 @@ static void config1() {
     cipher_t* b = bits == 52 ? ckks_encrypt(ctx, v2.data(), slots) : ckks_encrypt_scaled(ctx, v2.data(), slots, bits);
     cipher_t* c = ckks_mul(ctx, a, b);
     cd* out = ckks_decrypt(ctx, c, sk);
-    report("Config1_scale2^" + std::to_string(bits), maxerr(exp.data(), out, slots), ERRBOUND);
+    report("MulN4096K4_scale2^" + std::to_string(bits), maxerr(exp.data(), out, slots), ERRBOUND);
     free(out);
     for (cipher_t* x : {a, b, c}) {
       cipher_free(ctx, x);
@@ -281,7 +281,7 @@ static void config1() {
 
 int main(int argc, char** argv) {
   const bool quick = argc > 1 && std::string(argv[1]) == "quick";
-  config1();
+  mul_n4096_k4();
   run_params(3, 3, 4096, 55, true);
   run_params(5, 5, 8192, 55, true);
   if (!quick) {

@@ -37,7 +37,7 @@ def _plain_mul(P, evk, item):
 def test_batch_mul_mixed_levels_equals_plain(gpu):
     import torch
     from aoa_amd import Batch, Params
-    P = Params(32768, 20, 0, 48)  # config 3 shape
+    P = Params(32768, 20, 0, 48)  # N=2^15, k=20, K=21
     rng = np.random.default_rng(31)
     evk = [P.to_device(_rand(P, rng, P.k + P.kl)) for _ in range(2)]
     levels = [int(x) for x in rng.integers(1, 20, size=12)]
@@ -145,3 +145,25 @@ def test_autotuner_splits_agree(gpu):
     times = P.autotune_ntt(reps=3)
     assert set(times) == {64, 128, 256}
     assert P.ntt_split() in times
+
+
+def test_stage_times_runs_the_batch(gpu):
+    """The stage-timing run is an ordinary eager run of the batch: outputs equal
+    the plain form, every stage of HomMul is timed once per layer."""
+    import torch
+    from aoa_amd import Batch, Params
+    P = Params(16384, 6, 0, 48)
+    rng = np.random.default_rng(35)
+    evk = [P.to_device(_rand(P, rng, P.k + P.kl)) for _ in range(2)]
+    items = _mul_items(P, rng, [5, 3, 1])
+    ref = [_plain_mul(P, evk, it) for it in items]
+    B = Batch()
+    B.add_mul(P, evk, items)
+    split = B.stage_times()
+    torch.cuda.synchronize()
+    assert set(split) == {"tensor", "intt", "modup", "ntt", "key_intt", "moddown"}
+    assert all(n == 1 and ms > 0 for ms, n in split.values())
+    for it, r in zip(items, ref):
+        lv = it[3]
+        assert np.array_equal(P.to_host(it[2][0])[:lv * P.N], r[0])
+        assert np.array_equal(P.to_host(it[2][1])[:lv * P.N], r[1])

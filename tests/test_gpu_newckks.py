@@ -1,6 +1,6 @@
 """The newckks-compatible C++ API (libnewckks.so over the C-ABI) on the GPU:
 encode/encrypt/add/mul/square/rescale/rotate/sumslots/decrypt against FP64
-within the reference's error bounds, config 1 (N=4096, prec 52, 2048 slots)
+within the reference's error bounds at N=4096, prec 52, 2048 slots
 at scale 2^52 and 2^40, and the level-0 rescale error path."""
 import os
 import subprocess
@@ -20,7 +20,7 @@ def test_newckks_api(gpu):
     print(r.stderr[-2000:])
     lines = [ln for ln in r.stdout.splitlines() if ln.startswith(("PASS", "FAIL"))]
     assert lines, "no test output"
-    assert any("Config1_scale2^52" in ln for ln in lines)
+    assert any("MulN4096K4_scale2^52" in ln for ln in lines)
     assert any("Mul_N65536_k30_kl0_prec48_exactmoddown" in ln for ln in lines)
     failed = [ln for ln in lines if ln.startswith("FAIL")]
     assert not failed, "\n".join(failed)

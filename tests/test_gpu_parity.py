@@ -12,7 +12,7 @@ from orc import Oracle
 
 pytestmark = pytest.mark.gpu
 
-# (N, k, kl, prec): C1 shape, small/medium shapes, and the reference's test params.
+# (N, k, kl, prec): N=4096 k=4 prec 52, small/medium shapes, and the reference's test params.
 SMALL = [(4096, 4, 5, 52), (8192, 6, 0, 48), (16384, 5, 5, 55), (32768, 4, 0, 48), (4096, 3, 3, 45)]
 
 
@@ -128,7 +128,7 @@ def test_rotate_parity(gpu, shape):
 
 
 def test_hommul_c2_parity(gpu):
-    """Config 2 shape: N=2^16, 30 Q limbs (prec 48), 31 special limbs, top level."""
+    """N=2^16, 30 Q limbs (prec 48), 31 special limbs, top level."""
     P, O = _params(65536, 30, 0, 48)
     rng = np.random.default_rng(2)
     k, kl, N = O.k, O.kl, O.N
