@@ -12,6 +12,35 @@ struct alignas(16) tw2 {
   uint64_t w, q;  // constant and its Shoup quotient
 };
 
+// Read-only tables seen through the constant address space: loads at a
+// wave-uniform index become scalar loads, which the compiler cannot prove
+// safe through a generic pointer once the kernel also stores to memory.
+// The tables are written only by aoa_params_create, before any kernel runs.
+template <class T>
+using cptr = const T __attribute__((address_space(4)))*;
+template <class T>
+__device__ __forceinline__ cptr<T> as_const(const T* p) {
+  return (cptr<T>)p;
+}
+
+// Entry i of a read-only table of tw2 / DevPrime through the constant
+// address space (word by word: structs cannot be copied out of it).
+__device__ __forceinline__ tw2 ld_tw2(const tw2* base, size_t i) {
+  const cptr<uint64_t> w = as_const(reinterpret_cast<const uint64_t*>(base + i));
+  return tw2{w[0], w[1]};
+}
+__device__ __forceinline__ uint64_t ld_u64(const uint64_t* base, size_t i) { return as_const(base)[i]; }
+__device__ __forceinline__ DevPrime ld_prime(const DevPrime* base, int i) {
+  constexpr int W = (int)(sizeof(DevPrime) / 8);
+  const cptr<uint64_t> w = as_const(reinterpret_cast<const uint64_t*>(base + i));
+  uint64_t t[W];
+#pragma unroll
+  for (int x = 0; x < W; ++x) t[x] = w[x];
+  DevPrime d;
+  __builtin_memcpy(&d, t, sizeof(d));
+  return d;
+}
+
 // Where the limbs of one launch live.  Limb li of poly y is at
 // base[y] + li * N and uses prime index prime_of(li): the first `qlimbs`
 // limbs are Q limbs prime0, prime0+1, ..., the rest are special limbs

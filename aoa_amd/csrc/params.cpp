@@ -68,6 +68,10 @@ static std::string select_primes(int k, int kl, int prec, std::vector<aoa_prime_
   q->push_back(AOA_BUCKET_63[0]);
   for (int i = 0; i < k - 1; ++i) q->push_back(qb[i]);
   for (int j = 0; j < kl; ++j) b->push_back(bb[boff + j]);
+  // ModUp splits its operands (< b_j) at bit 28 and ModDown keeps lazy sums
+  // of K terms below 2^57 each: both need special primes below 2^56.
+  for (auto& e : *b)
+    if (e.p >> 56) return "special primes must be below 2^56";
   return "";
 }
 
@@ -105,6 +109,9 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
     d.r64_q = shoup_quot(d.r64, p);
     P.dprimes.push_back(d);
   }
+  for (int j = 0; j < kl; ++j)
+    for (int i = 1; i < k; ++i)
+      if (P.primes[i] > P.primes[k + j]) P.dprimes[k + j].modup_reduce = 1;
 
   // Twiddles: psi = g^((p-1)/2N); forward table psi^brev(i), inverse psi^-brev(i).
   P.tw_fwd.assign((size_t)R * N * 2, 0);

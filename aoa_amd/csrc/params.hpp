@@ -36,7 +36,10 @@ struct DevPrime {
   uint64_t quirk_lo;
   uint64_t quirk_thr;
   uint32_t nb;       // bit length of p
-  uint32_t pad;
+  // Special primes only: 1 when some Q prime after q0 exceeds this prime, so
+  // ModUp must reduce those residues before multiplying (q0 is checked
+  // separately).
+  uint32_t modup_reduce;
 };
 
 // Pointers into one device allocation holding every table of a parameter set.

@@ -16,6 +16,16 @@ __device__ __forceinline__ uint64_t reduce128(uint64_t hi, uint64_t lo, const De
   return add_mod(a, b, d.p);
 }
 
+// (hi, lo) = s0 + s1 2^28 + s2 2^56 as a 128-bit value.
+__device__ __forceinline__ void join_28(uint64_t s0, uint64_t s1, uint64_t s2, uint64_t* hi, uint64_t* lo) {
+  uint64_t l = s0 + (s1 << 28);
+  uint64_t h = (s1 >> 36) + (l < s0 ? 1u : 0u);
+  const uint64_t t = l + (s2 << 56);
+  h += (s2 >> 8) + (t < l ? 1u : 0u);
+  *hi = h;
+  *lo = t;
+}
+
 // ------------------------------------------------------------ element-wise --
 
 // Two elements per thread; a 256-thread block covers 512 elements of one limb.
@@ -118,31 +128,54 @@ __device__ __forceinline__ void modup_body(const DevParams& P, const uint64_t* x
   }
   __syncthreads();
   // Two special limbs per pass over i (shared LDS reads, two independent
-  // 128-bit accumulation chains).
+  // accumulation chains).  With v, w < 2^56 split at bit 28, v*w =
+  // s0 + s1 2^28 + s2 2^56 where each partial sum over at most 64 terms fits
+  // in 64 bits: four 32x32->64 multiply-adds per term, no carries.
+  constexpr uint64_t M28 = (1ull << 28) - 1;
   for (int j = g; j < P.K; j += 2 * BC_GROUPS) {
     const int j2 = j + BC_GROUPS;
     const bool two = j2 < P.K;
-    const DevPrime d1 = P.primes[P.k + j];
-    const DevPrime d2 = P.primes[P.k + (two ? j2 : j)];
+    const DevPrime d1 = ld_prime(P.primes, P.k + j);
+    const DevPrime d2 = ld_prime(P.primes, P.k + (two ? j2 : j));
     const uint64_t* c1 = P.qhat_mod_b + ((size_t)level * P.K + j) * P.k;
     const uint64_t* c2 = P.qhat_mod_b + ((size_t)level * P.K + (two ? j2 : j)) * P.k;
-    uint64_t h1 = 0, l1 = 0, h2 = 0, l2 = 0;
-#pragma unroll 2
-    for (int i = 0; i < L; ++i) {
+    uint64_t a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+    auto term = [&](int i, bool reduce) {
       const uint64_t v = tl[i * BC_COEF + c];
-      const uint64_t qi = P.primes[i].p;
-      const uint64_t v1 = qi > d1.p ? barrett_reduce(0, v, d1.p, d1.mu, (int)d1.nb) : v;
-      const uint64_t v2 = qi > d2.p ? barrett_reduce(0, v, d2.p, d2.mu, (int)d2.nb) : v;
-      const uint64_t w1 = c1[i], w2 = c2[i];
-      const uint64_t ph1 = mulhi64(v1, w1), pl1 = v1 * w1;
-      const uint64_t ph2 = mulhi64(v2, w2), pl2 = v2 * w2;
-      l1 += pl1;
-      h1 += ph1 + (l1 < pl1 ? 1u : 0u);
-      l2 += pl2;
-      h2 += ph2 + (l2 < pl2 ? 1u : 0u);
+      uint64_t v1 = v, v2 = v;
+      if (reduce) {
+        const uint64_t qi = ld_prime(P.primes, i).p;
+        if (qi > d1.p) v1 = barrett_reduce(0, v, d1.p, d1.mu, (int)d1.nb);
+        if (qi > d2.p) v2 = barrett_reduce(0, v, d2.p, d2.mu, (int)d2.nb);
+      }
+      const uint64_t w1 = ld_u64(c1, i), w2 = ld_u64(c2, i);
+      const uint32_t x1l = (uint32_t)(v1 & M28), x1h = (uint32_t)(v1 >> 28);
+      const uint32_t x2l = (uint32_t)(v2 & M28), x2h = (uint32_t)(v2 >> 28);
+      const uint32_t w1l = (uint32_t)(w1 & M28), w1h = (uint32_t)(w1 >> 28);
+      const uint32_t w2l = (uint32_t)(w2 & M28), w2h = (uint32_t)(w2 >> 28);
+      a0 += (uint64_t)x1l * w1l;
+      a1 += (uint64_t)x1l * w1h;
+      a1 += (uint64_t)x1h * w1l;
+      a2 += (uint64_t)x1h * w1h;
+      b0 += (uint64_t)x2l * w2l;
+      b1 += (uint64_t)x2l * w2h;
+      b1 += (uint64_t)x2h * w2l;
+      b2 += (uint64_t)x2h * w2h;
+    };
+    term(0, true);  // q0 is wider than every special prime
+    if (d1.modup_reduce | d2.modup_reduce) {
+      for (int i = 1; i < L; ++i) term(i, true);
+    } else {
+#pragma unroll 4
+      for (int i = 1; i < L; ++i) term(i, false);
     }
-    outB[(size_t)j * N + cid] = reduce128(h1, l1, d1);
-    if (two) outB[(size_t)j2 * N + cid] = reduce128(h2, l2, d2);
+    uint64_t h, l;
+    join_28(a0, a1, a2, &h, &l);
+    outB[(size_t)j * N + cid] = reduce128(h, l, d1);
+    if (two) {
+      join_28(b0, b1, b2, &h, &l);
+      outB[(size_t)j2 * N + cid] = reduce128(h, l, d2);
+    }
   }
 }
 
@@ -206,46 +239,75 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   }
   __syncthreads();
   const tw2* ibq = reinterpret_cast<const tw2*>(P.inv_b_mod_q);
+  const tw2* bq = reinterpret_cast<const tw2*>(P.bhat_mod_q);
   // Sum over j of the reference's products for limb i, reduced mod q_i.
-  // Terms below 2^58 bits stay lazy in [0, 2p) and are summed without
-  // reduction (K <= 64 terms); the 63-bit prime reduces every term.
+  // Terms of primes below 2^57 are exact Shoup products left in [0, 2p) and
+  // summed without reduction (K <= 64 terms).  A term whose remainder falls
+  // in the deviation window only raises a flag; a flagged lane (about one
+  // coefficient-limb in 4000 at 48-bit q_i) recomputes its sum with the
+  // reference formula per term.  The 63-bit prime reduces every term.
   auto conv = [&](int i, int j0, int j1) -> uint64_t {
-    const DevPrime dq = P.primes[i];
-    const tw2* ci = reinterpret_cast<const tw2*>(P.bhat_mod_q) + (size_t)i * K;
+    const DevPrime dq = ld_prime(P.primes, i);
+    const tw2* ci = bq + (size_t)i * K;
     uint64_t acc = 0;
     if (dq.nb <= 56) {
       const uint64_t thr = lazy_window(dq, P.moddown_exact);
+      bool hit = false;
 #pragma unroll 4
       for (int j = j0; j < j1; ++j) {
-        const tw2 w = ci[j];
-        acc += ref_mulmod_shoup_lazy(aux[j * BC_COEF + c], w.w, w.q, dq.p, dq.mu, (int)dq.nb, thr);
+        const tw2 w = ld_tw2(ci, j);
+        const uint64_t r = mul_shoup_lazy(aux[j * BC_COEF + c], w.w, w.q, dq.p);
+        hit |= (r < thr) | (r - dq.p < thr);
+        acc += r;
+      }
+      if (hit) {
+        acc = 0;
+        for (int j = j0; j < j1; ++j) {
+          const tw2 w = ld_tw2(ci, j);
+          acc += ref_mulmod_shoup_lazy(aux[j * BC_COEF + c], w.w, w.q, dq.p, dq.mu, (int)dq.nb, thr);
+        }
       }
       return barrett_reduce(0, acc, dq.p, dq.mu, (int)dq.nb);
     }
 #pragma unroll 4
-    for (int j = j0; j < j1; ++j)
-      acc = add_mod(acc, moddown_term(aux[j * BC_COEF + c], ci[j], dq, P.moddown_exact), dq.p);
+    for (int j = j0; j < j1; ++j) {
+      const tw2 w = ld_tw2(ci, j);
+      acc = add_mod(acc, moddown_term(aux[j * BC_COEF + c], w, dq, P.moddown_exact), dq.p);
+    }
     return acc;
   };
   // Two limbs at once: shared aux reads, two independent accumulation chains.
   auto conv2 = [&](int i, int i2, uint64_t* r1, uint64_t* r2) {
-    const DevPrime d1 = P.primes[i], d2 = P.primes[i2];
+    const DevPrime d1 = ld_prime(P.primes, i), d2 = ld_prime(P.primes, i2);
     if (d1.nb > 56 || d2.nb > 56) {
       *r1 = conv(i, 0, K);
       *r2 = conv(i2, 0, K);
       return;
     }
-    const tw2* c1 = reinterpret_cast<const tw2*>(P.bhat_mod_q) + (size_t)i * K;
-    const tw2* c2 = reinterpret_cast<const tw2*>(P.bhat_mod_q) + (size_t)i2 * K;
+    const tw2* c1 = bq + (size_t)i * K;
+    const tw2* c2 = bq + (size_t)i2 * K;
     const uint64_t t1 = lazy_window(d1, P.moddown_exact);
     const uint64_t t2 = lazy_window(d2, P.moddown_exact);
     uint64_t a1 = 0, a2 = 0;
-#pragma unroll 2
+    bool hit = false;
+#pragma unroll 4
     for (int j = 0; j < K; ++j) {
       const uint64_t a = aux[j * BC_COEF + c];
-      const tw2 w1 = c1[j], w2 = c2[j];
-      a1 += ref_mulmod_shoup_lazy(a, w1.w, w1.q, d1.p, d1.mu, (int)d1.nb, t1);
-      a2 += ref_mulmod_shoup_lazy(a, w2.w, w2.q, d2.p, d2.mu, (int)d2.nb, t2);
+      const tw2 w1 = ld_tw2(c1, j), w2 = ld_tw2(c2, j);
+      const uint64_t x1 = mul_shoup_lazy(a, w1.w, w1.q, d1.p);
+      const uint64_t x2 = mul_shoup_lazy(a, w2.w, w2.q, d2.p);
+      hit |= (x1 < t1) | (x1 - d1.p < t1) | (x2 < t2) | (x2 - d2.p < t2);
+      a1 += x1;
+      a2 += x2;
+    }
+    if (hit) {
+      a1 = a2 = 0;
+      for (int j = 0; j < K; ++j) {
+        const uint64_t a = aux[j * BC_COEF + c];
+        const tw2 w1 = ld_tw2(c1, j), w2 = ld_tw2(c2, j);
+        a1 += ref_mulmod_shoup_lazy(a, w1.w, w1.q, d1.p, d1.mu, (int)d1.nb, t1);
+        a2 += ref_mulmod_shoup_lazy(a, w2.w, w2.q, d2.p, d2.mu, (int)d2.nb, t2);
+      }
     }
     *r1 = barrett_reduce(0, a1, d1.p, d1.mu, (int)d1.nb);
     *r2 = barrett_reduce(0, a2, d2.p, d2.mu, (int)d2.nb);
