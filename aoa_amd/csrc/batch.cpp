@@ -318,6 +318,8 @@ int batch_add_mul(aoa_batch* b, const aoa_params* p, const uint64_t* evk_b, cons
     const aoa_mul_item& it = items[i];
     if (!it.c1[0] || !it.c1[1] || !it.c2[0] || !it.c2[1] || !it.out[0] || !it.out[1])
       return set_error(AOA_ERR_ARG, "null pointer in item");
+    if ((((uintptr_t)it.c1[0] | (uintptr_t)it.c1[1] | (uintptr_t)it.c2[0] | (uintptr_t)it.c2[1]) & 15) != 0)
+      return set_error(AOA_ERR_ARG, "device pointers must be 16-byte aligned");
     if (it.level == 0) return set_error(AOA_ERR_LEVEL, "The ciphertext level is 0. No more rescaling is possible.");
     if (it.level < 0 || it.level >= p->host.k) return set_error(AOA_ERR_ARG, "level out of range");
   }

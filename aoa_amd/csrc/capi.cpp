@@ -36,6 +36,9 @@ uint64_t galois_of(int rot_slots, int N) {
 
 }  // namespace aoa
 
+// Ciphertext and workspace pointers are moved as 16-byte vectors.
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
 static int fail(int code, const std::string& msg) { return set_error(code, msg); }
 
 extern "C" const char* aoa_last_error(void) { return g_err.c_str(); }
@@ -319,6 +322,9 @@ static int hommul_core(const aoa_params* h, void* stream, uint64_t* c3_0, uint64
                        const uint64_t* evk_a, int level, void* ws, size_t ws_bytes, int rescale) {
   if (!h || !c3_0 || !c3_1 || !a0 || !a1 || !b0 || !b1 || !evk_b || !evk_a || !ws)
     return fail(AOA_ERR_ARG, "null argument");
+  if (!aligned16(c3_0) || !aligned16(c3_1) || !aligned16(a0) || !aligned16(a1) || !aligned16(b0) ||
+      !aligned16(b1) || !aligned16(ws))
+    return fail(AOA_ERR_ARG, "device pointers must be 16-byte aligned");
   if (level == 0) return fail(AOA_ERR_LEVEL, "The ciphertext level is 0. No more rescaling is possible.");
   if (level < 0 || level >= h->host.k) return fail(AOA_ERR_ARG, "level out of range");
   if (ws_bytes < aoa_workspace_bytes(h, level)) return fail(AOA_ERR_WORKSPACE, "workspace too small");

@@ -102,6 +102,41 @@ AOA_HD uint64_t mul_barrett(uint64_t a, uint64_t b, uint64_t p, uint64_t mu, int
   return barrett_reduce(mulhi64(a, b), a * b, p, mu, nb);
 }
 
+// Full 128-bit product from four 32x32->64 multiply-adds.
+AOA_HD void mul128(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {
+  const uint64_t al = (uint32_t)a, ah = a >> 32, bl = (uint32_t)b, bh = b >> 32;
+  const uint64_t ll = al * bl;
+  const uint64_t lh = al * bh + (ll >> 32);
+  const uint64_t hl = ah * bl + (uint32_t)lh;
+  *hi = ah * bh + (lh >> 32) + (hl >> 32);
+  *lo = (hl << 32) | (uint32_t)ll;
+}
+
+// a * b mod p for a, b < p < 2^62, result in [0, 2p): HAC 14.42 with the
+// quotient floor(floor(t / 2^(nb-1)) * mu / 2^(nb+1)) taken as one mulhi by
+// mu2 = mu * 2^(63-nb); it is at most 2 below floor(t/p), so t - q*p < 3p
+// fits in 64 bits, and one conditional subtraction of 2p leaves [0, 2p).
+AOA_HD uint64_t mul_barrett62_lazy(uint64_t a, uint64_t b, uint64_t p, uint64_t mu2, int nb) {
+  uint64_t hi, lo;
+  mul128(a, b, &hi, &lo);
+  const int s = nb - 1;
+  const uint64_t q1 = (lo >> s) | (hi << (64 - s));
+  const uint64_t q = mulhi64(q1, mu2);
+  uint64_t r = lo - q * p;
+  const uint64_t p2 = 2 * p;
+  return r >= p2 ? r - p2 : r;
+}
+
+AOA_HD uint64_t mul_barrett62(uint64_t a, uint64_t b, uint64_t p, uint64_t mu2, int nb) {
+  const uint64_t r = mul_barrett62_lazy(a, b, p, mu2, nb);
+  return r >= p ? r - p : r;
+}
+
+// a * b mod p for a, b < p, any supported prime.
+AOA_HD uint64_t mul_var(uint64_t a, uint64_t b, uint64_t p, uint64_t mu, uint64_t mu2, int nb) {
+  return (p >> 62) ? mul_barrett(a, b, p, mu, nb) : mul_barrett62(a, b, p, mu2, nb);
+}
+
 // The reference's Barrett reduction of a 128-bit value (hi:lo) with
 // MU = floor(4^nb / p) (aoa-ntt/src/cuda/htrans/common.cu:257-291, constants at
 // :485-506).  Written from its arithmetic: q = floor(floor(a / 2^(2nb-64)) *

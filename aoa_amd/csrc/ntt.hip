@@ -31,19 +31,48 @@ struct Geo {
   static constexpr int ROW_LDS = C * (S + S / 16);
 };
 
+// Butterflies.  LZ (primes below 2^61): Harvey's lazy forms, values kept in
+// [0, 4p) through the forward transform and in [0, 2p) through the inverse,
+// one correction per butterfly instead of three.  Otherwise (the 63-bit
+// prime) every value stays canonical.
+template <bool LZ>
 __device__ __forceinline__ void ct_bfly(uint64_t& a, uint64_t& b, const tw2 t, uint64_t p) {
-  uint64_t v = mul_shoup(b, t.w, t.q, p);
-  uint64_t u = a;
-  uint64_t s = u + v;
-  a = s >= p ? s - p : s;
-  b = u >= v ? u - v : u + (p - v);
+  if (LZ) {
+    const uint64_t p2 = 2 * p;
+    uint64_t u = a;
+    u = u >= p2 ? u - p2 : u;
+    const uint64_t v = mul_shoup_lazy(b, t.w, t.q, p);
+    a = u + v;
+    b = u + p2 - v;
+  } else {
+    uint64_t v = mul_shoup(b, t.w, t.q, p);
+    uint64_t u = a;
+    uint64_t s = u + v;
+    a = s >= p ? s - p : s;
+    b = u >= v ? u - v : u + (p - v);
+  }
 }
 
+template <bool LZ>
 __device__ __forceinline__ void gs_bfly(uint64_t& a, uint64_t& b, const tw2 t, uint64_t p) {
   uint64_t u = a, v = b;
-  uint64_t s = u + v;
-  a = s >= p ? s - p : s;
-  b = mul_shoup(u + (p - v), t.w, t.q, p);
+  if (LZ) {
+    const uint64_t p2 = 2 * p;
+    const uint64_t s = u + v;
+    a = s >= p2 ? s - p2 : s;
+    b = mul_shoup_lazy(u + p2 - v, t.w, t.q, p);
+  } else {
+    uint64_t s = u + v;
+    a = s >= p ? s - p : s;
+    b = mul_shoup(u + (p - v), t.w, t.q, p);
+  }
+}
+
+// [0, 4p) -> [0, p)
+__device__ __forceinline__ uint64_t canon4(uint64_t x, uint64_t p) {
+  const uint64_t p2 = 2 * p;
+  x = x >= p2 ? x - p2 : x;
+  return x >= p ? x - p : x;
 }
 
 __device__ __forceinline__ int swz(int s) { return s + (s >> 4); }
@@ -53,7 +82,7 @@ __device__ __forceinline__ int swz(int s) { return s + (s >> 4); }
 // tile within the limb.
 
 // Column pass: global stages 0..log2(S)-1 on columns of an S x N2 matrix.
-template <int S>
+template <int S, bool LZ>
 __device__ __forceinline__ void fwd_col_body(const DevParams& P, const uint64_t* src, uint64_t* dst, int pr,
                                              int tile, int N2, uint64_t* lds) {
   using G = Geo<S>;
@@ -76,7 +105,7 @@ __device__ __forceinline__ void fwd_col_body(const DevParams& P, const uint64_t*
     for (int k = 0; k < 16; ++k) {
       if (k & half) continue;
       const tw2 t = load_tw(T, (1 << r) + (k >> (4 - r)));
-      ct_bfly(x[k], x[k + half], t, p);
+      ct_bfly<LZ>(x[k], x[k + half], t, p);
     }
   }
 #pragma unroll
@@ -92,7 +121,7 @@ __device__ __forceinline__ void fwd_col_body(const DevParams& P, const uint64_t*
     for (int k = 0; k < 16; ++k) {
       if (k & half) continue;
       const tw2 t = load_tw(T, (1 << r) + (sh << (r - (LOGS - 4))) + (k >> (LOGS - r)));
-      ct_bfly(x[k], x[k + half], t, p);
+      ct_bfly<LZ>(x[k], x[k + half], t, p);
     }
   }
 #pragma unroll
@@ -100,7 +129,7 @@ __device__ __forceinline__ void fwd_col_body(const DevParams& P, const uint64_t*
 }
 
 // Row pass: global stages g0..g0+log2(S)-1 on contiguous rows of length S.
-template <int S>
+template <int S, bool LZ>
 __device__ __forceinline__ void fwd_row_body(const DevParams& P, const uint64_t* src, uint64_t* dst, int pr,
                                              int tile, int g0, uint64_t* lds) {
   using G = Geo<S>;
@@ -126,7 +155,7 @@ __device__ __forceinline__ void fwd_row_body(const DevParams& P, const uint64_t*
     for (int k = 0; k < 16; ++k) {
       if (k & half) continue;
       const tw2 t = load_tw(T, base + (k >> (4 - r)));
-      ct_bfly(x[k], x[k + half], t, p);
+      ct_bfly<LZ>(x[k], x[k + half], t, p);
     }
   }
 #pragma unroll
@@ -143,8 +172,12 @@ __device__ __forceinline__ void fwd_row_body(const DevParams& P, const uint64_t*
     for (int k = 0; k < 16; ++k) {
       if (k & half) continue;
       const tw2 t = load_tw(T, base + (k >> (LOGS - r)));
-      ct_bfly(x[k], x[k + half], t, p);
+      ct_bfly<LZ>(x[k], x[k + half], t, p);
     }
+  }
+  if (LZ) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = canon4(x[k], p);
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) L[swz(16 * sh + k)] = x[k];
@@ -156,7 +189,7 @@ __device__ __forceinline__ void fwd_row_body(const DevParams& P, const uint64_t*
 // Row pass of the inverse: global stages g0+log2(S)-1 .. g0.  With a key, the
 // input limb is first multiplied by the key limb (key switching:
 // INTT(x * key_b), INTT(x * key_a) from one input).
-template <int S>
+template <int S, bool LZ>
 __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t* src, uint64_t* dst,
                                              const uint64_t* key, int pr, int tile, int g0, uint64_t* lds) {
   using G = Geo<S>;
@@ -178,7 +211,9 @@ __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t*
   if (key) {
     const uint64_t* kp = key + (size_t)row * S;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = mul_barrett(x[k], kp[sl + TS * k], p, dp.mu, (int)dp.nb);
+    for (int k = 0; k < 16; ++k)
+      x[k] = LZ ? mul_barrett62_lazy(x[k], kp[sl + TS * k], p, dp.mu2, (int)dp.nb)
+                : mul_barrett(x[k], kp[sl + TS * k], p, dp.mu, (int)dp.nb);
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) L[swz(sl + TS * k)] = x[k];
@@ -194,7 +229,7 @@ __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t*
     for (int k = 0; k < 16; ++k) {
       if (k & half) continue;
       const tw2 t = load_tw(T, base + (k >> (LOGS - r)));
-      gs_bfly(x[k], x[k + half], t, p);
+      gs_bfly<LZ>(x[k], x[k + half], t, p);
     }
   }
 #pragma unroll
@@ -210,7 +245,7 @@ __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t*
     for (int k = 0; k < 16; ++k) {
       if (k & half) continue;
       const tw2 t = load_tw(T, base + (k >> (4 - r)));
-      gs_bfly(x[k], x[k + half], t, p);
+      gs_bfly<LZ>(x[k], x[k + half], t, p);
     }
   }
 #pragma unroll
@@ -219,7 +254,7 @@ __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t*
 
 // Column pass of the inverse: global stages log2(S)-1 .. 0, N^-1 folded into
 // stage 0.
-template <int S>
+template <int S, bool LZ>
 __device__ __forceinline__ void inv_col_body(const DevParams& P, const uint64_t* src, uint64_t* dst, int pr,
                                              int tile, int N2, uint64_t* lds) {
   using G = Geo<S>;
@@ -243,7 +278,7 @@ __device__ __forceinline__ void inv_col_body(const DevParams& P, const uint64_t*
     for (int k = 0; k < 16; ++k) {
       if (k & half) continue;
       const tw2 t = load_tw(T, (1 << r) + (sh << (r - (LOGS - 4))) + (k >> (LOGS - r)));
-      gs_bfly(x[k], x[k + half], t, p);
+      gs_bfly<LZ>(x[k], x[k + half], t, p);
     }
   }
 #pragma unroll
@@ -259,7 +294,7 @@ __device__ __forceinline__ void inv_col_body(const DevParams& P, const uint64_t*
     for (int k = 0; k < 16; ++k) {
       if (k & half) continue;
       const tw2 t = load_tw(T, (1 << r) + (k >> (4 - r)));
-      gs_bfly(x[k], x[k + half], t, p);
+      gs_bfly<LZ>(x[k], x[k + half], t, p);
     }
   }
   // stage 0 with N^-1: a' = (u+v) N^-1, b' = (u-v) psi^-brev(1) N^-1
@@ -267,13 +302,18 @@ __device__ __forceinline__ void inv_col_body(const DevParams& P, const uint64_t*
   for (int k = 0; k < 8; ++k) {
     uint64_t u = x[k], v = x[k + 8];
     x[k] = mul_shoup(u + v, dp.ninv, dp.ninv_q, p);
-    x[k + 8] = mul_shoup(u + (p - v), dp.w1ninv, dp.w1ninv_q, p);
+    x[k + 8] = mul_shoup(LZ ? u + 2 * p - v : u + (p - v), dp.w1ninv, dp.w1ninv_q, p);
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) dst[(size_t)(sl + TS * k) * N2 + c] = x[k];
 }
 
 // ------------------------------------------------- argument-driven kernels --
+
+// Lazy butterflies need 4p < 2^63.
+__device__ __forceinline__ bool lazy_prime(const DevParams& P, int pr) {
+  return ld_prime(P.primes, pr).p < (1ull << 61);
+}
 
 template <int S>
 __global__ __launch_bounds__(256) void k_ntt_fwd_col(DevParams P, LimbMap m, int N2) {
@@ -282,7 +322,10 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(DevParams P, LimbMap m, int
   const int li = blockIdx.x / tiles;
   const int y = blockIdx.y;
   const size_t off = (size_t)li * P.N;
-  fwd_col_body<S>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, N2, lds);
+  if (lazy_prime(P, limb_prime(m, P.k, li)))
+    fwd_col_body<S, true>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, N2, lds);
+  else
+    fwd_col_body<S, false>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, N2, lds);
 }
 
 template <int S>
@@ -292,7 +335,10 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(DevParams P, LimbMap m, int
   const int li = blockIdx.x / tiles;
   const int y = blockIdx.y;
   const size_t off = (size_t)li * P.N;
-  fwd_row_body<S>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, g0, lds);
+  if (lazy_prime(P, limb_prime(m, P.k, li)))
+    fwd_row_body<S, true>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, g0, lds);
+  else
+    fwd_row_body<S, false>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, g0, lds);
 }
 
 template <int S>
@@ -306,7 +352,10 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row(DevParams P, LimbMap m, int
   const int pr = limb_prime(m, P.k, li);
   const uint64_t* key = key0 ? (y == 0 ? key0 : key1) + (size_t)pr * P.N : nullptr;
   const uint64_t* src = (key0 ? m.src[0] : m.src[y]) + off;
-  inv_row_body<S>(P, src, m.base[y] + off, key, pr, blockIdx.x % tiles, g0, lds);
+  if (lazy_prime(P, pr))
+    inv_row_body<S, true>(P, src, m.base[y] + off, key, pr, blockIdx.x % tiles, g0, lds);
+  else
+    inv_row_body<S, false>(P, src, m.base[y] + off, key, pr, blockIdx.x % tiles, g0, lds);
 }
 
 template <int S>
@@ -316,7 +365,10 @@ __global__ __launch_bounds__(256) void k_ntt_inv_col(DevParams P, LimbMap m, int
   const int li = blockIdx.x / tiles;
   const int y = blockIdx.y;
   const size_t off = (size_t)li * P.N;
-  inv_col_body<S>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, N2, lds);
+  if (lazy_prime(P, limb_prime(m, P.k, li)))
+    inv_col_body<S, true>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, N2, lds);
+  else
+    inv_col_body<S, false>(P, m.src[y] + off, m.base[y] + off, limb_prime(m, P.k, li), blockIdx.x % tiles, N2, lds);
 }
 
 // ----------------------------------------------- descriptor-driven kernels --
@@ -326,7 +378,10 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col_d(DevParams P, const NttDes
   __shared__ uint64_t lds[Geo<S>::COL_LDS];
   const int tiles = N2 / Geo<S>::C;
   const NttDesc e = d[blockIdx.x / tiles];
-  fwd_col_body<S>(P, e.src, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
+  if (lazy_prime(P, e.prime))
+    fwd_col_body<S, true>(P, e.src, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
+  else
+    fwd_col_body<S, false>(P, e.src, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
 }
 
 template <int S>
@@ -334,7 +389,10 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row_d(DevParams P, const NttDes
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
   const int tiles = (P.N / S) / Geo<S>::C;
   const NttDesc e = d[blockIdx.x / tiles];
-  fwd_row_body<S>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, g0, lds);
+  if (lazy_prime(P, e.prime))
+    fwd_row_body<S, true>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, g0, lds);
+  else
+    fwd_row_body<S, false>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, g0, lds);
 }
 
 template <int S>
@@ -342,7 +400,10 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_d(DevParams P, const NttDes
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
   const int tiles = (P.N / S) / Geo<S>::C;
   const NttDesc e = d[blockIdx.x / tiles];
-  inv_row_body<S>(P, e.src, e.dst, e.key, e.prime, blockIdx.x % tiles, g0, lds);
+  if (lazy_prime(P, e.prime))
+    inv_row_body<S, true>(P, e.src, e.dst, e.key, e.prime, blockIdx.x % tiles, g0, lds);
+  else
+    inv_row_body<S, false>(P, e.src, e.dst, e.key, e.prime, blockIdx.x % tiles, g0, lds);
 }
 
 template <int S>
@@ -350,7 +411,10 @@ __global__ __launch_bounds__(256) void k_ntt_inv_col_d(DevParams P, const NttDes
   __shared__ uint64_t lds[Geo<S>::COL_LDS];
   const int tiles = N2 / Geo<S>::C;
   const NttDesc e = d[blockIdx.x / tiles];
-  inv_col_body<S>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
+  if (lazy_prime(P, e.prime))
+    inv_col_body<S, true>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
+  else
+    inv_col_body<S, false>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
 }
 
 // ---------------------------------------------------------------- launch ---

@@ -102,6 +102,7 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
     d.p = p;
     d.nb = (uint32_t)bitlen64(p);
     d.mu = (uint64_t)((((unsigned __int128)1) << (2 * d.nb)) / p);
+    d.mu2 = d.mu << (63 - d.nb);
     d.ninv = inv_mod((uint64_t)N % p, p);
     d.ninv_q = shoup_quot(d.ninv, p);
     d.eps = (((uint64_t)1) << d.nb) - p;

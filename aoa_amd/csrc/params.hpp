@@ -35,6 +35,7 @@ struct DevPrime {
   // quirk_thr = delta_max * eps, or [0, 2^64) when that bound is not usable.
   uint64_t quirk_lo;
   uint64_t quirk_thr;
+  uint64_t mu2;      // mu * 2^(63 - nb): Barrett quotient as one mulhi (mul_barrett62)
   uint32_t nb;       // bit length of p
   // Special primes only: 1 when some Q prime after q0 exceeds this prime, so
   // ModUp must reduce those residues before multiplying (q0 is checked

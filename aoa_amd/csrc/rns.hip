@@ -43,9 +43,9 @@ __global__ __launch_bounds__(256) void k_eltwise(DevParams P, int op, uint64_t* 
     switch (op) {
       case ELT_ADD: r = add_mod(a[i], b[i], d.p); break;
       case ELT_SUB: r = sub_mod(a[i], b[i], d.p); break;
-      case ELT_MUL: r = mul_barrett(a[i], b[i], d.p, d.mu, (int)d.nb); break;
+      case ELT_MUL: r = mul_var(a[i], b[i], d.p, d.mu, d.mu2, (int)d.nb); break;
       case ELT_NEG: r = neg_mod(a[i], d.p); break;
-      case ELT_MULADD: r = add_mod(mul_barrett(a[i], b[i], d.p, d.mu, (int)d.nb), c[i], d.p); break;
+      case ELT_MULADD: r = add_mod(mul_var(a[i], b[i], d.p, d.mu, d.mu2, (int)d.nb), c[i], d.p); break;
       default: r = 0; break;
     }
     out[i] = r;
@@ -59,21 +59,31 @@ void launch_eltwise(const DevParams& P, int op, uint64_t* out, const uint64_t* a
 }
 
 // d0 = a0 b0, d2 = a1 b1, d1 = (a0 + a1)(b0 + b1) - d0 - d2 (= a0 b1 + a1 b0),
-// two elements per thread of one limb (pointers at element e of the limb).
+// two adjacent elements per thread of one limb (pointers at element e of the
+// limb, e even), moved as 16-byte vectors.
+struct alignas(16) u64x2 {
+  uint64_t v[2];
+};
+
 __device__ __forceinline__ void tensor_body(const DevPrime& d, uint64_t* d0, uint64_t* d1, uint64_t* d2,
                                             const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
                                             const uint64_t* b1, size_t e) {
+  const u64x2 X0 = *reinterpret_cast<const u64x2*>(a0 + e), X1 = *reinterpret_cast<const u64x2*>(a1 + e);
+  const u64x2 Y0 = *reinterpret_cast<const u64x2*>(b0 + e), Y1 = *reinterpret_cast<const u64x2*>(b1 + e);
+  u64x2 R0, R1, R2;
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    const size_t i = e + t;
-    const uint64_t x0 = a0[i], x1 = a1[i], y0 = b0[i], y1 = b1[i];
-    const uint64_t p0 = mul_barrett(x0, y0, d.p, d.mu, (int)d.nb);
-    const uint64_t p2 = mul_barrett(x1, y1, d.p, d.mu, (int)d.nb);
-    const uint64_t s = mul_barrett(add_mod(x0, x1, d.p), add_mod(y0, y1, d.p), d.p, d.mu, (int)d.nb);
-    d0[i] = p0;
-    d2[i] = p2;
-    d1[i] = sub_mod(sub_mod(s, p0, d.p), p2, d.p);
+    const uint64_t x0 = X0.v[t], x1 = X1.v[t], y0 = Y0.v[t], y1 = Y1.v[t];
+    const uint64_t p0 = mul_var(x0, y0, d.p, d.mu, d.mu2, (int)d.nb);
+    const uint64_t p2 = mul_var(x1, y1, d.p, d.mu, d.mu2, (int)d.nb);
+    const uint64_t s = mul_var(add_mod(x0, x1, d.p), add_mod(y0, y1, d.p), d.p, d.mu, d.mu2, (int)d.nb);
+    R0.v[t] = p0;
+    R2.v[t] = p2;
+    R1.v[t] = sub_mod(sub_mod(s, p0, d.p), p2, d.p);
   }
+  *reinterpret_cast<u64x2*>(d0 + e) = R0;
+  *reinterpret_cast<u64x2*>(d1 + e) = R1;
+  *reinterpret_cast<u64x2*>(d2 + e) = R2;
 }
 
 __global__ __launch_bounds__(256) void k_tensor(DevParams P, uint64_t* d0, uint64_t* d1, uint64_t* d2,

@@ -6,7 +6,8 @@
  * Every call is ordered on the HIP stream passed as `stream` (a hipStream_t,
  * NULL = default stream) and never synchronises the device.  No call
  * allocates or frees a caller buffer; scratch memory is passed in as a
- * workspace of aoa_*_workspace_bytes() bytes.
+ * workspace of aoa_*_workspace_bytes() bytes.  Ciphertext and workspace
+ * pointers must be 16-byte aligned (hipMalloc returns 256-byte alignment).
  *
  * Layouts follow the reference (aoa-ntt/include/newckks/defines.h:66-72,
  * aoa-ntt/src/arithmetic/poly_t.cpp:6-31): a Q polynomial has k limbs (primes

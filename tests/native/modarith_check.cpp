@@ -20,8 +20,9 @@ uint64_t chk_mul_barrett(uint64_t a, uint64_t b, uint64_t p, uint64_t mu, int nb
   return mul_barrett(a, b, p, mu, nb);
 }
 
-// mul_barrett and mul_shoup against the exact remainder, for every prime of a
-// parameter set; returns the number of mismatches (or -1 on bad params).
+// mul_barrett, mul_barrett62 (primes below 2^62), mul128 and mul_shoup
+// against exact 128-bit arithmetic, for every prime of a parameter set;
+// returns the number of mismatches (or -1 on bad params).
 int64_t chk_exact_mults(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
@@ -38,6 +39,15 @@ int64_t chk_exact_mults(int N, int k, int kl, int prec, int64_t trials, uint64_t
       }
       const uint64_t e = (uint64_t)(((unsigned __int128)a * b) % p);
       if (mul_barrett(a, b, p, d.mu, (int)d.nb) != e) ++bad;
+      if (p < (1ull << 62)) {
+        if (mul_barrett62(a, b, p, d.mu2, (int)d.nb) != e) ++bad;
+        const uint64_t lz = mul_barrett62_lazy(a, b, p, d.mu2, (int)d.nb);
+        if (lz >= 2 * p || lz % p != e) ++bad;
+      }
+      uint64_t hi, lo;
+      mul128(a, b, &hi, &lo);
+      const unsigned __int128 full = (unsigned __int128)a * b;
+      if (hi != (uint64_t)(full >> 64) || lo != (uint64_t)full) ++bad;
       const uint64_t wq = shoup_quot(b, p);
       const uint64_t x = rng();  // Shoup accepts any 64-bit left operand
       if (mul_shoup(x, b, wq, p) != (uint64_t)(((unsigned __int128)x * b) % p)) ++bad;
