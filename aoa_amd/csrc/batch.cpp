@@ -43,8 +43,10 @@ struct BatchGroup {
   int max_level = 0;
   NttDesc* d_fwd = nullptr;  // NTT of ModUp output (and of the rotated c1)
   int n_fwd = 0;
-  NttDesc* d_key = nullptr;  // key product + INTT
-  int n_key = 0;
+  KeyDesc2* d_key2 = nullptr;  // key product + first INTT pass, both key halves
+  int n_key2 = 0;
+  NttDesc* d_keycol = nullptr;  // second INTT pass of the key products
+  int n_keycol = 0;
   ModDownDesc* d_moddown = nullptr;
   int n_moddown = 0;
 };
@@ -86,6 +88,32 @@ static BatchGroup& group_for(aoa_batch* b, const aoa_params* p, const uint64_t* 
   return g;
 }
 
+// Key-switching product descriptors of one item: X (Lq + K limbs, NTT
+// domain) times both key halves into P0 / P1, one KeyDesc2 per limb for the
+// first inverse pass and one NttDesc per output limb for the second.
+static void add_key_descs(std::vector<KeyDesc2>& key2, std::vector<NttDesc>& keycol, const uint64_t* X,
+                          uint64_t* P0, uint64_t* P1, const uint64_t* kb, const uint64_t* ka, int Lq, int K, int k,
+                          size_t N) {
+  for (int m = 0; m < Lq + K; ++m) {
+    const int pr = m < Lq ? m : k + (m - Lq);
+    const size_t o = (size_t)m * N;
+    KeyDesc2 e;
+    e.src = X + o;
+    e.dst[0] = P0 + o;
+    e.dst[1] = P1 + o;
+    e.key[0] = kb + (size_t)pr * N;
+    e.key[1] = ka + (size_t)pr * N;
+    e.prime = pr;
+    e.pad = 0;
+    key2.push_back(e);
+  }
+  for (int y = 0; y < 2; ++y)
+    for (int m = 0; m < Lq + K; ++m) {
+      uint64_t* d = (y ? P1 : P0) + (size_t)m * N;
+      keycol.push_back({d, d, nullptr, m < Lq ? m : k + (m - Lq), 0});
+    }
+}
+
 // Workspace bytes (max over layers of the layer's sum) and descriptor bytes.
 void batch_sizes(const aoa_batch* b, size_t* ws_bytes, size_t* desc_bytes) {
   size_t ws = 0, desc = 0;
@@ -97,13 +125,13 @@ void batch_sizes(const aoa_batch* b, size_t* ws_bytes, size_t* desc_bytes) {
         const size_t Lq = it.level + 1;
         s += mul_ws_limbs((int)Lq, (int)K) * N * 8;
         desc += Lq * (sizeof(TensorDesc) + 3 * sizeof(NttDesc)) + sizeof(ModUpDesc) + K * sizeof(NttDesc) +
-                2 * (Lq + K) * sizeof(NttDesc) + 2 * sizeof(ModDownDesc);
+                (Lq + K) * sizeof(KeyDesc2) + 2 * (Lq + K) * sizeof(NttDesc) + 2 * sizeof(ModDownDesc);
       }
       for (auto& it : g.rots) {
         const size_t Lq = it.level + 1;
         s += rot_ws_limbs((int)Lq, (int)K) * N * 8;
         desc += Lq * (2 * sizeof(AutoDesc) + sizeof(NttDesc)) + sizeof(ModUpDesc) + K * sizeof(NttDesc) +
-                2 * (Lq + K) * sizeof(NttDesc) + 2 * sizeof(ModDownDesc);
+                (Lq + K) * sizeof(KeyDesc2) + 2 * (Lq + K) * sizeof(NttDesc) + 2 * sizeof(ModDownDesc);
       }
       for (auto& it : g.ntts) desc += 2 * (size_t)(it.level + 1) * sizeof(NttDesc);
       desc += 16 * 256;  // per-table alignment slack
@@ -137,7 +165,8 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
       const size_t N = (size_t)P.N;
       const int k = P.k, K = P.K;
       std::vector<TensorDesc> tens;
-      std::vector<NttDesc> prefwd, preinv, intt3, fwd, key;
+      std::vector<NttDesc> prefwd, preinv, intt3, fwd, keycol;
+      std::vector<KeyDesc2> key2;
       std::vector<ModUpDesc> mu;
       std::vector<ModDownDesc> md;
       std::vector<AutoDesc> au;
@@ -178,15 +207,9 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
         mu.push_back({D2, X + (size_t)Lq * N, lv, 0});
         for (int j = 0; j < K; ++j)
           fwd.push_back({X + (size_t)(Lq + j) * N, X + (size_t)(Lq + j) * N, nullptr, k + j, 0});
-        for (int y = 0; y < 2; ++y) {
-          uint64_t* Py = y ? P1 : P0;
-          const uint64_t* ky = y ? g.evk_a : g.evk_b;
-          for (int m = 0; m < Lq + K; ++m) {
-            const int pr = m < Lq ? m : k + (m - Lq);
-            key.push_back({X + (size_t)m * N, Py + (size_t)m * N, ky + (size_t)pr * N, pr, 0});
-          }
-          md.push_back({Py, Py + (size_t)Lq * N, y ? D1 : D0, it.out[y], lv, 1});
-        }
+        add_key_descs(key2, keycol, X, P0, P1, g.evk_b, g.evk_a, Lq, K, k, N);
+        for (int y = 0; y < 2; ++y)
+          md.push_back({y ? P1 : P0, (y ? P1 : P0) + (size_t)Lq * N, y ? D1 : D0, it.out[y], lv, 1});
       }
       for (auto& it : g.rots) {
         const int lv = it.level, Lq = lv + 1;
@@ -207,15 +230,10 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
         mu.push_back({R1, X + (size_t)Lq * N, lv, 0});
         for (int j = 0; j < K; ++j)
           fwd.push_back({X + (size_t)(Lq + j) * N, X + (size_t)(Lq + j) * N, nullptr, k + j, 0});
+        add_key_descs(key2, keycol, X, P0, P1, it.rtk_b, it.rtk_a, Lq, K, k, N);
         for (int y = 0; y < 2; ++y) {
-          uint64_t* Py = y ? P1 : P0;
-          const uint64_t* ky = y ? it.rtk_a : it.rtk_b;
-          for (int m = 0; m < Lq + K; ++m) {
-            const int pr = m < Lq ? m : k + (m - Lq);
-            key.push_back({X + (size_t)m * N, Py + (size_t)m * N, ky + (size_t)pr * N, pr, 0});
-          }
           const uint64_t* add = y == 0 ? R0 : (it.accumulate ? it.in[1] : nullptr);
-          md.push_back({Py, Py + (size_t)Lq * N, add, it.out[y], lv, 0});
+          md.push_back({y ? P1 : P0, (y ? P1 : P0) + (size_t)Lq * N, add, it.out[y], lv, 0});
         }
       }
       g.n_pre_fwd = (int)prefwd.size();
@@ -225,7 +243,8 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
       g.n_auto = (int)au.size();
       g.n_fwd = (int)fwd.size();
       g.n_modup = (int)mu.size();
-      g.n_key = (int)key.size();
+      g.n_key2 = (int)key2.size();
+      g.n_keycol = (int)keycol.size();
       g.n_moddown = (int)md.size();
       g.d_pre_fwd = (NttDesc*)(prefwd.empty() ? nullptr : put(prefwd.data(), prefwd.size() * sizeof(NttDesc)));
       g.d_pre_inv = (NttDesc*)(preinv.empty() ? nullptr : put(preinv.data(), preinv.size() * sizeof(NttDesc)));
@@ -234,7 +253,8 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
       g.d_auto = (AutoDesc*)(au.empty() ? nullptr : put(au.data(), au.size() * sizeof(AutoDesc)));
       g.d_fwd = (NttDesc*)(fwd.empty() ? nullptr : put(fwd.data(), fwd.size() * sizeof(NttDesc)));
       g.d_modup = (ModUpDesc*)(mu.empty() ? nullptr : put(mu.data(), mu.size() * sizeof(ModUpDesc)));
-      g.d_key = (NttDesc*)(key.empty() ? nullptr : put(key.data(), key.size() * sizeof(NttDesc)));
+      g.d_key2 = (KeyDesc2*)(key2.empty() ? nullptr : put(key2.data(), key2.size() * sizeof(KeyDesc2)));
+      g.d_keycol = (NttDesc*)(keycol.empty() ? nullptr : put(keycol.data(), keycol.size() * sizeof(NttDesc)));
       g.d_moddown = (ModDownDesc*)(md.empty() ? nullptr : put(md.data(), md.size() * sizeof(ModDownDesc)));
     }
   }
@@ -300,8 +320,9 @@ int batch_enqueue(aoa_batch* b, hipStream_t st, StageTimer* tm) {
     });
     run(AOA_STAGE_NTT, [](BatchGroup& g) { return g.n_fwd > 0; },
         [&](BatchGroup& g) { return ntt_forward_desc(g.params->dev, g.d_fwd, g.n_fwd, 0, st); });
-    run(AOA_STAGE_KEY_INTT, [](BatchGroup& g) { return g.n_key > 0; },
-        [&](BatchGroup& g) { return ntt_inverse_desc(g.params->dev, g.d_key, g.n_key, 0, st); });
+    run(AOA_STAGE_KEY_INTT, [](BatchGroup& g) { return g.n_key2 > 0; }, [&](BatchGroup& g) {
+      return ntt_inverse_key2_desc(g.params->dev, g.d_key2, g.n_key2, g.d_keycol, g.n_keycol, 0, st);
+    });
     run(AOA_STAGE_MODDOWN, [](BatchGroup& g) { return g.n_moddown > 0; }, [&](BatchGroup& g) {
       launch_moddown_desc(g.params->dev, g.d_moddown, g.n_moddown, st);
       return 0;

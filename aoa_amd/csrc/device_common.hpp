@@ -68,6 +68,17 @@ struct NttDesc {
   int pad;
 };
 
+// Key switching's inverse first pass with both key halves: dst[y] = first
+// inverse pass of (src * key[y]) for y = 0, 1 (the column pass then runs on
+// dst[0] and dst[1] as ordinary NttDesc entries).
+struct KeyDesc2 {
+  const uint64_t* src;
+  uint64_t* dst[2];
+  const uint64_t* key[2];
+  int prime;
+  int pad;
+};
+
 struct TensorDesc {
   const uint64_t* a0;
   const uint64_t* a1;

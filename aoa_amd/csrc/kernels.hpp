@@ -62,6 +62,10 @@ void launch_rho(const DevParams& P, uint64_t* out, const uint64_t* in, hipStream
 // every descriptor of the table (device memory).
 int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st);
 int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st);
+// Key-switching inverse NTT: first pass from KeyDesc2 entries (one input limb,
+// two keys, two outputs), second pass over the NttDesc list of outputs.
+int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const NttDesc* dcol, int ncol, int N1,
+                          hipStream_t st);
 void launch_tensor_desc(const DevParams& P, const TensorDesc* d, int ndesc, hipStream_t st);
 void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int max_level, hipStream_t st);
 void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, hipStream_t st);

@@ -189,32 +189,13 @@ __device__ __forceinline__ void fwd_row_body(const DevParams& P, const uint64_t*
 // Row pass of the inverse: global stages g0+log2(S)-1 .. g0.  With a key, the
 // input limb is first multiplied by the key limb (key switching:
 // INTT(x * key_b), INTT(x * key_a) from one input).
+// The inverse row pass after the input limb is in registers (x[k] = element
+// sl + TS*k of the thread's row): stages g0+log2(S)-1 .. g0, result to dst.
 template <int S, bool LZ>
-__device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t* src, uint64_t* dst,
-                                             const uint64_t* key, int pr, int tile, int g0, uint64_t* lds) {
+__device__ __forceinline__ void inv_row_core(const DevParams& P, uint64_t (&x)[16], uint64_t* dst, uint64_t p,
+                                             const uint64_t* T, int row, int sl, int g0, uint64_t* L) {
   using G = Geo<S>;
-  constexpr int C = G::C, TS = G::TS, LOGS = G::LOGS, PS = S + S / 16;
-  const int row0 = tile * C;
-  const DevPrime dp = P.primes[pr];
-  const uint64_t p = dp.p;
-  const uint64_t* T = P.tw_inv + (size_t)pr * P.N * 2;
-  const int tid = threadIdx.x;
-  const int rl = tid / TS;
-  const int sl = tid % TS;
-  const int row = row0 + rl;
-  src += (size_t)row * S;
-  dst += (size_t)row * S;
-  uint64_t* L = lds + rl * PS;
-  uint64_t x[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) x[k] = src[sl + TS * k];
-  if (key) {
-    const uint64_t* kp = key + (size_t)row * S;
-#pragma unroll
-    for (int k = 0; k < 16; ++k)
-      x[k] = LZ ? mul_barrett62_lazy(x[k], kp[sl + TS * k], p, dp.mu2, (int)dp.nb)
-                : mul_barrett(x[k], kp[sl + TS * k], p, dp.mu, (int)dp.nb);
-  }
+  constexpr int TS = G::TS, LOGS = G::LOGS;
 #pragma unroll
   for (int k = 0; k < 16; ++k) L[swz(sl + TS * k)] = x[k];
   __syncthreads();
@@ -250,6 +231,62 @@ __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t*
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) dst[sl + TS * k] = x[k];
+}
+
+// x * key mod p for a key-switching product (LZ: result in [0, 2p), which the
+// lazy inverse butterflies accept).
+template <bool LZ>
+__device__ __forceinline__ uint64_t key_mul(uint64_t x, uint64_t k, const DevPrime& dp) {
+  return LZ ? mul_barrett62_lazy(x, k, dp.p, dp.mu2, (int)dp.nb) : mul_barrett(x, k, dp.p, dp.mu, (int)dp.nb);
+}
+
+// Row pass of the inverse: global stages g0+log2(S)-1 .. g0.  With a key, the
+// input limb is first multiplied by the key limb.
+template <int S, bool LZ>
+__device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t* src, uint64_t* dst,
+                                             const uint64_t* key, int pr, int tile, int g0, uint64_t* lds) {
+  using G = Geo<S>;
+  constexpr int C = G::C, TS = G::TS, PS = S + S / 16;
+  const DevPrime dp = P.primes[pr];
+  const uint64_t* T = P.tw_inv + (size_t)pr * P.N * 2;
+  const int rl = threadIdx.x / TS, sl = threadIdx.x % TS;
+  const int row = tile * C + rl;
+  src += (size_t)row * S;
+  uint64_t x[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = src[sl + TS * k];
+  if (key) {
+    const uint64_t* kp = key + (size_t)row * S;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(x[k], kp[sl + TS * k], dp);
+  }
+  inv_row_core<S, LZ>(P, x, dst + (size_t)row * S, dp.p, T, row, sl, g0, lds + rl * PS);
+}
+
+// Key switching's two products from one input limb: dst0 = first inverse
+// pass of (src * key0), dst1 = same with key1.  The workgroup reads its src
+// tile twice; the second read is served by the L2 cache, so HBM sees the
+// input once while the registers hold one tile at a time.
+template <int S, bool LZ>
+__device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint64_t* src, uint64_t* dst0,
+                                                  uint64_t* dst1, const uint64_t* key0, const uint64_t* key1,
+                                                  int pr, int tile, int g0, uint64_t* lds) {
+  using G = Geo<S>;
+  constexpr int C = G::C, TS = G::TS, PS = S + S / 16;
+  const DevPrime dp = P.primes[pr];
+  const uint64_t* T = P.tw_inv + (size_t)pr * P.N * 2;
+  const int rl = threadIdx.x / TS, sl = threadIdx.x % TS;
+  const int row = tile * C + rl;
+  const size_t off = (size_t)row * S;
+#pragma unroll
+  for (int y = 0; y < 2; ++y) {
+    const uint64_t* kp = (y ? key1 : key0) + off;
+    uint64_t x[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(src[off + sl + TS * k], kp[sl + TS * k], dp);
+    if (y) __syncthreads();  // the row's LDS block is reused
+    inv_row_core<S, LZ>(P, x, (y ? dst1 : dst0) + off, dp.p, T, row, sl, g0, lds + rl * PS);
+  }
 }
 
 // Column pass of the inverse: global stages log2(S)-1 .. 0, N^-1 folded into
@@ -407,6 +444,21 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_d(DevParams P, const NttDes
 }
 
 template <int S>
+__global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const KeyDesc2* d, int g0) {
+  __shared__ uint64_t lds[Geo<S>::ROW_LDS];
+  const int tiles = (P.N / S) / Geo<S>::C;
+  const KeyDesc2& e = d[blockIdx.x / tiles];
+  const uint64_t* src = e.src;
+  uint64_t *d0 = e.dst[0], *d1 = e.dst[1];
+  const uint64_t *k0 = e.key[0], *k1 = e.key[1];
+  const int pr = e.prime;
+  if (lazy_prime(P, pr))
+    inv_row_key2_body<S, true>(P, src, d0, d1, k0, k1, pr, blockIdx.x % tiles, g0, lds);
+  else
+    inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, pr, blockIdx.x % tiles, g0, lds);
+}
+
+template <int S>
 __global__ __launch_bounds__(256) void k_ntt_inv_col_d(DevParams P, const NttDesc* d, int N2) {
   __shared__ uint64_t lds[Geo<S>::COL_LDS];
   const int tiles = N2 / Geo<S>::C;
@@ -539,6 +591,26 @@ int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
   {
     const int tiles = N2 / (4096 / N1);
 #define CALLC(S) k_ntt_inv_col_d<S><<<dim3(tiles * ndesc), 256, 0, st>>>(P, d, N2)
+    AOA_DISPATCH_S(N1, CALLC)
+#undef CALLC
+  }
+  return 0;
+}
+
+int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const NttDesc* dcol, int ncol, int N1,
+                          hipStream_t st) {
+  N1 = pick_n1(P, N1);
+  if (!N1 || n2 <= 0 || ncol <= 0) return -1;
+  const int N2 = P.N / N1, g0 = log2i(N1);
+  {
+    const int tiles = N1 / (4096 / N2);
+#define CALLR(S) k_ntt_inv_row_key2_d<S><<<dim3(tiles * n2), 256, 0, st>>>(P, d2, g0)
+    AOA_DISPATCH_S(N2, CALLR)
+#undef CALLR
+  }
+  {
+    const int tiles = N2 / (4096 / N1);
+#define CALLC(S) k_ntt_inv_col_d<S><<<dim3(tiles * ncol), 256, 0, st>>>(P, dcol, N2)
     AOA_DISPATCH_S(N1, CALLC)
 #undef CALLC
   }

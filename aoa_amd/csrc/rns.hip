@@ -262,12 +262,15 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
     uint64_t acc = 0;
     if (dq.nb <= 56) {
       const uint64_t thr = lazy_window(dq, P.moddown_exact);
+      const uint64_t pthr = dq.p + thr;
       bool hit = false;
 #pragma unroll 4
       for (int j = j0; j < j1; ++j) {
         const tw2 w = ld_tw2(ci, j);
         const uint64_t r = mul_shoup_lazy(aux[j * BC_COEF + c], w.w, w.q, dq.p);
-        hit |= (r < thr) | (r - dq.p < thr);
+        // r in [0, 2p): its canonical value is below thr iff r < thr or
+        // p <= r < p + thr (three compares, no subtraction)
+        hit |= (r < thr) | ((r >= dq.p) & (r < pthr));
         acc += r;
       }
       if (hit) {
@@ -298,6 +301,7 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
     const tw2* c2 = bq + (size_t)i2 * K;
     const uint64_t t1 = lazy_window(d1, P.moddown_exact);
     const uint64_t t2 = lazy_window(d2, P.moddown_exact);
+    const uint64_t pt1 = d1.p + t1, pt2 = d2.p + t2;
     uint64_t a1 = 0, a2 = 0;
     bool hit = false;
 #pragma unroll 4
@@ -306,7 +310,7 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
       const tw2 w1 = ld_tw2(c1, j), w2 = ld_tw2(c2, j);
       const uint64_t x1 = mul_shoup_lazy(a, w1.w, w1.q, d1.p);
       const uint64_t x2 = mul_shoup_lazy(a, w2.w, w2.q, d2.p);
-      hit |= (x1 < t1) | (x1 - d1.p < t1) | (x2 < t2) | (x2 - d2.p < t2);
+      hit |= (x1 < t1) | ((x1 >= d1.p) & (x1 < pt1)) | (x2 < t2) | ((x2 >= d2.p) & (x2 < pt2));
       a1 += x1;
       a2 += x2;
     }
