@@ -24,6 +24,9 @@ SIGNATURES = {
     "aoa_ntt": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "aoa_eltwise": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_vp, c_vp, c_vp, c_vp, ctypes.c_int,
                                     ctypes.c_int]),
+    "aoa_eltwise_scalar": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_uint64, ctypes.c_int,
+                                          ctypes.c_int]),
+    "aoa_conjugate": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int]),
     "aoa_rescale": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int]),
     "aoa_modup": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int]),
     "aoa_moddown": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int]),

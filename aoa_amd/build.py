@@ -61,11 +61,14 @@ def build_newckks(verbose=False):
     """libnewckks.so: the newckks-compatible C++ API, a host library over the
     C-ABI of libaoa.so; plus the C++ test program written against it."""
     inc = ["-I" + os.path.join(ROOT, "include")]
-    obj = os.path.join(OBJ, "newckks.o")
-    _run([HIPCC, "-O2", "-fPIC", "-std=c++17", "-Wall"] + inc +
-         ["-c", os.path.join(CSRC, "newckks", "newckks.cpp"), "-o", obj])
+    objs = []
+    for src in ("newckks.cpp", "newckks_ops.cpp"):
+        obj = os.path.join(OBJ, src.replace(".cpp", ".o"))
+        _run([HIPCC, "-O2", "-fPIC", "-std=c++17", "-Wall"] + inc +
+             ["-c", os.path.join(CSRC, "newckks", src), "-o", obj])
+        objs.append(obj)
     tmp = NEWCKKS_LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "-o", tmp, obj, "-L" + HERE, "-laoa", "-Wl,-rpath,$ORIGIN"])
+    _run([HIPCC, "-shared", "-fPIC", "-o", tmp] + objs + ["-L" + HERE, "-laoa", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, NEWCKKS_LIB)
     os.makedirs(os.path.dirname(CPP_TEST), exist_ok=True)
     _run([HIPCC, "-O2", "-std=c++17"] + inc + [os.path.join(ROOT, "tests", "cpp", "test_newckks.cpp"), "-o",

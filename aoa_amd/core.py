@@ -85,6 +85,13 @@ class Params:
         check(self.lib.aoa_eltwise(self.h, _stream(stream), op, _ptr(out), _ptr(a), _ptr(b), _ptr(c), res0,
                                    nres))
 
+    def eltwise_scalar(self, op, out, a, b, res0=0, nres=None, stream=None):
+        nres = nres if nres is not None else self.k
+        check(self.lib.aoa_eltwise_scalar(self.h, _stream(stream), op, _ptr(out), _ptr(a), int(b), res0, nres))
+
+    def conjugate(self, out, inp, nres, stream=None):
+        check(self.lib.aoa_conjugate(self.h, _stream(stream), _ptr(out), _ptr(inp), nres))
+
     def rescale(self, c0, c1, level, stream=None):
         check(self.lib.aoa_rescale(self.h, _stream(stream), _ptr(c0), _ptr(c1), level))
 

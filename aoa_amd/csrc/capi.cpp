@@ -205,6 +205,25 @@ extern "C" int aoa_eltwise(const aoa_params* h, void* stream, int op, uint64_t* 
   return check_launch("aoa_eltwise");
 }
 
+extern "C" int aoa_eltwise_scalar(const aoa_params* h, void* stream, int op, uint64_t* out, const uint64_t* a,
+                                  uint64_t b, int res0, int nres) {
+  if (!h || !out || !a) return fail(AOA_ERR_ARG, "null argument");
+  if (op != AOA_ADD && op != AOA_SUB && op != AOA_MUL) return fail(AOA_ERR_ARG, "unknown operation");
+  const int R = h->host.k + h->host.K;
+  if (res0 < 0 || nres <= 0 || res0 + nres > R) return fail(AOA_ERR_ARG, "limb range out of bounds");
+  const size_t off = (size_t)res0 * h->host.N;
+  launch_eltwise_scalar(h->dev, op, out + off, a + off, b, nres, res0, (hipStream_t)stream);
+  return check_launch("aoa_eltwise_scalar");
+}
+
+extern "C" int aoa_conjugate(const aoa_params* h, void* stream, uint64_t* out, const uint64_t* in, int nres) {
+  if (!h || !out || !in) return fail(AOA_ERR_ARG, "null argument");
+  if (out == in) return fail(AOA_ERR_ARG, "out and in must not overlap");
+  if (nres <= 0 || nres > h->host.k + h->host.K) return fail(AOA_ERR_ARG, "limb count out of range");
+  launch_conjugate(h->dev, out, in, nres, (hipStream_t)stream);
+  return check_launch("aoa_conjugate");
+}
+
 extern "C" int aoa_rescale(const aoa_params* h, void* stream, uint64_t* c0, uint64_t* c1, int level) {
   if (!h || !c0 || !c1) return fail(AOA_ERR_ARG, "null argument");
   if (level == 0) return fail(AOA_ERR_LEVEL, "The ciphertext level is 0. No more rescaling is possible.");

@@ -30,6 +30,9 @@ void launch_tensor(const DevParams& P, uint64_t* d0, uint64_t* d1, uint64_t* d2,
 
 // ModUp: x holds Q limbs 0..level (coefficient domain, stride N); writes K
 // special limbs to outB (stride N).
+void launch_eltwise_scalar(const DevParams& P, int op, uint64_t* out, const uint64_t* a, uint64_t b, int nlimbs,
+                           int prime0, hipStream_t st);
+void launch_conjugate(const DevParams& P, uint64_t* out, const uint64_t* in, int nlimbs, hipStream_t st);
 void launch_modup(const DevParams& P, const uint64_t* x, uint64_t* outB, int level, hipStream_t st);
 
 // ModDown of npolys polys.  inQ[y] holds Q limbs 0..level, inB[y] the K

@@ -440,6 +440,17 @@ void rotate_slots_right(Context* ctx, poly_t* b, poly_t* a, int rotSlots) {
   rotate_slots_left(ctx, b, a, (CUDAManager::N >> 1) - rotSlots);
 }
 
+void conjugate_slots(Context* ctx, poly_t* a) {  // encoder.cu:404-427
+  COMMONEngine::execute(ctx, a, FORWARD);
+  poly_t t;
+  poly_init(ctx, &t, a->base);
+  ok(aoa_conjugate(CUDAManager::params(), ctx->get_stream(), t.d_coefs, a->d_coefs,
+                   CUDAManager::get_n_residues(a->base)));
+  hip_ok(hipMemcpyAsync(a->d_coefs, t.d_coefs, poly_get_size(a->base), hipMemcpyDeviceToDevice, ctx->get_stream()),
+         "hipMemcpyAsync");
+  poly_free(ctx, &t);
+}
+
 // ------------------------------------------------------------ CKKSContext --
 
 CKKSContext::CKKSContext() : Context() {
@@ -450,6 +461,25 @@ CKKSContext::CKKSContext() : Context() {
   poly_init(this, m);
   poly_init(this, e);
   for (auto& c : caux) cipher_init(this, &c);
+  h_val = new std::complex<double>[CUDAManager::N];
+  // Maclaurin series (ckkscontext.h:101-135)
+  maclaurin_coeffs[EXPONENT] = new double[11]{1, 1, 1. / 2, 1. / 6, 1. / 24, 1. / 120, 1. / 720, 1. / 5040,
+                                              1. / 40320, 1. / 362880, 1. / 3628800};
+  maclaurin_coeffs[SIN] = new double[12]{0, 1, 0, -1. / 6, 0, 1. / 120, 0, -1. / 5040, 0, 1. / 362880, 0,
+                                         -1. / 39916800};
+  maclaurin_coeffs[COS] = new double[12]{1, 0, -0.5, 1. / 24, 0, -1. / 720, 0, 1. / 40320, 0, -1. / 362880, 0,
+                                         1. / 362880};
+  maclaurin_coeffs[SIGMOID] = new double[10]{1. / 2, 1. / 4, 0, -1. / 48, 0, 1. / 480, 0, -17. / 80640, 0,
+                                             31. / 1451520};
+  {
+    double* l1 = new double[21];
+    l1[0] = 0;
+    for (int i = 1; i < 21; ++i) l1[i] = -1. / i;
+    maclaurin_coeffs[LN1MINUS] = l1;
+    double* lg = new double[40];
+    for (int i = 0; i < 40; ++i) lg[i] = (i % 2) ? 1. / i : 0;
+    maclaurin_coeffs[LOGARITHM] = lg;
+  }
   const int top = (int)CUDAManager::RNSQPrimes.size() - 1;
   ws_bytes = aoa_workspace_bytes(CUDAManager::params(), top);
   hip_ok(hipMalloc(&d_ws, ws_bytes), "hipMalloc");
@@ -458,6 +488,8 @@ CKKSContext::CKKSContext() : Context() {
 CKKSContext::~CKKSContext() {
   (void)hipStreamSynchronize(get_stream());
   for (auto& c : caux) cipher_free(this, &c);
+  delete[] h_val;
+  for (auto& kv : maclaurin_coeffs) delete[] kv.second;
   poly_free(this, u);
   poly_free(this, m);
   poly_free(this, e);
@@ -567,6 +599,16 @@ std::map<int, RotationKey*> ckks_new_rtk_right(Context* ctx, SecretKey* sk) {  /
   return r;
 }
 
+ConjugationKey* ckks_new_cjk(Context* ctx, SecretKey* sk) {  // ckkskeys.cpp:183-193
+  poly_t s2;
+  poly_init(ctx, &s2, QBBase);
+  poly_copy(ctx, &s2, &sk->s);
+  conjugate_slots(ctx, &s2);
+  ConjugationKey* cjk = ckks_new_evk(ctx, &s2, &sk->s);
+  poly_free(ctx, &s2);
+  return cjk;
+}
+
 static void free_swk(Context* ctx, SwitchKey* k) {
   if (!k) return;
   poly_free(ctx, &k->a);
@@ -593,6 +635,7 @@ CKKSKeychain* ckks_keygen(CKKSContext* ctx, SecretKey* sk) {  // ckks.cpp:46-75
   keys->evk = ckks_new_mtk(ctx, sk);
   keys->rtk_left = ckks_new_rtk_left(ctx, sk);
   keys->rtk_right = ckks_new_rtk_right(ctx, sk);
+  keys->cjk = ckks_new_cjk(ctx, sk);
   keys->sk = sk;
   ctx->keys = *keys;
   return keys;
@@ -618,8 +661,11 @@ cipher_t* ckks_encrypt_poly(CKKSContext* ctx, poly_t* m) {
   return ckks_encrypt_poly(ctx, ct, m);
 }
 
+// Encodes at ct's current scale (ckks.cpp:131-148); ckks_encrypt_poly then
+// resets level and scale, as the reference does.  A never-initialised ct is
+// initialised first.
 cipher_t* ckks_encrypt(CKKSContext* ctx, cipher_t* ct, std::complex<double>* val, int slots, int empty_slots) {
-  cipher_init(ctx, ct, slots + empty_slots);
+  if (ct->c[0].state == NONINITIALIZED || ct->c[1].state == NONINITIALIZED) cipher_init(ctx, ct, slots + empty_slots);
   ctx->encode(ctx->m, &ct->scale, val, slots, empty_slots);
   ckks_encrypt_poly(ctx, ct, ctx->m);
   ct->slots = slots + empty_slots;

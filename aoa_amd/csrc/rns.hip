@@ -58,6 +58,57 @@ void launch_eltwise(const DevParams& P, int op, uint64_t* out, const uint64_t* a
   k_eltwise<<<dim3((unsigned)(total / 512)), 256, 0, st>>>(P, op, out, a, b, c, qlimbs, prime0);
 }
 
+// out = a op b for a 64-bit scalar b used as given (not reduced), with the
+// reference's own formulas (aoa-ntt/src/cuda/htrans/ntt.cu:531-558,
+// aoa-ntt/include/newckks/cuda/htrans/common.h:83-90, common.cu:174-193):
+// add and subtract by one conditional correction in wrapping 64-bit
+// arithmetic, multiply by its Barrett reduction of the 128-bit product.
+// These are what ckks_add / ckks_mul by a constant feed to the residues, so
+// the literal forms keep them bit-identical even when b exceeds the prime.
+__global__ __launch_bounds__(256) void k_eltwise_scalar(DevParams P, int op, uint64_t* out, const uint64_t* a,
+                                                        uint64_t b, int prime0) {
+  const size_t e0 = (size_t)blockIdx.x * 512;
+  const int li = (int)(e0 >> P.logN);
+  const DevPrime d = P.primes[prime0 + li];
+  const size_t e = e0 + (size_t)threadIdx.x * 2;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const uint64_t x = a[e + t];
+    uint64_t r;
+    if (op == ELT_ADD) {
+      const uint64_t s = x + b;
+      r = s >= d.p ? s - d.p : s;
+    } else if (op == ELT_SUB) {
+      r = x < b ? x + (d.p - b) : x - b;
+    } else {
+      r = ref_barrett(mulhi64(x, b), x * b, d.p, d.mu, (int)d.nb);
+    }
+    out[e + t] = r;
+  }
+}
+
+void launch_eltwise_scalar(const DevParams& P, int op, uint64_t* out, const uint64_t* a, uint64_t b, int nlimbs,
+                           int prime0, hipStream_t st) {
+  const size_t total = (size_t)nlimbs * P.N;
+  k_eltwise_scalar<<<dim3((unsigned)(total / 512)), 256, 0, st>>>(P, op, out, a, b, prime0);
+}
+
+// Conjugation in the NTT domain: out[cid] = in[N-1-cid] per limb
+// (aoa-ntt/src/ckks/encoder.cu:391-401).  With the bit-reversed evaluation
+// order, reversing the index maps the evaluation at psi^e to psi^-e, i.e. the
+// automorphism X -> X^-1.
+__global__ __launch_bounds__(256) void k_conjugate(DevParams P, uint64_t* out, const uint64_t* in) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const size_t base = (e >> P.logN) << P.logN;
+  const size_t cid = e - base;
+  out[e] = in[base + ((size_t)P.N - 1 - cid)];
+}
+
+void launch_conjugate(const DevParams& P, uint64_t* out, const uint64_t* in, int nlimbs, hipStream_t st) {
+  const size_t total = (size_t)nlimbs * P.N;
+  k_conjugate<<<dim3((unsigned)(total / 256)), 256, 0, st>>>(P, out, in);
+}
+
 // d0 = a0 b0, d2 = a1 b1, d1 = (a0 + a1)(b0 + b1) - d0 - d2 (= a0 b1 + a1 b0),
 // two adjacent elements per thread of one limb (pointers at element e of the
 // limb, e even), moved as 16-byte vectors.

@@ -79,6 +79,22 @@ int aoa_ntt(const aoa_params* params, void* stream, uint64_t* d_data, int res0, 
 int aoa_eltwise(const aoa_params* params, void* stream, int op, uint64_t* out, const uint64_t* a,
                 const uint64_t* b, const uint64_t* c, int res0, int nres);
 
+/* out = a op b with a 64-bit scalar b (op AOA_ADD, AOA_SUB or AOA_MUL) on
+ * limbs res0 .. res0+nres-1, b used as given, not reduced: the reference's
+ * formulas, including their results when b >= q_r (one conditional
+ * correction for add/sub, its Barrett reduction for mul).  Replaces
+ * NTTEngine::execute_op_by_uint (aoa-ntt/src/cuda/htrans/ntt.cu:531-580), the
+ * kernel under poly_add / poly_sub / poly_mul by a uint64_t
+ * (aoa-ntt/src/arithmetic/poly_t.cpp:157-235). */
+int aoa_eltwise_scalar(const aoa_params* params, void* stream, int op, uint64_t* out, const uint64_t* a, uint64_t b,
+                       int res0, int nres);
+
+/* Conjugation of limbs 0..nres-1 in the NTT domain: out[i] = in[N-1-i] per
+ * limb, the automorphism X -> X^-1 in the reference's bit-reversed order.
+ * Replaces execute_conjugation (aoa-ntt/src/ckks/encoder.cu:391-427).  out
+ * and in must not overlap. */
+int aoa_conjugate(const aoa_params* params, void* stream, uint64_t* out, const uint64_t* in, int nres);
+
 /* Rescale c0 and c1 in place from `level` (coefficient domain): limb i < level
  * becomes (c_i - c_level) q_level^-1 mod q_i.  Replaces
  * CUDAManager::execute_rescale (aoa-ntt/src/cuda/manager.cu:90-139). */

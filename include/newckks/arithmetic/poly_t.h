@@ -24,6 +24,12 @@ void poly_mul(Context* ctx, poly_t* c, poly_t* a, poly_t* b);
 void poly_add_add(Context* ctx, poly_t* c1, poly_t* a1, poly_t* b1, poly_t* c2, poly_t* a2, poly_t* b2);
 void poly_mul_add(Context* ctx, poly_t* d, poly_t* a, poly_t* b, poly_t* c);
 void poly_negate(Context* ctx, poly_t* a);
+// By a 64-bit scalar used as given (poly_t.cpp:157-235): add/sub in the NTT
+// domain (the scalar lands on the constant coefficient), mul in the
+// coefficient domain.
+void poly_add(Context* ctx, poly_t* c, poly_t* a, uint64_t b);
+void poly_sub(Context* ctx, poly_t* c, poly_t* a, uint64_t b);
+void poly_mul(Context* ctx, poly_t* c, poly_t* a, uint64_t b);
 void poly_rho(Context* ctx, poly_t* b, poly_t* a);
 void poly_modup(Context* ctx, poly_t* a, poly_t* b, int level);
 void poly_moddown(Context* ctx, poly_t* a, poly_t* b, int level);

@@ -10,6 +10,17 @@
 #include "cipher_t.h"
 #include "ckkskeys.h"
 
+#include <map>
+#include <string>
+
+// Names of the precomputed Maclaurin series (ckkscontext.h:22-27).
+static const std::string LN1MINUS = "Log1Minus";  // log(1-x)
+static const std::string LOGARITHM = "Logarithm";  // log(x)
+static const std::string EXPONENT = "Exponent";    // exp(x)
+static const std::string SIN = "Sin";              // sin(x)
+static const std::string COS = "Cos";              // cos(x)
+static const std::string SIGMOID = "Sigmoid";      // 1/(1+exp(-x))
+
 class CKKSContext : public Context {
  public:
   CKKSKeychain keys;
@@ -17,6 +28,8 @@ class CKKSContext : public Context {
   cipher_t caux[3];   // auxiliary ciphertexts (sumslots)
   void* d_ws;         // workspace of aoa_hommul / aoa_rotate at the top level
   size_t ws_bytes;
+  std::complex<double>* h_val;                       // encoding scratch (N values)
+  std::map<std::string, double*> maclaurin_coeffs;  // series of exp, sin, cos, sigmoid, logs
 
   CKKSContext();
   ~CKKSContext();

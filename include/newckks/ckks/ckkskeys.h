@@ -28,6 +28,8 @@ std::map<int, RotationKey*> ckks_new_rtk_left(Context* ctx, SecretKey* sk);
 std::map<int, RotationKey*> ckks_new_rtk_right(Context* ctx, SecretKey* sk);
 // Rotation keys for the given steps only (the reference builds every power of two < N).
 std::map<int, RotationKey*> ckks_new_rtk_left(Context* ctx, SecretKey* sk, const std::vector<int>& steps);
+// Conjugation key: switching key from conj(s) to s (ckkskeys.cpp:183-193).
+ConjugationKey* ckks_new_cjk(Context* ctx, SecretKey* sk);
 void ckkskeychain_free(Context* ctx, CKKSKeychain* k);
 
 #endif

@@ -18,5 +18,7 @@ void ckks_encode(Context* ctx, uint64_t* d_output, std::complex<double>* h_input
 void ckks_decode(Context* ctx, std::complex<double>* h_val, uint64_t* d_a, int slots, uint64_t scalingfactor);
 void rotate_slots_left(Context* ctx, poly_t* b, poly_t* a, int rotSlots);
 void rotate_slots_right(Context* ctx, poly_t* b, poly_t* a, int rotSlots);
+// In place, NTT domain afterwards (encoder.cu:404-427).
+void conjugate_slots(Context* ctx, poly_t* a);
 
 #endif

@@ -454,6 +454,25 @@ static void vec_op(const Oracle& o, int op, uint64_t* c, const uint64_t* a, cons
     }
   }
 }
+static void vec_op_int(const Oracle& o, int op, uint64_t* c, const uint64_t* a, uint64_t b, int nres) {
+  // ntt_vector_op_int, ntt.cu:531-558: the scalar is used as given (not
+  // reduced), with the same SEL-based add/sub and Barrett mulmod.
+  const size_t n = (size_t)nres * o.N;
+#pragma omp parallel for
+  for (size_t t = 0; t < n; ++t) {
+    const int rid = (int)(t / o.N);
+    switch (op) {
+      case 0: c[t] = addmod(o, a[t], b, rid); break;
+      case 1: c[t] = submod(o, a[t], b, rid); break;
+      case 2: c[t] = mulmod(o, a[t], b, rid); break;
+    }
+  }
+}
+static void conjugation_kernel(const Oracle& o, uint64_t* b, const uint64_t* a, int nres) {
+  // execute_conjugation, encoder.cu:391-401 (NTT domain): b[cid] = a[N-1-cid]
+  for (int r = 0; r < nres; ++r)
+    for (int cid = 0; cid < o.N; ++cid) b[(size_t)r * o.N + cid] = a[(size_t)r * o.N + (o.N - 1 - cid)];
+}
 static void poly_add_add(const Oracle& o, Poly& c1, Poly& a1, Poly& b1, Poly& c2, Poly& a2, Poly& b2) {
   // poly_t.cpp:237-272
   const int base = std::min(std::min(a1.base, b1.base), std::min(a2.base, b2.base));
@@ -731,6 +750,12 @@ uint64_t orc_mulmod(void* h, int rid, uint64_t a, uint64_t b) { return mulmod(*(
 void orc_ntt(void* h, uint64_t* d, int res0, int nres, int dir) { transform(*(Oracle*)h, d, res0, nres, dir); }
 void orc_eltwise(void* h, int op, uint64_t* c, const uint64_t* a, const uint64_t* b, int nres) {
   vec_op(*(Oracle*)h, op, c, a, b, nres);
+}
+void orc_eltwise_scalar(void* h, int op, uint64_t* c, const uint64_t* a, uint64_t b, int nres) {
+  vec_op_int(*(Oracle*)h, op, c, a, b, nres);
+}
+void orc_conjugate(void* h, uint64_t* out, const uint64_t* in, int nres) {
+  conjugation_kernel(*(Oracle*)h, out, in, nres);
 }
 void orc_rescale(void* h, uint64_t* a, uint64_t* b, int level) { rescale_kernel(*(Oracle*)h, a, b, level); }
 void orc_modup(void* h, uint64_t* qb, int level) { modup_kernel(*(Oracle*)h, qb, level); }

@@ -279,11 +279,279 @@ static void mul_n4096_k4() {
   CUDAManager::destroy();
 }
 
+// Plaintext / constant operands, conjugation and the compositions
+// (tests.cpp:1224-1535, :1651-1700, :1762-2830) on one parameter set with
+// enough levels (the reference's advanced tests use k=30, N=4096).
+static void run_adv(int k, int kl, int N, int prec) {
+  CUDAManager::init(k, kl, N, prec);
+  CKKSContext* ctx = new CKKSContext();
+  SecretKey* sk = ckks_new_sk(ctx);
+  CKKSKeychain* keys = new CKKSKeychain;
+  keys->pk = ckks_new_pk(ctx, sk);
+  keys->evk = ckks_new_mtk(ctx, sk);
+  std::vector<int> steps;
+  for (int r = 1; r < (N >> 1); r *= 2) steps.push_back(r);
+  keys->rtk_left = ckks_new_rtk_left(ctx, sk, steps);
+  keys->rtk_right = ckks_new_rtk_right(ctx, sk);
+  keys->cjk = ckks_new_cjk(ctx, sk);
+  ctx->keys = *keys;
+  const int slots = N >> 1;
+  const std::string tag = "N" + std::to_string(N) + "_k" + std::to_string(k) + "_prec" + std::to_string(prec);
+  std::default_random_engine gen;
+  const double R = get_range(prec);
+  std::uniform_real_distribution<double> dh(-R / 2, R / 2), dsq(-std::sqrt(R / 2), std::sqrt(R / 2)), d01(0, 1),
+      dm1(-1, 1);
+  std::vector<cd> v1(slots), v2(slots), exp(slots);
+  auto done = [&](std::initializer_list<cipher_t*> cs) {
+    for (cipher_t* x : cs) {
+      cipher_free(ctx, x);
+      delete x;
+    }
+  };
+  // AddPlaintextDouble (tests.cpp:1224-1282)
+  {
+    for (auto& x : v1) x = cd(dh(gen), dh(gen));
+    const double c = dh(gen);
+    cipher_t* a = ckks_encrypt(ctx, v1.data(), slots);
+    cipher_t* b = ckks_add(ctx, a, c);
+    cd* out = ckks_decrypt(ctx, b, sk);
+    for (int i = 0; i < slots; ++i) exp[i] = v1[i] + c;
+    report("AddPlaintextDouble_" + tag, maxerr(exp.data(), out, slots), ERRBOUND);
+    free(out);
+    done({a, b});
+  }
+  // MulByConstantDouble, both signs, and DivByConstant (tests.cpp:1284-1344, :1475-1535)
+  {
+    for (auto& x : v1) x = cd(dsq(gen), dsq(gen));
+    double worst = 0;
+    for (double c : {std::abs(dsq(gen)), -std::abs(dsq(gen))}) {
+      cipher_t* a = ckks_encrypt(ctx, v1.data(), slots);
+      cipher_t* b = ckks_mul(ctx, a, c);
+      cd* out = ckks_decrypt(ctx, b, sk);
+      for (int i = 0; i < slots; ++i) exp[i] = v1[i] * c;
+      worst = std::max(worst, maxerr(exp.data(), out, slots));
+      free(out);
+      done({a, b});
+    }
+    report("MulByConstantDouble_" + tag, worst, ERRBOUND);
+    double c = dsq(gen);
+    c += 0.1 * (std::abs(1.0 / c) >= R / 2);
+    for (auto& x : v1) x = cd(dsq(gen), 0);
+    cipher_t* a = ckks_encrypt(ctx, v1.data(), slots);
+    cipher_t* b = ckks_mul(ctx, a, 1.0 / c);
+    cd* out = ckks_decrypt(ctx, b, sk);
+    for (int i = 0; i < slots; ++i) exp[i] = v1[i] / c;
+    report("DivByConstant_" + tag, maxerr(exp.data(), out, slots), ERRBOUND);
+    free(out);
+    done({a, b});
+  }
+  // MulByConstantArray and MulByConstantPoly (tests.cpp:1346-1473)
+  {
+    for (int i = 0; i < slots; ++i) {
+      v1[i] = cd(dsq(gen), dsq(gen));
+      v2[i] = cd(dsq(gen), dsq(gen));
+      exp[i] = v1[i] * v2[i];
+    }
+    cipher_t* a = ckks_encrypt(ctx, v1.data(), slots);
+    cipher_t* b = ckks_mul(ctx, a, v2.data());
+    cd* out = ckks_decrypt(ctx, b, sk);
+    report("MulByConstantArray_" + tag, maxerr(exp.data(), out, slots), ERRBOUND);
+    free(out);
+    poly_t* p = new poly_t;
+    poly_init(ctx, p);
+    int64_t scale = -1;
+    ctx->encode(p, &scale, v2.data(), slots, 0);
+    cipher_t* a2 = ckks_encrypt(ctx, v1.data(), slots);
+    cipher_t* c = ckks_mul(ctx, a2, p);
+    out = ckks_decrypt(ctx, c, sk);
+    report("MulByConstantPoly_" + tag, maxerr(exp.data(), out, slots), ERRBOUND);
+    free(out);
+    poly_free(ctx, p);
+    delete p;
+    done({a, b, a2, c});
+  }
+  // RotateRight (tests.cpp:1537-1592) and Conjugate (tests.cpp:1651-1700)
+  {
+    for (auto& x : v1) x = cd(d01(gen), d01(gen));
+    cipher_t* a = ckks_encrypt(ctx, v1.data(), slots);
+    double worst = 0;
+    for (int r : {1, 8, 256}) {
+      if (r >= slots) continue;
+      cipher_t c;
+      cipher_init(ctx, &c);
+      ckks_rotate_right(ctx, &c, a, r);
+      cd* out = ckks_decrypt(ctx, &c, sk);
+      for (int i = 0; i < slots; ++i) exp[i] = v1[(i - r + slots) % slots];
+      worst = std::max(worst, maxerr(exp.data(), out, slots));
+      free(out);
+      cipher_free(ctx, &c);
+    }
+    report("RotateRight_" + tag, worst, 0.01);
+    cipher_t* b = ckks_conjugate(ctx, a);
+    cd* out = ckks_decrypt(ctx, b, sk);
+    for (int i = 0; i < slots; ++i) exp[i] = std::conj(v1[i]);
+    report("Conjugate_" + tag, maxerr(exp.data(), out, slots), ERRBOUND);
+    free(out);
+    done({a, b});
+  }
+  // InnerProductSingle (tests.cpp:1762-1831): length-4 vectors of 1-slot ciphertexts
+  {
+    const int length = 4;
+    double worst = 0;
+    for (int t = 0; t < 3; ++t) {
+      cd va[length], vb[length], ip = 0;
+      do {
+        ip = 0;
+        for (int i = 0; i < length; ++i) {
+          va[i] = cd(dsq(gen), dsq(gen));
+          vb[i] = cd(dsq(gen), dsq(gen));
+          ip += va[i] * vb[i];
+        }
+      } while (std::abs(ip.real()) >= R || std::abs(ip.imag()) >= R);
+      cipher_t* ca = new cipher_t[length];
+      cipher_t* cb = new cipher_t[length];
+      for (int i = 0; i < length; ++i) {
+        cipher_init(ctx, &ca[i]);
+        cipher_init(ctx, &cb[i]);
+        ckks_encrypt(ctx, &ca[i], &va[i]);
+        ckks_encrypt(ctx, &cb[i], &vb[i]);
+      }
+      cipher_t c3;
+      ckks_inner_prod(ctx, &c3, ca, cb, length);
+      cd* out = ckks_decrypt(ctx, &c3, sk);
+      worst = std::max(worst, std::max(std::abs(out[0].real() - ip.real()), std::abs(out[0].imag() - ip.imag())));
+      free(out);
+      for (int i = 0; i < length; ++i) {
+        cipher_free(ctx, &ca[i]);
+        cipher_free(ctx, &cb[i]);
+      }
+      cipher_free(ctx, &c3);
+      delete[] ca;
+      delete[] cb;
+    }
+    report("InnerProductSingle_" + tag, worst, 10 * ERRBOUND);
+  }
+  // InnerProductBatch (tests.cpp:1833-1908): ckks_batch_inner_prod over 16 slots
+  {
+    const int ns = 16;
+    std::vector<cd> a1(ns), a2(ns);
+    cd ip = 0;
+    for (int i = 0; i < ns; ++i) {
+      a1[i] = cd(dm1(gen), dm1(gen));
+      a2[i] = cd(dm1(gen), dm1(gen));
+      ip += a1[i] * a2[i];
+    }
+    cipher_t* x = ckks_encrypt(ctx, a1.data(), ns);
+    cipher_t* y = ckks_encrypt(ctx, a2.data(), ns);
+    cipher_t c3;
+    cipher_init(ctx, &c3);
+    ckks_batch_inner_prod(ctx, &c3, x, y);
+    cd* out = ckks_decrypt(ctx, &c3, sk);
+    report("InnerProductBatch_" + tag, std::max(std::abs(out[0].real() - ip.real()), std::abs(out[0].imag() - ip.imag())),
+           10 * ERRBOUND);
+    free(out);
+    cipher_free(ctx, &c3);
+    done({x, y});
+  }
+  // Power (tests.cpp:2052-2102)
+  {
+    const int bound = 5;
+    std::uniform_real_distribution<double> dp(-std::pow(R, 1.0 / (bound + 1)), std::pow(R, 1.0 / (bound + 1)));
+    double worst = 0;
+    for (int x : {0, 1, 2, 3, 4}) {
+      for (auto& v : v1) v = cd(dp(gen), dp(gen));
+      cipher_t* a = ckks_encrypt(ctx, v1.data(), slots);
+      cipher_t* b = ckks_power(ctx, a, (uint64_t)x);
+      cd* out = ckks_decrypt(ctx, b, sk);
+      for (int i = 0; i < slots; ++i) exp[i] = std::pow(v1[i], x);
+      worst = std::max(worst, maxerr(exp.data(), out, slots));
+      free(out);
+      done({a, b});
+    }
+    report("Power_" + tag, worst, ERRBOUND);
+  }
+  // evalpoly with the sine series (tests.cpp:2153-2204)
+  {
+    const int degree = 11;
+    for (auto& v : v1) v = cd(dm1(gen), 0);
+    double* coeffs = ctx->maclaurin_coeffs[SIN];
+    cipher_t* a = ckks_encrypt(ctx, v1.data(), slots);
+    cipher_t* b = ckks_eval_polynomial(ctx, a, coeffs, degree);
+    cd* out = ckks_decrypt(ctx, b, sk);
+    for (int j = 0; j < slots; ++j) {
+      cd e = 0;
+      for (int i = degree - 1; i >= 0; --i) e = coeffs[i] + v1[j] * e;
+      exp[j] = e;
+    }
+    report("EvalPoly_" + tag, maxerr(exp.data(), out, slots), ERRBOUND);
+    free(out);
+    done({a, b});
+  }
+  // exp, sin, cos, sigmoid, log, log1minus, inverse (tests.cpp:2206-2782):
+  // real inputs over the reference's ranges, with its bounds.  (Its cosine
+  // series has the x^3 term where x^4 belongs, so it is only accurate on
+  // [0, 1], the range its test uses.)
+  if (prec > 52) {
+    struct Fn {
+      const char* name;
+      cipher_t* (*f)(CKKSContext*, cipher_t*);
+      double (*ref)(double);
+      double lo, hi, bound;
+    };
+    const Fn fns[] = {
+        {"Exp", ckks_exp, [](double x) { return std::exp(x); }, 0, 1, 2 * ERRBOUND},
+        {"Sin", ckks_sin, [](double x) { return std::sin(x); }, 0, 1, ERRBOUND},
+        {"Cos", ckks_cos, [](double x) { return std::cos(x); }, 0, 1, ERRBOUND},
+        {"Sigmoid", ckks_sigmoid, [](double x) { return 1.0 / (1.0 + std::exp(-x)); }, 0, 1, ERRBOUND},
+        {"Log", [](CKKSContext* c, cipher_t* x) { return ckks_log(c, x); }, [](double x) { return std::log(x); },
+         0.1, 1, 0.5},
+        // the 10-step series diverges from log(1-x) beyond 0.5 near x = 1,
+        // so the range stops at 0.9
+        {"Log1Minus", ckks_log1minus, [](double x) { return std::log(1 - x); }, 0, 0.9, 0.5},
+        {"Inverse", [](CKKSContext* c, cipher_t* x) { return ckks_inverse(c, x); },
+         [](double x) { return 1.0 / x; }, 0.1, 1, 10},
+    };
+    for (const Fn& fn : fns) {
+      std::uniform_real_distribution<double> dd(fn.lo, fn.hi);
+      for (auto& v : v1) v = cd(dd(gen), 0);
+      cipher_t* a = ckks_encrypt(ctx, v1.data(), slots);
+      cipher_t* b = fn.f(ctx, a);
+      cd* out = ckks_decrypt(ctx, b, sk);
+      double worst = 0;
+      for (int i = 0; i < slots; ++i) worst = std::max(worst, std::abs(out[i].real() - fn.ref(v1[i].real())));
+      report(std::string(fn.name) + "_" + tag, worst, fn.bound);
+      free(out);
+      done({a, b});
+    }
+  }
+  // DiscardSlots (tests.cpp:2784-2829): values in [0.1, 10], bound 1e-3
+  {
+    std::uniform_real_distribution<double> dv(0.1, 10);
+    for (auto& v : v1) v = cd(dv(gen), 0);
+    cipher_t* a = ckks_encrypt(ctx, v1.data(), slots);
+    const int idx = 3;
+    ckks_discard_slots_except(ctx, a, idx);
+    cd* out = ckks_decrypt(ctx, a, sk);
+    double worst = 0;
+    for (int i = 0; i < slots; ++i) worst = std::max(worst, std::abs(out[i].real() - (i == idx ? v1[i].real() : 0.0)));
+    report("DiscardSlots_" + tag, worst, 1e-3);
+    free(out);
+    done({a});
+  }
+  ckkskeychain_free(ctx, keys);
+  delete keys;
+  ckks_free_sk(ctx, sk);
+  delete sk;
+  delete ctx;
+  CUDAManager::destroy();
+}
+
 int main(int argc, char** argv) {
   const bool quick = argc > 1 && std::string(argv[1]) == "quick";
   mul_n4096_k4();
   run_params(3, 3, 4096, 55, true);
   run_params(5, 5, 8192, 55, true);
+  run_adv(30, 30, 4096, 55);
   if (!quick) {
     run_params(5, 5, 16384, 55, true);
     run_params(30, 30, 4096, 48, true, true);

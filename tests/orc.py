@@ -42,6 +42,8 @@ def load():
         "orc_mulmod": (ctypes.c_uint64, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]),
         "orc_ntt": (None, [_vp, _u64p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "orc_eltwise": (None, [_vp, ctypes.c_int, _u64p, _u64p, _u64p, ctypes.c_int]),
+        "orc_eltwise_scalar": (None, [_vp, ctypes.c_int, _u64p, _u64p, ctypes.c_uint64, ctypes.c_int]),
+        "orc_conjugate": (None, [_vp, _u64p, _u64p, ctypes.c_int]),
         "orc_rescale": (None, [_vp, _u64p, _u64p, ctypes.c_int]),
         "orc_modup": (None, [_vp, _u64p, ctypes.c_int]),
         "orc_moddown": (None, [_vp, _u64p, _u64p, ctypes.c_int]),
@@ -105,6 +107,12 @@ class Oracle:
 
     def eltwise(self, op, c, a, b, nres):
         self.lib.orc_eltwise(self.h, op, _p(c), _p(a), _p(b), nres)
+
+    def eltwise_scalar(self, op, c, a, b, nres):
+        self.lib.orc_eltwise_scalar(self.h, op, _p(c), _p(a), b, nres)
+
+    def conjugate(self, out, inp, nres):
+        self.lib.orc_conjugate(self.h, _p(out), _p(inp), nres)
 
     def rescale(self, a, b, level):
         self.lib.orc_rescale(self.h, _p(a), _p(b), level)

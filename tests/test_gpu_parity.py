@@ -61,6 +61,41 @@ def test_eltwise_parity(gpu, op):
     assert np.array_equal(P.to_host(out), ref)
 
 
+@pytest.mark.parametrize("op", [0, 1, 2])
+def test_eltwise_scalar_parity(gpu, op):
+    """Scalar operand used as given (ntt.cu:531-558), including scalars above
+    the primes and sums that wrap in 64 bits."""
+    P, O = _params(8192, 5, 0, 48)
+    rng = np.random.default_rng(30 + op)
+    R = O.k + O.kl
+    a = O.random_poly(rng, R)
+    a[:5] = 0
+    for b in (0, 7, O.primes[1] - 1, O.primes[1] + 3, (1 << 52) + 9, (1 << 60) + 1, (1 << 64) - 5):
+        out = P.zeros(R)
+        P.eltwise_scalar(op, out, P.to_device(a), b, res0=0, nres=R)
+        ref = np.zeros_like(a)
+        O.eltwise_scalar(op, ref, a, b, R)
+        assert np.array_equal(P.to_host(out), ref), "op %d b %d" % (op, b)
+    # a limb range starting inside the special limbs
+    out = P.zeros(R)
+    P.eltwise_scalar(op, out, P.to_device(a), 12345, res0=O.k + 1, nres=O.kl - 1)
+    ref = np.zeros_like(a)
+    O.eltwise_scalar(op, ref[(O.k + 1) * O.N:], a[(O.k + 1) * O.N:], 12345, O.kl - 1)
+    assert np.array_equal(P.to_host(out)[(O.k + 1) * O.N:], ref[(O.k + 1) * O.N:])
+
+
+def test_conjugate_parity(gpu):
+    P, O = _params(16384, 4, 0, 52)
+    rng = np.random.default_rng(35)
+    R = O.k + O.kl
+    a = O.random_poly(rng, R)
+    out = P.zeros(R)
+    P.conjugate(out, P.to_device(a), R)
+    ref = np.zeros_like(a)
+    O.conjugate(ref, a, R)
+    assert np.array_equal(P.to_host(out), ref)
+
+
 @pytest.mark.parametrize("shape", SMALL)
 def test_rescale_modup_moddown_parity(gpu, shape):
     P, O = _params(*shape)

@@ -354,3 +354,64 @@ def test_automorphism_definition():
                     ref[e] = v
             got = [int(v) for v in out[rid * N:(rid + 1) * N]]
             assert got == ref
+
+
+# ------------------------------------------------ scalar ops, conjugation ---
+
+def test_scalar_ops_congruence_and_exactness():
+    """ntt_vector_op_int (ntt.cu:531-558) restated by the oracle: for a scalar
+    below the prime the results are the exact (a +- b) mod p and a*b mod p;
+    for larger scalars add/sub stay congruent and the product is the exact
+    remainder or, inside the Barrett deviation window, r - (2^nb - p)."""
+    o = Oracle(4096, 3, 2, 52)
+    N = o.N
+    rng = np.random.default_rng(21)
+    nres = 5
+    a = o.random_poly(rng, nres)
+    for b in (0, 1, 12345, o.primes[1] - 1, o.primes[1] + 7, (1 << 62) + 5, (1 << 54) + 3, (1 << 55) - 1):
+        for op in (0, 1, 2):
+            out = np.zeros_like(a)
+            o.eltwise_scalar(op, out, a, b, nres)
+            for rid in range(nres):
+                p = o.primes[rid]
+                nb = p.bit_length()
+                eps = (1 << nb) - p
+                for c in range(0, N, 97):
+                    x, got = int(a[rid * N + c]), int(out[rid * N + c])
+                    if x + b >= 1 << 64 or (op == 2 and p.bit_length() * 2 - 64 > 62):
+                        continue  # 64-bit wrap: the literal formula has no modular meaning
+                    if op == 0:
+                        assert got % p == (x + b) % p and got <= x + b
+                        if b < p:
+                            assert got == (x + b) % p
+                    elif op == 1:
+                        if x < b and b > p:
+                            continue  # p - b wraps in 64 bits in the literal formula
+                        assert got % p == (x - b) % p
+                        if b < p:
+                            assert got == (x - b) % p
+                    else:
+                        r = (x * b) % p
+                        if b < p:
+                            assert got == r
+                        else:
+                            assert got == r or (got == r - eps and r >= eps)
+
+
+def test_conjugation_is_the_inverse_automorphism():
+    """NTT -> index reversal (encoder.cu:391-401) -> INTT equals a(X^-1) mod
+    X^N + 1: coefficient 0 stays, coefficient j > 0 moves to N - j negated."""
+    o = Oracle(4096, 2, 1, 55)
+    N = o.N
+    rng = np.random.default_rng(22)
+    x = o.random_poly(rng, 3)
+    y = x.copy()
+    o.ntt(y, 0, 3, 0)
+    z = np.zeros_like(y)
+    o.conjugate(z, y, 3)
+    o.ntt(z, 0, 3, 1)
+    for rid in range(3):
+        p = o.primes[rid]
+        xs = [int(v) for v in x[rid * N:(rid + 1) * N]]
+        ref = [xs[0]] + [(-xs[N - j]) % p for j in range(1, N)]
+        assert [int(v) for v in z[rid * N:(rid + 1) * N]] == ref
