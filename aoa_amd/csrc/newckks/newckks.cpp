@@ -462,7 +462,8 @@ CKKSContext::CKKSContext() : Context() {
   poly_init(this, e);
   for (auto& c : caux) cipher_init(this, &c);
   h_val = new std::complex<double>[CUDAManager::N];
-  // Maclaurin series (ckkscontext.h:101-135)
+  // Maclaurin series (ckkscontext.h:101-135), the reference's values as they
+  // are (its cosine row has 1/24 at x^3, which matches cos only on [0, 1])
   maclaurin_coeffs[EXPONENT] = new double[11]{1, 1, 1. / 2, 1. / 6, 1. / 24, 1. / 120, 1. / 720, 1. / 5040,
                                               1. / 40320, 1. / 362880, 1. / 3628800};
   maclaurin_coeffs[SIN] = new double[12]{0, 1, 0, -1. / 6, 0, 1. / 120, 0, -1. / 5040, 0, 1. / 362880, 0,

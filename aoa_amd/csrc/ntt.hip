@@ -469,9 +469,212 @@ __global__ __launch_bounds__(256) void k_ntt_inv_col_d(DevParams P, const NttDes
     inv_col_body<S, false>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
 }
 
+// ------------------------------------------- single-workgroup transform --
+// N <= 4096: one workgroup holds a whole limb (V = N/256 values per thread,
+// 2^R = V) and runs every stage, exchanging through LDS between rounds of R
+// stages, so each limb is read and written once.  In a round whose Rr active
+// bits sit at positions [lo, lo + Rr) of the element index, value j of thread
+// t is element sp_index(t, j): the active bits come from j, the rest from t
+// (and, when Rr < R, from the high bits of j).  Loads and stores go through
+// LDS in the order i = t + 256 m, so every global access is coalesced.
+
+template <int R>
+__device__ __forceinline__ int sp_index(int t, int j, int lo, int Rr) {
+  const int jj = j & ((1 << Rr) - 1), g = j >> Rr;
+  const int o = (t << (R - Rr)) | g;
+  const int low = o & ((1 << lo) - 1), high = o >> lo;
+  return (high << (lo + Rr)) | (jj << lo) | low;
+}
+
+__device__ __forceinline__ int spz(int i) { return i + (i >> 5); }
+
+template <int LOGN>
+struct SGeo {
+  static constexpr int N = 1 << LOGN, V = N / 256, R = (V == 16) ? 4 : 3, LDS = N + N / 32;
+};
+
+// Coalesced load of a limb (optionally times a key limb) into LDS.
+template <int LOGN, bool LZ>
+__device__ __forceinline__ void sp_load(const uint64_t* src, const uint64_t* key, const DevPrime& dp,
+                                        uint64_t* lds) {
+  constexpr int V = SGeo<LOGN>::V;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int i = t + 256 * m;
+    uint64_t v = src[i];
+    if (key) v = key_mul<LZ>(v, key[i], dp);
+    lds[spz(i)] = v;
+  }
+  __syncthreads();
+}
+
+template <int LOGN>
+__device__ __forceinline__ void sp_store(uint64_t* dst, const uint64_t* lds) {
+  constexpr int V = SGeo<LOGN>::V;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < V; ++m) dst[t + 256 * m] = lds[spz(t + 256 * m)];
+}
+
+template <int LOGN, bool LZ>
+__device__ __forceinline__ void fwd_single_body(const DevParams& P, const uint64_t* src, uint64_t* dst, int pr,
+                                                uint64_t* lds) {
+  using G = SGeo<LOGN>;
+  constexpr int V = G::V, R = G::R;
+  const DevPrime dp = P.primes[pr];
+  const uint64_t p = dp.p;
+  const uint64_t* T = P.tw_fwd + (size_t)pr * G::N * 2;
+  const int t = threadIdx.x;
+  sp_load<LOGN, LZ>(src, nullptr, dp, lds);
+  uint64_t x[V];
+#pragma unroll
+  for (int s0 = 0; s0 < LOGN; s0 += R) {
+    const int Rr = (LOGN - s0) < R ? (LOGN - s0) : R;
+    const int lo = LOGN - s0 - Rr;
+#pragma unroll
+    for (int j = 0; j < V; ++j) x[j] = lds[spz(sp_index<R>(t, j, lo, Rr))];
+#pragma unroll
+    for (int r = 0; r < Rr; ++r) {
+      const int s = s0 + r, half = 1 << (Rr - 1 - r);
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        if (j & half) continue;
+        const int i = sp_index<R>(t, j, lo, Rr);
+        const tw2 w = load_tw(T, ((size_t)1 << s) + (size_t)(i >> (LOGN - s)));
+        ct_bfly<LZ>(x[j], x[j + half], w, p);
+      }
+    }
+    if (LZ && s0 + Rr >= LOGN) {
+#pragma unroll
+      for (int j = 0; j < V; ++j) x[j] = canon4(x[j], p);
+    }
+    __syncthreads();  // every thread has read the previous layout
+#pragma unroll
+    for (int j = 0; j < V; ++j) lds[spz(sp_index<R>(t, j, lo, Rr))] = x[j];
+    __syncthreads();
+  }
+  sp_store<LOGN>(dst, lds);
+}
+
+// Inverse: stages LOGN-1 .. 0 (Gentleman-Sande), N^-1 folded into stage 0;
+// with a key the input limb is first multiplied by the key limb.
+template <int LOGN, bool LZ>
+__device__ __forceinline__ void inv_single_body(const DevParams& P, const uint64_t* src, uint64_t* dst,
+                                                const uint64_t* key, int pr, uint64_t* lds) {
+  using G = SGeo<LOGN>;
+  constexpr int V = G::V, R = G::R;
+  const DevPrime dp = P.primes[pr];
+  const uint64_t p = dp.p;
+  const uint64_t* T = P.tw_inv + (size_t)pr * G::N * 2;
+  const int t = threadIdx.x;
+  sp_load<LOGN, LZ>(src, key, dp, lds);
+  uint64_t x[V];
+#pragma unroll
+  for (int done = 0; done < LOGN; done += R) {
+    const int shi = LOGN - 1 - done;  // highest stage of this round
+    const int Rr = (LOGN - done) < R ? (LOGN - done) : R;
+    const int lo = LOGN - 1 - shi;
+#pragma unroll
+    for (int j = 0; j < V; ++j) x[j] = lds[spz(sp_index<R>(t, j, lo, Rr))];
+#pragma unroll
+    for (int r = 0; r < Rr; ++r) {
+      const int s = shi - r, half = 1 << r;
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        if (j & half) continue;
+        if (s == 0) {
+          const uint64_t u = x[j], v = x[j + half];
+          x[j] = mul_shoup(u + v, dp.ninv, dp.ninv_q, p);
+          x[j + half] = mul_shoup(LZ ? u + 2 * p - v : u + (p - v), dp.w1ninv, dp.w1ninv_q, p);
+        } else {
+          const int i = sp_index<R>(t, j, lo, Rr);
+          const tw2 w = load_tw(T, ((size_t)1 << s) + (size_t)(i >> (LOGN - s)));
+          gs_bfly<LZ>(x[j], x[j + half], w, p);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < V; ++j) lds[spz(sp_index<R>(t, j, lo, Rr))] = x[j];
+    __syncthreads();
+  }
+  sp_store<LOGN>(dst, lds);
+}
+
+template <int LOGN>
+__global__ __launch_bounds__(256) void k_ntt_fwd_single(DevParams P, LimbMap m) {
+  __shared__ uint64_t lds[SGeo<LOGN>::LDS];
+  const int li = blockIdx.x, y = blockIdx.y, pr = limb_prime(m, P.k, li);
+  const size_t off = (size_t)li * P.N;
+  if (lazy_prime(P, pr))
+    fwd_single_body<LOGN, true>(P, m.src[y] + off, m.base[y] + off, pr, lds);
+  else
+    fwd_single_body<LOGN, false>(P, m.src[y] + off, m.base[y] + off, pr, lds);
+}
+
+template <int LOGN>
+__global__ __launch_bounds__(256) void k_ntt_inv_single(DevParams P, LimbMap m, const uint64_t* key0,
+                                                         const uint64_t* key1) {
+  __shared__ uint64_t lds[SGeo<LOGN>::LDS];
+  const int li = blockIdx.x, y = blockIdx.y, pr = limb_prime(m, P.k, li);
+  const size_t off = (size_t)li * P.N;
+  const uint64_t* key = key0 ? (y == 0 ? key0 : key1) + (size_t)pr * P.N : nullptr;
+  const uint64_t* src = (key0 ? m.src[0] : m.src[y]) + off;
+  if (lazy_prime(P, pr))
+    inv_single_body<LOGN, true>(P, src, m.base[y] + off, key, pr, lds);
+  else
+    inv_single_body<LOGN, false>(P, src, m.base[y] + off, key, pr, lds);
+}
+
+template <int LOGN>
+__global__ __launch_bounds__(256) void k_ntt_fwd_single_d(DevParams P, const NttDesc* d) {
+  __shared__ uint64_t lds[SGeo<LOGN>::LDS];
+  const NttDesc& e = d[blockIdx.x];
+  const uint64_t* src = e.src;
+  uint64_t* dst = e.dst;
+  const int pr = e.prime;
+  if (lazy_prime(P, pr))
+    fwd_single_body<LOGN, true>(P, src, dst, pr, lds);
+  else
+    fwd_single_body<LOGN, false>(P, src, dst, pr, lds);
+}
+
+template <int LOGN>
+__global__ __launch_bounds__(256) void k_ntt_inv_single_d(DevParams P, const NttDesc* d) {
+  __shared__ uint64_t lds[SGeo<LOGN>::LDS];
+  const NttDesc& e = d[blockIdx.x];
+  const uint64_t* src = e.src;
+  uint64_t* dst = e.dst;
+  const uint64_t* key = e.key;
+  const int pr = e.prime;
+  if (lazy_prime(P, pr))
+    inv_single_body<LOGN, true>(P, src, dst, key, pr, lds);
+  else
+    inv_single_body<LOGN, false>(P, src, dst, key, pr, lds);
+}
+
+// Both key products of one input limb, one workgroup per (limb, key half).
+template <int LOGN>
+__global__ __launch_bounds__(256) void k_ntt_inv_single_key2_d(DevParams P, const KeyDesc2* d) {
+  __shared__ uint64_t lds[SGeo<LOGN>::LDS];
+  const KeyDesc2& e = d[blockIdx.x >> 1];
+  const int y = blockIdx.x & 1;
+  const uint64_t* src = e.src;
+  uint64_t* dst = y ? e.dst[1] : e.dst[0];
+  const uint64_t* key = y ? e.key[1] : e.key[0];
+  const int pr = e.prime;
+  if (lazy_prime(P, pr))
+    inv_single_body<LOGN, true>(P, src, dst, key, pr, lds);
+  else
+    inv_single_body<LOGN, false>(P, src, dst, key, pr, lds);
+}
+
 // ---------------------------------------------------------------- launch ---
 
+// N1 == N selects the single-workgroup transform (N <= 4096).
 bool ntt_valid_split(int N, int N1) {
+  if (N1 == N) return N == 2048 || N == 4096;
   if (N1 != 64 && N1 != 128 && N1 != 256) return false;
   if (N % N1) return false;
   int N2 = N / N1;
@@ -480,7 +683,8 @@ bool ntt_valid_split(int N, int N1) {
 
 int ntt_default_n1(int N) {
   switch (N) {
-    case 4096: return 64;
+    case 2048: return 2048;
+    case 4096: return 4096;
     case 8192: return 128;
     case 16384: return 128;
     case 32768: return 128;
@@ -508,12 +712,26 @@ static int pick_n1(const DevParams& P, int N1) {
   return ntt_valid_split(P.N, N1) ? N1 : 0;
 }
 
+#define AOA_DISPATCH_LOGN(N_, CALL) \
+  switch (N_) {                      \
+    case 2048: CALL(11); break;      \
+    case 4096: CALL(12); break;      \
+    default: return -1;              \
+  }
+
 int ntt_forward(const DevParams& P, LimbMap m, int N1, hipStream_t st) {
   N1 = pick_n1(P, N1);
   if (!N1) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
   for (int y = 0; y < m.npolys; ++y)
     if (!m.src[y]) m.src[y] = m.base[y];
+  if (N1 == P.N) {
+    dim3 grid(m.nlimbs, m.npolys);
+#define CALL1(L) k_ntt_fwd_single<L><<<grid, 256, 0, st>>>(P, m)
+    AOA_DISPATCH_LOGN(P.N, CALL1)
+#undef CALL1
+    return 0;
+  }
   {
     const int tiles = N2 / (4096 / N1);
     dim3 grid(tiles * m.nlimbs, m.npolys);
@@ -540,6 +758,13 @@ int ntt_inverse(const DevParams& P, LimbMap m, int N1, const uint64_t* key0, con
   const int N2 = P.N / N1, g0 = log2i(N1);
   for (int y = 0; y < m.npolys; ++y)
     if (!m.src[y]) m.src[y] = m.base[y];
+  if (N1 == P.N) {
+    dim3 grid(m.nlimbs, m.npolys);
+#define CALL1(L) k_ntt_inv_single<L><<<grid, 256, 0, st>>>(P, m, key0, key1)
+    AOA_DISPATCH_LOGN(P.N, CALL1)
+#undef CALL1
+    return 0;
+  }
   {
     const int tiles = N1 / (4096 / N2);
     dim3 grid(tiles * m.nlimbs, m.npolys);
@@ -563,6 +788,12 @@ int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
   N1 = pick_n1(P, N1);
   if (!N1 || ndesc <= 0) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
+  if (N1 == P.N) {
+#define CALL1(L) k_ntt_fwd_single_d<L><<<dim3(ndesc), 256, 0, st>>>(P, d)
+    AOA_DISPATCH_LOGN(P.N, CALL1)
+#undef CALL1
+    return 0;
+  }
   {
     const int tiles = N2 / (4096 / N1);
 #define CALLC(S) k_ntt_fwd_col_d<S><<<dim3(tiles * ndesc), 256, 0, st>>>(P, d, N2)
@@ -582,6 +813,12 @@ int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
   N1 = pick_n1(P, N1);
   if (!N1 || ndesc <= 0) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
+  if (N1 == P.N) {
+#define CALL1(L) k_ntt_inv_single_d<L><<<dim3(ndesc), 256, 0, st>>>(P, d)
+    AOA_DISPATCH_LOGN(P.N, CALL1)
+#undef CALL1
+    return 0;
+  }
   {
     const int tiles = N1 / (4096 / N2);
 #define CALLR(S) k_ntt_inv_row_d<S><<<dim3(tiles * ndesc), 256, 0, st>>>(P, d, g0)
@@ -602,6 +839,12 @@ int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const 
   N1 = pick_n1(P, N1);
   if (!N1 || n2 <= 0 || ncol <= 0) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
+  if (N1 == P.N) {
+#define CALL1(L) k_ntt_inv_single_key2_d<L><<<dim3(2 * n2), 256, 0, st>>>(P, d2)
+    AOA_DISPATCH_LOGN(P.N, CALL1)
+#undef CALL1
+    return 0;
+  }
   {
     const int tiles = N1 / (4096 / N2);
 #define CALLR(S) k_ntt_inv_row_key2_d<S><<<dim3(tiles * n2), 256, 0, st>>>(P, d2, g0)

@@ -76,7 +76,7 @@ static std::string select_primes(int k, int kl, int prec, std::vector<aoa_prime_
 }
 
 std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
-  if (N < 4096 || N > 65536 || (N & (N - 1)) != 0) return "NTT is not working for this N";
+  if (N < 2048 || N > 65536 || (N & (N - 1)) != 0) return "NTT is not working for this N";
   if (kl <= 0) kl = k + 1;
   std::vector<aoa_prime_entry> qe, be;
   std::string err = select_primes(k, kl, prec, &qe, &be);

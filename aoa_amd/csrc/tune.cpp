@@ -59,7 +59,7 @@ extern "C" int aoa_autotune_ntt(aoa_params* h, void* stream, int nlimbs, int rep
   (void)hipEventCreate(&e1);
   int best = 0, count = 0;
   double best_us = 1e30;
-  for (int n1 : {64, 128, 256}) {
+  for (int n1 : {64, 128, 256, h->host.N}) {
     if (!ntt_valid_split(h->host.N, n1)) continue;
     DevParams P = h->dev;
     P.ntt_n1 = n1;

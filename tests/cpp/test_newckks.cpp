@@ -549,9 +549,11 @@ static void run_adv(int k, int kl, int N, int prec) {
 int main(int argc, char** argv) {
   const bool quick = argc > 1 && std::string(argv[1]) == "quick";
   mul_n4096_k4();
+  run_params(2, 2, 2048, 55, true);
   run_params(3, 3, 4096, 55, true);
   run_params(5, 5, 8192, 55, true);
   run_adv(30, 30, 4096, 55);
+  run_adv(20, 20, 2048, 55);
   if (!quick) {
     run_params(5, 5, 16384, 55, true);
     run_params(30, 30, 4096, 48, true, true);

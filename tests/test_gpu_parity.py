@@ -12,8 +12,10 @@ from orc import Oracle
 
 pytestmark = pytest.mark.gpu
 
-# (N, k, kl, prec): N=4096 k=4 prec 52, small/medium shapes, and the reference's test params.
-SMALL = [(4096, 4, 5, 52), (8192, 6, 0, 48), (16384, 5, 5, 55), (32768, 4, 0, 48), (4096, 3, 3, 45)]
+# (N, k, kl, prec): N=4096 k=4 prec 52, small/medium shapes, and the reference's test params
+# (N=2048 and 4096 run the single-workgroup transform, larger N the two-pass one).
+SMALL = [(4096, 4, 5, 52), (8192, 6, 0, 48), (16384, 5, 5, 55), (32768, 4, 0, 48), (4096, 3, 3, 45),
+         (2048, 2, 2, 55)]
 
 
 def _params(N, k, kl, prec):
@@ -21,7 +23,25 @@ def _params(N, k, kl, prec):
     return Params(N, k, kl, prec), Oracle(N, k, kl, prec)
 
 
-@pytest.mark.parametrize("shape", SMALL + [(65536, 3, 0, 48)])
+def test_ntt_single_and_two_pass_agree(gpu):
+    """At N=4096 the single-workgroup transform (split 4096) and the two-pass
+    one (split 64) give the oracle's residues."""
+    P, O = _params(4096, 4, 5, 52)
+    rng = np.random.default_rng(12)
+    R = O.k + O.kl
+    x = O.random_poly(rng, R)
+    ref = x.copy()
+    O.ntt(ref, 0, R, 0)
+    for n1 in (64, 4096):
+        P.set_ntt_split(n1)
+        d = P.to_device(x)
+        P.ntt(d, 0, R, 0)
+        assert np.array_equal(P.to_host(d), ref), "split %d" % n1
+        P.ntt(d, 0, R, 1)
+        assert np.array_equal(P.to_host(d), x), "split %d inverse" % n1
+
+
+@pytest.mark.parametrize("shape", SMALL + [(65536, 3, 0, 48), (2048, 20, 20, 55)])
 def test_ntt_parity(gpu, shape):
     P, O = _params(*shape)
     rng = np.random.default_rng(11)
@@ -80,7 +100,7 @@ def test_eltwise_scalar_parity(gpu, op):
     out = P.zeros(R)
     P.eltwise_scalar(op, out, P.to_device(a), 12345, res0=O.k + 1, nres=O.kl - 1)
     ref = np.zeros_like(a)
-    O.eltwise_scalar(op, ref[(O.k + 1) * O.N:], a[(O.k + 1) * O.N:], 12345, O.kl - 1)
+    O.eltwise_scalar(op, ref, a, 12345, R)  # the oracle numbers residues from 0: compare the tail
     assert np.array_equal(P.to_host(out)[(O.k + 1) * O.N:], ref[(O.k + 1) * O.N:])
 
 
