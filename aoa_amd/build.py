@@ -62,7 +62,7 @@ def build_newckks(verbose=False):
     C-ABI of libaoa.so; plus the C++ test program written against it."""
     inc = ["-I" + os.path.join(ROOT, "include")]
     objs = []
-    for src in ("newckks.cpp", "newckks_ops.cpp"):
+    for src in ("newckks.cpp", "newckks_ops.cpp", "newckks_io.cpp"):
         obj = os.path.join(OBJ, src.replace(".cpp", ".o"))
         _run([HIPCC, "-O2", "-fPIC", "-std=c++17", "-Wall"] + inc +
              ["-c", os.path.join(CSRC, "newckks", src), "-o", obj])

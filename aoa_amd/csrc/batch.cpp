@@ -236,6 +236,19 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
           md.push_back({y ? P1 : P0, (y ? P1 : P0) + (size_t)Lq * N, add, it.out[y], lv, 0});
         }
       }
+      // Prime-major order: the limbs of one prime from every item run next
+      // to each other, so their workgroups (spread over the XCDs by block
+      // index) find that prime's twiddles, and in the key stage its key
+      // limbs, in the L2 cache.  Entries of a stage are independent.
+      auto by_prime = [](auto& v) {
+        std::stable_sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.prime < b.prime; });
+      };
+      by_prime(prefwd);
+      by_prime(preinv);
+      by_prime(intt3);
+      by_prime(fwd);
+      by_prime(key2);
+      by_prime(keycol);
       g.n_pre_fwd = (int)prefwd.size();
       g.n_pre_inv = (int)preinv.size();
       g.n_tensor = (int)tens.size();

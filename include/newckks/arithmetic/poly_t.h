@@ -36,5 +36,9 @@ void poly_moddown(Context* ctx, poly_t* a, poly_t* b, int level);
 void poly_rescale(Context* ctx, poly_t* a, poly_t* b, int level);
 
 std::string poly_residue_to_string(Context* ctx, poly_t* a, int id = -1);
+// Text form of every residue of the base, coefficient domain, each followed by
+// a comma (poly_t.cpp:445-473).  poly_import returns a new Q-base polynomial.
+std::string poly_export(Context* ctx, poly_t* p);
+poly_t* poly_import(Context* ctx, std::string s);
 
 #endif

@@ -2,6 +2,11 @@
 #ifndef NEWCKKS_CIPHER_T_H
 #define NEWCKKS_CIPHER_T_H
 
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
 #include "../arithmetic/poly_t.h"
 
 struct ciphertext {
@@ -16,5 +21,14 @@ void cipher_init(Context* ctx, cipher_t* c, int slots = 1);
 void cipher_clear(Context* ctx, cipher_t* ct);
 void cipher_free(Context* ctx, cipher_t* c);
 void cipher_copy(Context* ctx, cipher_t* b, cipher_t* a);
+// The reference's JSON layout (cipher_t.cpp:42-82) as a string:
+// {"ct": {"0": "<poly_export>", "1": "..."}, "props": {"level", "scale", "slots"}}.
+std::string cipher_export_json(Context* ctx, cipher_t* ct);
+cipher_t* cipher_import_json(Context* ctx, const std::string& doc);
+// Binary form (no counterpart in the reference): a 40-byte header (magic,
+// N, k, level, slots, scale, domain of each polynomial) and the k limbs of
+// both polynomials as little-endian uint64, in their current domain.
+std::vector<uint8_t> cipher_export_binary(Context* ctx, cipher_t* ct);
+cipher_t* cipher_import_binary(Context* ctx, const uint8_t* data, size_t size);
 
 #endif

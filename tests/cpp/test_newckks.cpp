@@ -186,6 +186,29 @@ static void run_params(int k, int kl, int N, int prec, bool heavy, bool exact = 
       delete x;
     }
   }
+  // Serialisation: the reference's text and JSON layouts and the binary form
+  // round-trip bit for bit (poly_t.cpp:445-473, cipher_t.cpp:42-82)
+  {
+    for (auto& x : v1) x = cd(dist(gen), dist(gen));
+    cipher_t* a = ckks_encrypt(ctx, v1.data(), slots);
+    poly_t* p = poly_import(ctx, poly_export(ctx, &a->c[0]));
+    bool same = poly_are_equal(ctx, p, &a->c[0]);
+    poly_free(ctx, p);
+    delete p;
+    cipher_t* j = cipher_import_json(ctx, cipher_export_json(ctx, a));
+    same = same && poly_are_equal(ctx, &j->c[0], &a->c[0]) && poly_are_equal(ctx, &j->c[1], &a->c[1]) &&
+           j->level == a->level && j->scale == a->scale && j->slots == a->slots;
+    std::vector<uint8_t> bin = cipher_export_binary(ctx, a);
+    cipher_t* b = cipher_import_binary(ctx, bin.data(), bin.size());
+    same = same && poly_are_equal(ctx, &b->c[0], &a->c[0]) && poly_are_equal(ctx, &b->c[1], &a->c[1]);
+    cd* out = ckks_decrypt(ctx, b, sk);
+    report("Serialisation_" + tag, same ? maxerr(v1.data(), out, slots) : 1.0, ERRBOUND);
+    free(out);
+    for (cipher_t* x : {a, j, b}) {
+      cipher_free(ctx, x);
+      delete x;
+    }
+  }
   // level 0 cannot be rescaled (ckks.cpp:224-227)
   {
     cipher_t* a = ckks_encrypt(ctx, 1.0, slots);
