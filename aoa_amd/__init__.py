@@ -4,4 +4,5 @@ The Python layer is plumbing for tests and benchmarks: device buffers come
 from torch, every computation runs in libaoa.so.
 """
 from ._lib import AoaError, check, load  # noqa: F401
-from .core import ADD, FORWARD, INVERSE, MUL, MULADD, NEGATE, SUB, Batch, Params, Server  # noqa: F401
+from .core import (ADD, FORWARD, INVERSE, MUL, MULADD, NEGATE, SAMPLE_GAUSSIAN, SAMPLE_HW,  # noqa: F401
+                   SAMPLE_NARROW, SAMPLE_UNIFORM, SAMPLE_ZO, SUB, Batch, Params, Server)

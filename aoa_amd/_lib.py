@@ -27,6 +27,11 @@ SIGNATURES = {
     "aoa_eltwise_scalar": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_uint64, ctypes.c_int,
                                           ctypes.c_int]),
     "aoa_conjugate": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int]),
+    "aoa_special_fft": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int, ctypes.c_int]),
+    "aoa_encode": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int]),
+    "aoa_decode": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int, ctypes.c_uint64]),
+    "aoa_sample": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                  ctypes.c_uint64]),
     "aoa_rescale": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int]),
     "aoa_modup": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int]),
     "aoa_moddown": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int]),

@@ -11,6 +11,7 @@ from ._lib import MulItem, Op, RotItem, check, load
 
 FORWARD, INVERSE = 0, 1
 ADD, SUB, MUL, NEGATE, MULADD = 0, 1, 2, 3, 5
+SAMPLE_UNIFORM, SAMPLE_GAUSSIAN, SAMPLE_ZO, SAMPLE_HW, SAMPLE_NARROW = 0, 1, 2, 3, 4
 
 
 def _ptr(t):
@@ -91,6 +92,27 @@ class Params:
 
     def conjugate(self, out, inp, nres, stream=None):
         check(self.lib.aoa_conjugate(self.h, _stream(stream), _ptr(out), _ptr(inp), nres))
+
+    def special_fft(self, vals, n, direction, stream=None):
+        """In-place encoder FFT of n complex values (float64 device buffer of
+        2n values, re/im interleaved); INVERSE is the encoding direction."""
+        check(self.lib.aoa_special_fft(self.h, _stream(stream), _ptr(vals), n, direction))
+
+    def encode(self, out, vals, slots, scale, nres=None, stream=None):
+        """out (nres x N residues) <- encoding of `slots` complex values held
+        in vals (float64 device buffer, transformed in place)."""
+        nres = self.k if nres is None else nres
+        check(self.lib.aoa_encode(self.h, _stream(stream), _ptr(out), _ptr(vals), slots, int(scale), nres))
+
+    def decode(self, vals, inp, slots, scale, stream=None):
+        """vals (float64 device buffer, 2*slots) <- slot values of limb 0 of inp (coefficient domain)."""
+        check(self.lib.aoa_decode(self.h, _stream(stream), _ptr(vals), _ptr(inp), slots, int(scale)))
+
+    def sample(self, out, kind, seed, nonce, res0=0, nres=None, stream=None):
+        """Limbs res0..res0+nres-1 of out drawn from distribution `kind`
+        (SAMPLE_UNIFORM, _GAUSSIAN, _ZO, _HW, _NARROW) keyed by (seed, nonce)."""
+        nres = self.k + self.kl - res0 if nres is None else nres
+        check(self.lib.aoa_sample(self.h, _stream(stream), _ptr(out), res0, nres, kind, int(seed), int(nonce)))
 
     def rescale(self, c0, c1, level, stream=None):
         check(self.lib.aoa_rescale(self.h, _stream(stream), _ptr(c0), _ptr(c1), level))

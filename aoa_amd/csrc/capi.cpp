@@ -70,6 +70,9 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
       {H.inv_b_mod_q.data(), H.inv_b_mod_q.size() * 8, 0},
       {H.b_mod_q.data(), H.b_mod_q.size() * 8, 0},
       {H.inv_ql_mod_qi.data(), H.inv_ql_mod_qi.size() * 8, 0},
+      {H.enc_w.data(), H.enc_w.size() * 8, 0},
+      {H.rot_group.data(), H.rot_group.size() * 4, 0},
+      {H.gauss_cdt.data(), H.gauss_cdt.size() * 8, 0},
   };
   const int nsec = sizeof(secs) / sizeof(secs[0]);
   size_t total = 0;
@@ -105,6 +108,9 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
   D.inv_b_mod_q = (const uint64_t*)(b + secs[7].off);
   D.b_mod_q = (const uint64_t*)(b + secs[8].off);
   D.inv_ql_mod_qi = (const uint64_t*)(b + secs[9].off);
+  D.enc_w = (const double*)(b + secs[10].off);
+  D.rot_group = (const uint32_t*)(b + secs[11].off);
+  D.gauss_cdt = (const uint64_t*)(b + secs[12].off);
   *out = h;
   return AOA_OK;
 }
@@ -153,6 +159,14 @@ extern "C" int64_t aoa_params_table(const aoa_params* h, int which, uint64_t* ou
     case 9:
       for (int r = 0; r < R; ++r) v.push_back(H.dprimes[r].mu);
       break;
+    case 11:
+      v.resize(H.enc_w.size());
+      std::memcpy(v.data(), H.enc_w.data(), v.size() * 8);
+      break;
+    case 12:
+      for (uint32_t g : H.rot_group) v.push_back(g);
+      break;
+    case 13: v = H.gauss_cdt; break;
     default: return -fail(AOA_ERR_ARG, "unknown table");
   }
   if (!out) return (int64_t)v.size();
@@ -222,6 +236,50 @@ extern "C" int aoa_conjugate(const aoa_params* h, void* stream, uint64_t* out, c
   if (nres <= 0 || nres > h->host.k + h->host.K) return fail(AOA_ERR_ARG, "limb count out of range");
   launch_conjugate(h->dev, out, in, nres, (hipStream_t)stream);
   return check_launch("aoa_conjugate");
+}
+
+static bool slots_ok(const aoa_params* h, int n) { return n >= 1 && n <= h->host.N / 2 && !(n & (n - 1)); }
+
+extern "C" int aoa_special_fft(const aoa_params* h, void* stream, double* vals, int n, int dir) {
+  if (!h || !vals) return fail(AOA_ERR_ARG, "null argument");
+  if (!slots_ok(h, n)) return fail(AOA_ERR_ARG, "size must be a power of two <= N/2");
+  if (dir != AOA_FORWARD && dir != AOA_INVERSE) return fail(AOA_ERR_ARG, "unknown direction");
+  if (!aligned16(vals)) return fail(AOA_ERR_ARG, "values must be 16-byte aligned");
+  launch_special_fft(h->dev, vals, n, dir == AOA_INVERSE, (hipStream_t)stream);
+  return check_launch("aoa_special_fft");
+}
+
+extern "C" int aoa_encode(const aoa_params* h, void* stream, uint64_t* out, double* vals, int slots, uint64_t scale,
+                          int nres) {
+  if (!h || !out || !vals) return fail(AOA_ERR_ARG, "null argument");
+  if (!slots_ok(h, slots)) return fail(AOA_ERR_ARG, "slots must be a power of two <= N/2");
+  if (scale == 0) return fail(AOA_ERR_ARG, "scale must be positive");
+  if (nres <= 0 || nres > h->host.k) return fail(AOA_ERR_ARG, "limb count out of range");
+  if (!aligned16(vals)) return fail(AOA_ERR_ARG, "values must be 16-byte aligned");
+  launch_special_fft(h->dev, vals, slots, 1, (hipStream_t)stream);
+  launch_encode_pack(h->dev, out, vals, slots, (double)scale, nres, (hipStream_t)stream);
+  return check_launch("aoa_encode");
+}
+
+extern "C" int aoa_decode(const aoa_params* h, void* stream, double* vals, const uint64_t* in, int slots,
+                          uint64_t scale) {
+  if (!h || !vals || !in) return fail(AOA_ERR_ARG, "null argument");
+  if (!slots_ok(h, slots)) return fail(AOA_ERR_ARG, "slots must be a power of two <= N/2");
+  if (scale == 0) return fail(AOA_ERR_ARG, "scale must be positive");
+  if (!aligned16(vals)) return fail(AOA_ERR_ARG, "values must be 16-byte aligned");
+  launch_decode_unpack(h->dev, vals, in, slots, (double)scale, (hipStream_t)stream);
+  launch_special_fft(h->dev, vals, slots, 0, (hipStream_t)stream);
+  return check_launch("aoa_decode");
+}
+
+extern "C" int aoa_sample(const aoa_params* h, void* stream, uint64_t* out, int res0, int nres, int kind,
+                          uint64_t seed, uint64_t nonce) {
+  if (!h || !out) return fail(AOA_ERR_ARG, "null argument");
+  if (kind < AOA_SAMPLE_UNIFORM || kind > AOA_SAMPLE_NARROW) return fail(AOA_ERR_ARG, "unknown distribution");
+  const int R = h->host.k + h->host.K;
+  if (res0 < 0 || nres <= 0 || res0 + nres > R) return fail(AOA_ERR_ARG, "limb range out of bounds");
+  launch_sample(h->dev, out, res0, nres, kind, seed, nonce, (hipStream_t)stream);
+  return check_launch("aoa_sample");
 }
 
 extern "C" int aoa_rescale(const aoa_params* h, void* stream, uint64_t* c0, uint64_t* c1, int level) {

@@ -9,6 +9,9 @@
 namespace aoa {
 
 enum EltOp { ELT_ADD = 0, ELT_SUB = 1, ELT_MUL = 2, ELT_NEG = 3, ELT_MULADD = 5 };
+// Sampler distributions (same values as the C-ABI's AOA_SAMPLE_*).
+enum SampleKind { SAMPLE_UNIFORM = 0, SAMPLE_GAUSSIAN = 1, SAMPLE_ZO = 2, SAMPLE_HW = 3, SAMPLE_NARROW = 4 };
+constexpr int kHwMaxAttempts = 4096;
 
 int ntt_default_n1(int N);
 bool ntt_valid_split(int N, int N1);
@@ -33,6 +36,21 @@ void launch_tensor(const DevParams& P, uint64_t* d0, uint64_t* d1, uint64_t* d2,
 void launch_eltwise_scalar(const DevParams& P, int op, uint64_t* out, const uint64_t* a, uint64_t b, int nlimbs,
                            int prime0, hipStream_t st);
 void launch_conjugate(const DevParams& P, uint64_t* out, const uint64_t* in, int nlimbs, hipStream_t st);
+
+// CKKS encoder (FP64).  vals holds n complex values (re, im interleaved);
+// the special FFT runs in place (inverse: encoding direction, with the
+// division by n).  encode_pack writes k limbs of N residues from slots values;
+// decode_unpack reads limb 0 (coefficient domain) into slots values.
+// Sampler: limbs res0..res0+nres-1 of out (poly base pointer, limb r at
+// out + r*N) from distribution `kind`, keyed by (seed, nonce).
+void launch_sample(const DevParams& P, uint64_t* out, int res0, int nres, int kind, uint64_t seed, uint64_t nonce,
+                   hipStream_t st);
+
+void launch_special_fft(const DevParams& P, double* vals, int n, int inverse, hipStream_t st);
+void launch_encode_pack(const DevParams& P, uint64_t* out, const double* vals, int slots, double scale, int k,
+                        hipStream_t st);
+void launch_decode_unpack(const DevParams& P, double* vals, const uint64_t* in, int slots, double scale,
+                          hipStream_t st);
 void launch_modup(const DevParams& P, const uint64_t* x, uint64_t* outB, int level, hipStream_t st);
 
 // ModDown of npolys polys.  inQ[y] holds Q limbs 0..level, inB[y] the K

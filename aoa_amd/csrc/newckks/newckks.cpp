@@ -18,7 +18,6 @@
 #include <vector>
 
 #include "../../../include/newckks/newckks.h"
-#include "fft.hpp"
 
 // ------------------------------------------------------------ helpers ------
 
@@ -279,69 +278,26 @@ std::string poly_residue_to_string(Context* ctx, poly_t* a, int id) {
 
 // ------------------------------------------------------------ sampler ------
 
-Sampler::Sampler(Context* c, uint64_t seed) : ctx(c), rng(seed) {}
+Sampler::Sampler(Context* c, uint64_t s) : ctx(c), seed(s), nonce(0) {}
 
-// Small signed values, the same in every residue (negatives as p - |v|).
-void Sampler::upload_small(poly_t* p, poly_bases b, const std::vector<int64_t>& v) {
+// Every draw takes the next nonce, so a context's sequence of polynomials is
+// fixed by its seed.
+void Sampler::draw(poly_t* p, poly_bases b, int kind) {
   poly_init(ctx, p, b);
-  const int N = CUDAManager::N, R = CUDAManager::get_n_residues(b);
-  std::vector<uint64_t> h((size_t)N * R);
-  for (int r = 0; r < R; ++r) {
-    const uint64_t q = r < (int)CUDAManager::RNSQPrimes.size() ? CUDAManager::RNSQPrimes[r]
-                                                                : CUDAManager::RNSBPrimes[r - CUDAManager::RNSQPrimes.size()];
-    for (int i = 0; i < N; ++i) {
-      const int64_t x = v[i];
-      h[(size_t)r * N + i] = x >= 0 ? (uint64_t)x % q : (q - ((uint64_t)(-x) % q)) % q;
-    }
-  }
-  hip_ok(hipMemcpyAsync(p->d_coefs, h.data(), h.size() * 8, hipMemcpyHostToDevice, ctx->get_stream()), "hipMemcpyAsync");
-  hip_ok(hipStreamSynchronize(ctx->get_stream()), "hipStreamSynchronize");
+  ok(aoa_sample(CUDAManager::params(), ctx->get_stream(), p->d_coefs, 0, CUDAManager::get_n_residues(b), kind, seed,
+                nonce++));
   p->state = RNSSTATE;
 }
 
-void Sampler::sample_uniform(poly_t* p, poly_bases b) {
-  poly_init(ctx, p, b);
-  const int N = CUDAManager::N, R = CUDAManager::get_n_residues(b);
-  std::vector<uint64_t> h((size_t)N * R);
-  for (int r = 0; r < R; ++r) {
-    const uint64_t q = r < (int)CUDAManager::RNSQPrimes.size() ? CUDAManager::RNSQPrimes[r]
-                                                                : CUDAManager::RNSBPrimes[r - CUDAManager::RNSQPrimes.size()];
-    std::uniform_int_distribution<uint64_t> d(0, q - 1);
-    for (int i = 0; i < N; ++i) h[(size_t)r * N + i] = d(rng);
-  }
-  hip_ok(hipMemcpyAsync(p->d_coefs, h.data(), h.size() * 8, hipMemcpyHostToDevice, ctx->get_stream()), "hipMemcpyAsync");
-  hip_ok(hipStreamSynchronize(ctx->get_stream()), "hipStreamSynchronize");
-  p->state = RNSSTATE;
-}
+void Sampler::sample_uniform(poly_t* p, poly_bases b) { draw(p, b, AOA_SAMPLE_UNIFORM); }  // sampler.cu:21-45
 
-void Sampler::sample_DG(poly_t* p, poly_bases b) {  // sampler.cu:72-117: round(sigma * normal)
-  std::normal_distribution<double> d(0.0, 1.0);
-  std::vector<int64_t> v(CUDAManager::N);
-  for (auto& x : v) x = (int64_t)std::nearbyint(d(rng) * GAUSSIAN_STD_DEVIATION + GAUSSIAN_BOUND);
-  upload_small(p, b, v);
-}
+void Sampler::sample_DG(poly_t* p, poly_bases b) { draw(p, b, AOA_SAMPLE_GAUSSIAN); }  // sampler.cu:72-117
 
-// ternary values with exactly `weight` nonzeros at random positions
-static std::vector<int64_t> sparse_ternary(std::mt19937_64& rng, int N, int weight) {
-  std::vector<int64_t> v(N, 0);
-  for (int i = 0; i < weight && i < N; ++i) v[i] = (rng() & 1) ? 1 : -1;
-  std::shuffle(v.begin(), v.end(), rng);
-  return v;
-}
+void Sampler::sample_Z0(poly_t* p, poly_bases b) { draw(p, b, AOA_SAMPLE_ZO); }  // sampler.cu:119-155
 
-void Sampler::sample_Z0(poly_t* p, poly_bases b) {  // sampler.cu:119-155: N/2 nonzeros
-  upload_small(p, b, sparse_ternary(rng, CUDAManager::N, CUDAManager::N / 2));
-}
+void Sampler::sample_hw(poly_t* p, poly_bases b) { draw(p, b, AOA_SAMPLE_HW); }  // sampler.cu:157-192
 
-void Sampler::sample_hw(poly_t* p, poly_bases b) {  // sampler.cu:157-192: Hamming weight 64
-  upload_small(p, b, sparse_ternary(rng, CUDAManager::N, HAMMINGWEIGHT));
-}
-
-void Sampler::sample_narrow(poly_t* p, poly_bases b) {
-  std::vector<int64_t> v(CUDAManager::N);
-  for (auto& x : v) x = (int64_t)(rng() % 3) - 1;
-  upload_small(p, b, v);
-}
+void Sampler::sample_narrow(poly_t* p, poly_bases b) { draw(p, b, AOA_SAMPLE_NARROW); }  // sampler.cu:194-225
 
 // ------------------------------------------------------------ ciphertexts --
 
@@ -379,51 +335,50 @@ void cipher_copy(Context* ctx, cipher_t* b, cipher_t* a) {
 
 // ------------------------------------------------------------ encoder ------
 
-void fftSpecial(std::complex<double>* vals, const long size) { aoa_fft::special_fft(vals, size); }
+// The FFTs run on the device (aoa_special_fft): same loops, twiddles and
+// rounding as the reference's host fftSpecial / fftSpecialInv.
+static void special_fft_host(std::complex<double>* vals, long size, int dir) {
+  void* d = nullptr;
+  const size_t bytes = (size_t)size * sizeof(std::complex<double>);
+  hip_ok(hipMalloc(&d, bytes), "hipMalloc");
+  hip_ok(hipMemcpy(d, vals, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+  const int rc = aoa_special_fft(CUDAManager::params(), nullptr, (double*)d, (int)size, dir);
+  hipError_t e = rc == AOA_OK ? hipMemcpy(vals, d, bytes, hipMemcpyDeviceToHost) : hipSuccess;
+  (void)hipFree(d);
+  ok(rc);
+  hip_ok(e, "hipMemcpy");
+}
 
-void fftSpecialInv(std::complex<double>* vals, const long size) { aoa_fft::special_fft_inv(vals, size); }
+void fftSpecial(std::complex<double>* vals, const long size) { special_fft_host(vals, size, AOA_FORWARD); }
 
-// encoder.cu:143-221: value * scale truncated to int64, reduced per residue,
-// real parts at s*gap and imaginary parts at N/2 + s*gap.
-void ckks_encode(Context* ctx, uint64_t* d_out, std::complex<double>* in, int filled, int empty, uint64_t scale) {
+void fftSpecialInv(std::complex<double>* vals, const long size) { special_fft_host(vals, size, AOA_INVERSE); }
+
+// encoder.cu:143-221: the values (filled, then zeros) go to the device, where
+// aoa_encode runs the inverse FFT and writes every Q residue.
+void ckks_encode(CKKSContext* ctx, uint64_t* d_out, std::complex<double>* in, int filled, int empty,
+                 uint64_t scale) {
   const int slots = filled + empty;
   const int N = CUDAManager::N;
-  if (slots <= 0 || slots > N / 2 || (slots & (slots - 1))) throw std::runtime_error("slots must be a power of two <= N/2");
-  std::vector<std::complex<double>> v(slots, 0.0);
-  for (int i = 0; i < filled; ++i) v[i] = in[i];
-  fftSpecialInv(v.data(), slots);
-  const int k = (int)CUDAManager::RNSQPrimes.size();
-  const int gap = (N / 2) / slots;
-  std::vector<uint64_t> h((size_t)N * k, 0);
-  for (int s = 0; s < slots; ++s) {
-    const int64_t re = (int64_t)(v[s].real() * (double)scale);
-    const int64_t im = (int64_t)(v[s].imag() * (double)scale);
-    for (int l = 0; l < k; ++l) {
-      const uint64_t p = CUDAManager::RNSQPrimes[l];
-      auto adj = [p](int64_t x) -> uint64_t { return x >= 0 ? (uint64_t)x % p : (p - ((uint64_t)(-x) % p)) % p; };
-      h[(size_t)l * N + (size_t)s * gap] = adj(re);
-      h[(size_t)l * N + (size_t)s * gap + N / 2] = adj(im);
-    }
-  }
-  hip_ok(hipMemcpyAsync(d_out, h.data(), h.size() * 8, hipMemcpyHostToDevice, ctx->get_stream()), "hipMemcpyAsync");
+  if (filled < 0 || empty < 0 || slots <= 0 || slots > N / 2 || (slots & (slots - 1)))
+    throw std::runtime_error("slots must be a power of two <= N/2");
+  if (in != ctx->h_val) std::memmove(ctx->h_val, in, (size_t)filled * sizeof(std::complex<double>));
+  for (int i = filled; i < slots; ++i) ctx->h_val[i] = 0.0;
+  hip_ok(hipMemcpyAsync(ctx->d_val, ctx->h_val, (size_t)slots * sizeof(std::complex<double>), hipMemcpyHostToDevice,
+                        ctx->get_stream()),
+         "hipMemcpyAsync");
+  ok(aoa_encode(CUDAManager::params(), ctx->get_stream(), d_out, (double*)ctx->d_val, slots, scale,
+                (int)CUDAManager::RNSQPrimes.size()));
   hip_ok(hipStreamSynchronize(ctx->get_stream()), "hipStreamSynchronize");
 }
 
-// encoder.cu:234-298: residue 0 centred, divided by the scale, then the FFT.
-void ckks_decode(Context* ctx, std::complex<double>* out, uint64_t* d_a, int slots, uint64_t scale) {
-  const int N = CUDAManager::N;
-  std::vector<uint64_t> h(N);
-  hip_ok(hipMemcpyAsync(h.data(), d_a, (size_t)N * 8, hipMemcpyDeviceToHost, ctx->get_stream()), "hipMemcpyAsync");
+// encoder.cu:223-298: residue 0 centred and divided by the scale, then the
+// forward FFT, on the device; the slot values come back to the host.
+void ckks_decode(CKKSContext* ctx, std::complex<double>* out, uint64_t* d_a, int slots, uint64_t scale) {
+  ok(aoa_decode(CUDAManager::params(), ctx->get_stream(), (double*)ctx->d_val, d_a, slots, scale));
+  hip_ok(hipMemcpyAsync(out, ctx->d_val, (size_t)slots * sizeof(std::complex<double>), hipMemcpyDeviceToHost,
+                        ctx->get_stream()),
+         "hipMemcpyAsync");
   hip_ok(hipStreamSynchronize(ctx->get_stream()), "hipStreamSynchronize");
-  const uint64_t p = CUDAManager::RNSQPrimes[0];
-  const int gap = (N / 2) / slots;
-  auto centred = [p, scale](uint64_t x) -> double {
-    const double v = x <= (p >> 1) ? (double)x : -(double)(p - x);
-    return v / (double)scale;
-  };
-  for (int s = 0; s < slots; ++s)
-    out[s] = std::complex<double>(centred(h[(size_t)s * gap]), centred(h[(size_t)s * gap + N / 2]));
-  fftSpecial(out, slots);
 }
 
 // encoder.cu:325-363: the input is brought to the coefficient domain in place,
@@ -461,7 +416,8 @@ CKKSContext::CKKSContext() : Context() {
   poly_init(this, m);
   poly_init(this, e);
   for (auto& c : caux) cipher_init(this, &c);
-  h_val = new std::complex<double>[CUDAManager::N];
+  hip_ok(hipHostMalloc((void**)&h_val, (size_t)CUDAManager::N * sizeof(std::complex<double>), 0), "hipHostMalloc");
+  hip_ok(hipMalloc((void**)&d_val, (size_t)CUDAManager::N * sizeof(std::complex<double>)), "hipMalloc");
   // Maclaurin series (ckkscontext.h:101-135), the reference's values as they
   // are (its cosine row has 1/24 at x^3, which matches cos only on [0, 1])
   maclaurin_coeffs[EXPONENT] = new double[11]{1, 1, 1. / 2, 1. / 6, 1. / 24, 1. / 120, 1. / 720, 1. / 5040,
@@ -489,7 +445,8 @@ CKKSContext::CKKSContext() : Context() {
 CKKSContext::~CKKSContext() {
   (void)hipStreamSynchronize(get_stream());
   for (auto& c : caux) cipher_free(this, &c);
-  delete[] h_val;
+  (void)hipHostFree(h_val);
+  (void)hipFree(d_val);
   for (auto& kv : maclaurin_coeffs) delete[] kv.second;
   poly_free(this, u);
   poly_free(this, m);

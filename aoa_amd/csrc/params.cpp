@@ -1,6 +1,7 @@
 // Host-side construction of a parameter set (primes, twiddles, RNS constants).
 #include "params.hpp"
 
+#include <cmath>
 #include <cstring>
 
 namespace aoa {
@@ -220,6 +221,41 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
     P.inv_b_mod_q[2 * i + 1] = shoup_quot(inv, qi);
     P.b_mod_q[2 * i] = bfull;
     P.b_mod_q[2 * i + 1] = shoup_quot(bfull, qi);
+  }
+  // Encoder tables: the roots exp(2 pi i t / 2N) and the rotation group
+  // 5^j mod 2N (aoa-ntt/src/ckks/encoder.cu:36-48).  The reference's ksiPows
+  // come from cos(angle) and sin(angle) in host code that an optimising g++
+  // turns into one glibc sincos call, whose results differ from separate sin
+  // and cos in the last bit for a few angles; sincos is called here through a
+  // volatile pointer so that no compiler substitutes anything else.
+  {
+    void (*volatile sincos_fn)(double, double*, double*) = ::sincos;
+    const size_t M = 2 * (size_t)N;
+    P.enc_w.resize(2 * M);
+    for (size_t t = 0; t < M; ++t) {
+      const double ang = 2.0 * M_PI * (double)t / (double)M;
+      sincos_fn(ang, &P.enc_w[2 * t + 1], &P.enc_w[2 * t]);
+    }
+    // Rounded Gaussian e = round(sigma z): P(e <= m) = Phi((m + 1/2) / sigma),
+    // scaled to 2^64 and rounded; the upper half from the complement, so
+    // that every entry keeps its relative precision; the last entry
+    // saturates at 2^64 - 1.
+    P.gauss_cdt.resize(2 * kGaussHalf);
+    for (int j = 0; j < 2 * kGaussHalf; ++j) {
+      const double x = ((double)(j - kGaussHalf) + 0.5) / kGaussSigma;
+      if (x < 0) {
+        P.gauss_cdt[j] = (uint64_t)std::nearbyint(std::ldexp(0.5 * std::erfc(-x / std::sqrt(2.0)), 64));
+      } else {
+        const uint64_t tail = (uint64_t)std::nearbyint(std::ldexp(0.5 * std::erfc(x / std::sqrt(2.0)), 64));
+        P.gauss_cdt[j] = tail == 0 ? ~(uint64_t)0 : (uint64_t)0 - tail;
+      }
+    }
+    P.rot_group.resize(N / 2);
+    uint64_t g = 1;
+    for (int j = 0; j < N / 2; ++j) {
+      P.rot_group[j] = (uint32_t)g;
+      g = g * 5 % M;
+    }
   }
   return "";
 }

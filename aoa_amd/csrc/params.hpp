@@ -45,6 +45,13 @@ struct DevPrime {
 
 // Pointers into one device allocation holding every table of a parameter set.
 // Passed to kernels by value.
+// Sampler constants: the reference's Gaussian standard deviation and
+// Hamming weight (aoa-ntt/include/newckks/cuda/sampler.h:71-74), and the
+// half-width of the Gaussian table (P(|e| > 31) < 2^-64 at sigma 3.2).
+constexpr double kGaussSigma = 3.2;
+constexpr int kGaussHalf = 32;
+constexpr int kHammingWeight = 64;
+
 struct DevParams {
   int N, logN, k, K;
   int ntt_n1;  // NTT split N = N1 x (N/N1); 0 = built-in default (autotuner sets it)
@@ -61,6 +68,9 @@ struct DevParams {
   const uint64_t* inv_b_mod_q;   // [k][2]: B^-1 mod q_i, Shoup
   const uint64_t* b_mod_q;       // [k][2]: B mod q_i, Shoup
   const uint64_t* inv_ql_mod_qi; // [k][k][2]: (l,i) -> q_l^-1 mod q_i, Shoup
+  const double* enc_w;           // [2N][2]: exp(2 pi i t / 2N), the encoder's roots of unity
+  const uint32_t* rot_group;     // [N/2]: 5^j mod 2N
+  const uint64_t* gauss_cdt;     // [2 kGaussHalf]: round(2^64 P(e <= j - kGaussHalf)), e = round(3.2 z)
 };
 
 struct HostParams {
@@ -70,6 +80,9 @@ struct HostParams {
   std::vector<DevPrime> dprimes;
   std::vector<uint64_t> tw_fwd, tw_inv, inv_qhat, qhat_mod_b, inv_bhat, bhat_mod_q,
       inv_b_mod_q, b_mod_q, inv_ql_mod_qi;
+  std::vector<double> enc_w;        // 2N complex roots, interleaved (re, im)
+  std::vector<uint32_t> rot_group;  // N/2 entries
+  std::vector<uint64_t> gauss_cdt;  // 2 kGaussHalf entries
 };
 
 // Builds every table.  Returns an empty string on success, else the reason

@@ -63,7 +63,9 @@ int aoa_params_primes(const aoa_params* params, uint64_t* out, int n);
  * which: 0 forward twiddles [k+kl][N] (psi^brev(i)), 1 inverse twiddles,
  * 2 (Q_l/q_i)^-1 mod q_i [k][k], 3 (Q_l/q_i) mod b_j [k][kl][k] (l, j, i),
  * 4 (B/b_j)^-1 mod b_j [kl], 5 (B/b_j) mod q_i [k][kl], 6 B^-1 mod q_i [k],
- * 7 B mod q_i [k], 8 q_l^-1 mod q_i [k][k], 9 Barrett MU floor(4^nb/p) [k+kl].
+ * 7 B mod q_i [k], 8 q_l^-1 mod q_i [k][k], 9 Barrett MU floor(4^nb/p) [k+kl],
+ * 11 encoder roots exp(2 pi i t / 2N) [2N][re, im] (IEEE double bit patterns),
+ * 12 rotation group 5^j mod 2N [N/2], 13 Gaussian table of aoa_sample [64].
  * Returns the number of words written (or needed, if out is NULL). */
 int64_t aoa_params_table(const aoa_params* params, int which, uint64_t* out, int64_t n);
 
@@ -94,6 +96,50 @@ int aoa_eltwise_scalar(const aoa_params* params, void* stream, int op, uint64_t*
  * Replaces execute_conjugation (aoa-ntt/src/ckks/encoder.cu:391-427).  out
  * and in must not overlap. */
 int aoa_conjugate(const aoa_params* params, void* stream, uint64_t* out, const uint64_t* in, int nres);
+
+/* CKKS encoder on the device (FP64).  `vals` is a device buffer of n complex
+ * values (re, im interleaved doubles, 16-byte aligned), transformed in place.
+ *
+ * aoa_special_fft: the canonical-embedding FFT of n (a power of two <= N/2)
+ * values; AOA_FORWARD is the decoding direction, fftSpecial
+ * (aoa-ntt/src/ckks/encoder.cu:33-73), AOA_INVERSE the encoding direction
+ * including the division by n, fftSpecialInv (encoder.cu:75-124).  Same loop
+ * order, twiddles and rounding as those host loops.
+ *
+ * aoa_encode: ckks_encode (encoder.cu:143-221) after the caller has placed
+ * the `slots` values (filled, then zeros) in `vals`: the inverse FFT in
+ * place, then value*scale truncated to int64 and reduced per prime into
+ * limbs 0..nres-1 of `out` at coefficients s*gap (real) and N/2 + s*gap
+ * (imaginary), gap = N/2/slots, zeros elsewhere.  A negative value whose
+ * magnitude is a multiple of q_r is written as 0 (the reference writes q_r,
+ * the same residue).  |value*scale| must be below 2^63.
+ *
+ * aoa_decode: ckks_decode (encoder.cu:223-298): limb 0 of `in` (coefficient
+ * domain) at s*gap and N/2 + s*gap, centred and divided by scale, then the
+ * forward FFT, into `vals` (slots values). */
+int aoa_special_fft(const aoa_params* params, void* stream, double* vals, int n, int dir);
+int aoa_encode(const aoa_params* params, void* stream, uint64_t* out, double* vals, int slots, uint64_t scale,
+               int nres);
+int aoa_decode(const aoa_params* params, void* stream, double* vals, const uint64_t* in, int slots,
+               uint64_t scale);
+
+/* Sampler distributions: Sampler::sample_uniform, sample_DG, sample_Z0,
+ * sample_hw and sample_narrow (aoa-ntt/include/newckks/cuda/sampler.h:109-119). */
+enum { AOA_SAMPLE_UNIFORM = 0, AOA_SAMPLE_GAUSSIAN = 1, AOA_SAMPLE_ZO = 2, AOA_SAMPLE_HW = 3, AOA_SAMPLE_NARROW = 4 };
+
+/* Draw limbs res0 .. res0+nres-1 of the polynomial at `out` (limb r at
+ * out + r*N, prime r) on the device.  Replaces the Sampler methods
+ * (aoa-ntt/src/cuda/sampler.cu:21-225), which draw from cuRAND and host rand().
+ * Here every value is a function of (seed, nonce, distribution, limb,
+ * coefficient) through Philox4x32-10: the same arguments give the same
+ * polynomial, and a caller takes a fresh nonce per draw.  UNIFORM draws each
+ * residue uniformly in [0, q_r); GAUSSIAN draws e = round(3.2 z), z ~ N(0,1);
+ * ZO draws 0, +1, -1 with probabilities 1/2, 1/4, 1/4; NARROW draws -1, 0, 1
+ * uniformly; HW places exactly 64 values +-1 at distinct positions, zeros
+ * elsewhere.  The small distributions draw one value per coefficient, the
+ * same on every limb (negatives as q_r + v).  Coefficient domain. */
+int aoa_sample(const aoa_params* params, void* stream, uint64_t* out, int res0, int nres, int kind, uint64_t seed,
+               uint64_t nonce);
 
 /* Rescale c0 and c1 in place from `level` (coefficient domain): limb i < level
  * becomes (c_i - c_level) q_level^-1 mod q_i.  Replaces

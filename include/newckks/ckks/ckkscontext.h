@@ -28,7 +28,8 @@ class CKKSContext : public Context {
   cipher_t caux[3];   // auxiliary ciphertexts (sumslots)
   void* d_ws;         // workspace of aoa_hommul / aoa_rotate at the top level
   size_t ws_bytes;
-  std::complex<double>* h_val;                       // encoding scratch (N values)
+  std::complex<double>* h_val;                       // encoding scratch, host (pinned, N values)
+  std::complex<double>* d_val;                       // encoding scratch, device (N values)
   std::map<std::string, double*> maclaurin_coeffs;  // series of exp, sin, cos, sigmoid, logs
 
   CKKSContext();

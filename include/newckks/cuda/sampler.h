@@ -1,14 +1,13 @@
-// Sampler (aoa-ntt/include/newckks/cuda/sampler.h:80-122): uniform, discrete
-// Gaussian (sigma 3.2), ternary with N/2 nonzeros (Z0) and Hamming weight 64.
-// Seeded host generator (std::mt19937_64); streams differ from the
-// reference's cuRAND streams, the distributions are the same.
+// Sampler (aoa-ntt/include/newckks/cuda/sampler.h:80-122): uniform, rounded
+// Gaussian (sigma 3.2), ternary ZO, Hamming weight 64 and narrow ternary,
+// drawn on the device (aoa_sample: Philox4x32-10 keyed by the seed, one
+// nonce per draw).  The streams differ from the reference's cuRAND and
+// rand() streams; the distributions are the same, except that ZO draws each
+// coefficient independently (the reference fixes N/2 nonzeros).
 #ifndef NEWCKKS_SAMPLER_H
 #define NEWCKKS_SAMPLER_H
 
 #include <stdint.h>
-
-#include <random>
-#include <vector>
 
 #include "../defines.h"
 
@@ -26,12 +25,16 @@ class Sampler {
   void sample_Z0(poly_t* p, poly_bases b);
   void sample_hw(poly_t* p, poly_bases b);
   void sample_narrow(poly_t* p, poly_bases b);
-  void set_seed(uint64_t seed) { rng.seed(seed); }
+  // Restarts the sequence of draws.
+  void set_seed(uint64_t s) {
+    seed = s;
+    nonce = 0;
+  }
 
  private:
   Context* ctx;
-  std::mt19937_64 rng;
-  void upload_small(poly_t* p, poly_bases b, const std::vector<int64_t>& v);
+  uint64_t seed, nonce;
+  void draw(poly_t* p, poly_bases b, int kind);
 };
 
 #endif

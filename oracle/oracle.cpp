@@ -15,7 +15,10 @@
 // negacyclic products vs schoolbook; RNS constants vs Python big integers;
 // ModUp vs the CRT identity; ModDown vs exact big-integer ModDown plus the
 // Barrett deviation; rescale vs (a - a_l)/q_l.  The encoder FFT is pinned by
-// FP64 round trips and by the canonical-embedding definition.
+// FP64 round trips and by the canonical-embedding definition; encode/decode
+// by the embedding (the residues, read as a polynomial over the integers,
+// evaluate to scale * slot values at exp(i pi 5^j / N)), by agreement of the
+// signed value across limbs and by decode o encode = id.
 #include <omp.h>
 #include <stdint.h>
 
@@ -591,6 +594,14 @@ static long powmod_small(long base, long e, long m) {
 }
 
 // ------------------------------------------------ encoder FFT (encoder.cu:10-124)
+// ksiPows[j] = (cos(angle), sin(angle)) (encoder.cu:44-48): an optimising g++
+// (the reference's Release build, -O3) emits one glibc sincos call for the
+// pair, which differs from separate cos and sin in the last bit for a few
+// angles; sincos is called explicitly, through a volatile pointer.
+static void libm_sincos(double a, double* s, double* c) {
+  void (*volatile fn)(double, double*, double*) = ::sincos;
+  fn(a, s, c);
+}
 static void array_bit_reverse(std::complex<double>* vals, long size) {  // encoder.cu:12-23
   for (long i = 1, j = 0; i < size; ++i) {
     long bit = size >> 1;
@@ -611,7 +622,9 @@ static void fft_special(std::complex<double>* vals, long size, long ringN) {
   std::vector<std::complex<double>> ksi(M + 1);
   for (long j = 0; j < M; ++j) {
     const double angle = 2.0 * M_PI * j / M;
-    ksi[j] = std::complex<double>(std::cos(angle), std::sin(angle));
+    double sn, cs;
+    libm_sincos(angle, &sn, &cs);
+    ksi[j] = std::complex<double>(cs, sn);
   }
   ksi[M] = ksi[0];
   array_bit_reverse(vals, size);
@@ -638,7 +651,9 @@ static void fft_special_inv(std::complex<double>* vals, long size, long ringN) {
   std::vector<std::complex<double>> ksi(M + 1);
   for (long j = 0; j < M; ++j) {
     const double angle = 2.0 * M_PI * j / M;
-    ksi[j] = std::complex<double>(std::cos(angle), std::sin(angle));
+    double sn, cs;
+    libm_sincos(angle, &sn, &cs);
+    ksi[j] = std::complex<double>(cs, sn);
   }
   ksi[M] = ksi[0];
   for (long len = size; len >= 1; len >>= 1)
@@ -654,6 +669,142 @@ static void fft_special_inv(std::complex<double>* vals, long size, long ringN) {
     }
   array_bit_reverse(vals, size);
   for (long i = 0; i < size; ++i) vals[i] /= (double)size;
+}
+
+// ------------------------------------------------ encoder (encoder.cu:126-298)
+// ADJUSTNEG (encoder.cu:128): a negative value whose magnitude is a multiple
+// of p gives p, not 0.
+static inline uint64_t adjust_neg(int64_t x, uint64_t p) {
+  return x >= 0 ? (uint64_t)x % p : p - ((uint64_t)(-x) % p);
+}
+// ADJUSTSCALE (encoder.cu:129).
+static inline double adjust_scale(uint64_t x, uint64_t p, uint64_t factor) {
+  return x <= (p >> 1) ? ((double)x) / factor : (((double)x) - ((double)p)) / (double)factor;
+}
+// ckks_encode (encoder.cu:166-221) with ckks_convert_C2PC (:141-164): the
+// caller's values (filled, then zeros) go through fftSpecialInv; residues of
+// nres limbs, zero outside the slot positions.
+static void encode(const Oracle& o, uint64_t* out, std::complex<double>* vals, int slots, uint64_t scale, int nres) {
+  fft_special_inv(vals, slots, o.N);
+  std::memset(out, 0, (size_t)nres * o.N * sizeof(uint64_t));
+  const int gap = (o.N >> 1) / slots;
+  for (int lid = 0; lid < nres; ++lid)
+    for (int sid = 0; sid < slots; ++sid) {
+      const int64_t mir = vals[sid].real() * scale;
+      const int64_t mii = vals[sid].imag() * scale;
+      const uint64_t p = o.P[lid];
+      out[(size_t)sid * gap + (size_t)lid * o.N] = adjust_neg(mir, p);
+      out[(size_t)sid * gap + (o.N >> 1) + (size_t)lid * o.N] = adjust_neg(mii, p);
+    }
+}
+// ckks_decode (encoder.cu:259-298) with ckks_convert_PC2C (:232-257): limb 0
+// (coefficient domain), then fftSpecial.
+static void decode(const Oracle& o, std::complex<double>* out, const uint64_t* a, int slots, uint64_t scale) {
+  const uint64_t p = o.P[0];
+  const int gap = (o.N >> 1) / slots;
+  for (int sid = 0; sid < slots; ++sid)
+    out[sid] = std::complex<double>(adjust_scale(a[(size_t)sid * gap], p, scale),
+                                    adjust_scale(a[(size_t)sid * gap + (o.N >> 1)], p, scale));
+  fft_special(out, slots, o.N);
+}
+
+// ------------------------------------------------ sampler
+// The reference draws from cuRAND (Sobol64 for uniform, XORWOW normals for
+// the Gaussian) and host rand() (sampler.cu:21-225), which cannot be
+// reproduced.  The product's sampler keeps the reference's distributions with
+// a counter-based generator; this is its restatement: Philox4x32-10 as
+// published by Salmon et al. (SC 2011, Random123), the counter / key layout,
+// and the mapping of 128 random bits to each distribution.
+struct Ctr4 {
+  uint32_t v[4];
+};
+static Ctr4 philox4x32_10(Ctr4 c, uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.v[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.v[2];
+    Ctr4 n;
+    n.v[0] = (uint32_t)(p1 >> 32) ^ c.v[1] ^ k0;
+    n.v[1] = (uint32_t)p1;
+    n.v[2] = (uint32_t)(p0 >> 32) ^ c.v[3] ^ k1;
+    n.v[3] = (uint32_t)p0;
+    c = n;
+  }
+  return c;
+}
+enum { S_UNIFORM = 0, S_GAUSSIAN = 1, S_ZO = 2, S_HW = 3, S_NARROW = 4 };
+static Ctr4 draw(uint64_t seed, uint64_t nonce, uint32_t tag, uint32_t idx) {
+  Ctr4 c{{idx, tag, (uint32_t)nonce, (uint32_t)(nonce >> 32)}};
+  return philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+// P(round(3.2 z) <= j - 32) * 2^64, rounded; upper half via the complement;
+// the last entry saturates.
+static std::vector<uint64_t> gauss_cdt() {
+  std::vector<uint64_t> t(64);
+  for (int j = 0; j < 64; ++j) {
+    const double x = ((double)(j - 32) + 0.5) / 3.2;
+    if (x < 0) {
+      t[j] = (uint64_t)std::nearbyint(std::ldexp(0.5 * std::erfc(-x / std::sqrt(2.0)), 64));
+    } else {
+      const uint64_t tail = (uint64_t)std::nearbyint(std::ldexp(0.5 * std::erfc(x / std::sqrt(2.0)), 64));
+      t[j] = tail == 0 ? ~(uint64_t)0 : (uint64_t)0 - tail;
+    }
+  }
+  return t;
+}
+static uint64_t small_res(int64_t v, uint64_t q) { return v >= 0 ? (uint64_t)v : q + v; }
+static void sample(const Oracle& o, uint64_t* out, int res0, int nres, int kind, uint64_t seed, uint64_t nonce) {
+  const int N = o.N;
+  if (kind == S_UNIFORM) {
+    for (int l = res0; l < res0 + nres; ++l)
+      for (int i = 0; i < N; ++i) {
+        const Ctr4 c = draw(seed, nonce, ((uint32_t)S_UNIFORM << 24) | (uint32_t)l, (uint32_t)i);
+        const unsigned __int128 X = ((unsigned __int128)(((uint64_t)c.v[3] << 32) | c.v[2]) << 64) |
+                                    (((uint64_t)c.v[1] << 32) | c.v[0]);
+        // floor(X q / 2^128) from the 192-bit product
+        const uint64_t q = o.P[l];
+        const unsigned __int128 lo = (unsigned __int128)(uint64_t)X * q;
+        const unsigned __int128 hi = (unsigned __int128)(uint64_t)(X >> 64) * q;
+        out[(size_t)l * N + i] = (uint64_t)((hi + (lo >> 64)) >> 64);
+      }
+    return;
+  }
+  if (kind == S_HW) {
+    for (int l = res0; l < res0 + nres; ++l) std::memset(out + (size_t)l * N, 0, (size_t)N * 8);
+    std::vector<char> used(N, 0);
+    const int weight = std::min(64, N);
+    int n = 0;
+    for (uint32_t a = 0; a < 4096 && n < weight; ++a) {
+      const Ctr4 c = draw(seed, nonce, ((uint32_t)S_HW << 24) | 0xFFFFFFu, a);
+      const uint32_t p = c.v[0] & (uint32_t)(N - 1);
+      if (used[p]) continue;
+      used[p] = 1;
+      const int64_t v = (c.v[1] & 1u) ? -1 : 1;
+      for (int l = res0; l < res0 + nres; ++l) out[(size_t)l * N + p] = small_res(v, o.P[l]);
+      ++n;
+    }
+    return;
+  }
+  static const std::vector<uint64_t> cdt = gauss_cdt();
+  for (int i = 0; i < N; ++i) {
+    const Ctr4 c = draw(seed, nonce, ((uint32_t)kind << 24) | 0xFFFFFFu, (uint32_t)i);
+    const uint64_t u = ((uint64_t)c.v[1] << 32) | c.v[0];
+    int64_t v;
+    if (kind == S_GAUSSIAN) {
+      int cnt = 0;
+      for (int j = 0; j < 64; ++j) cnt += u >= cdt[j];
+      v = cnt - 32;
+    } else if (kind == S_ZO) {
+      const uint32_t b = c.v[0] & 3u;
+      v = b < 2 ? 0 : (b == 2 ? 1 : -1);
+    } else {
+      v = (int64_t)(uint64_t)(((unsigned __int128)u * 3) >> 64) - 1;
+    }
+    for (int l = res0; l < res0 + nres; ++l) out[(size_t)l * N + i] = small_res(v, o.P[l]);
+  }
 }
 
 }  // namespace orc
@@ -702,6 +853,26 @@ void orc_fft_special(double* v, long size, long ringN, int inverse) {
     fft_special_inv(c, size, ringN);
   else
     fft_special(c, size, ringN);
+}
+// Encoder on interleaved (re, im) doubles: vals (slots values) is
+// transformed in place as in the reference.
+void orc_encode(void* h, uint64_t* out, double* vals, int slots, uint64_t scale, int nres) {
+  encode(*(Oracle*)h, out, reinterpret_cast<std::complex<double>*>(vals), slots, scale, nres);
+}
+void orc_decode(void* h, double* vals, const uint64_t* in, int slots, uint64_t scale) {
+  decode(*(Oracle*)h, reinterpret_cast<std::complex<double>*>(vals), in, slots, scale);
+}
+void orc_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  Ctr4 c{{ctr[0], ctr[1], ctr[2], ctr[3]}};
+  c = philox4x32_10(c, key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = c.v[i];
+}
+void orc_gauss_cdt(uint64_t* out) {
+  const std::vector<uint64_t> t = gauss_cdt();
+  for (int i = 0; i < 64; ++i) out[i] = t[i];
+}
+void orc_sample(void* h, uint64_t* out, int res0, int nres, int kind, uint64_t seed, uint64_t nonce) {
+  sample(*(Oracle*)h, out, res0, nres, kind, seed, nonce);
 }
 int orc_roots(void* h, uint64_t* out) {
   Oracle* o = (Oracle*)h;

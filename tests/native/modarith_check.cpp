@@ -5,7 +5,7 @@
 #include <random>
 
 #include "modarith.hpp"
-#include "newckks/fft.hpp"
+#include <cstring>
 #include "params.hpp"
 
 using namespace aoa;
@@ -110,15 +110,6 @@ int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t s
   return 0;
 }
 
-// The product's encoder FFT on interleaved (re, im) doubles.
-void chk_fft_special(double* v, long size, int inverse) {
-  auto* c = reinterpret_cast<std::complex<double>*>(v);
-  if (inverse)
-    aoa_fft::special_fft_inv(c, size);
-  else
-    aoa_fft::special_fft(c, size);
-}
-
 // The product's host tables (same numbering as aoa_params_table), built
 // without a GPU.  Returns the number of words (out may be null).
 int64_t chk_table(int N, int k, int kl, int prec, int which, uint64_t* out) {
@@ -142,6 +133,14 @@ int64_t chk_table(int N, int k, int kl, int prec, int which, uint64_t* out) {
       for (auto& d : H.dprimes) v.push_back(d.mu);
       break;
     case 10: v = H.primes; break;
+    case 11:  // encoder roots exp(2 pi i t / 2N) as (re, im) bit patterns
+      v.resize(H.enc_w.size());
+      std::memcpy(v.data(), H.enc_w.data(), v.size() * 8);
+      break;
+    case 12:
+      for (uint32_t g : H.rot_group) v.push_back(g);
+      break;
+    case 13: v = H.gauss_cdt; break;
     default: return -1;
   }
   if (out)

@@ -9,8 +9,7 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "native", "modarith_check.cpp")
 OUT = os.path.join(HERE, "native", "_build", "libchk.so")
 DEPS = [SRC, os.path.join(ROOT, "aoa_amd", "csrc", "params.cpp"), os.path.join(ROOT, "aoa_amd", "csrc", "modarith.hpp"),
-        os.path.join(ROOT, "aoa_amd", "csrc", "params.hpp"),
-        os.path.join(ROOT, "aoa_amd", "csrc", "newckks", "fft.hpp")]
+        os.path.join(ROOT, "aoa_amd", "csrc", "params.hpp")]
 _lib = None
 
 
@@ -34,8 +33,6 @@ def load():
     lib.chk_moddown_terms.restype = ctypes.c_int
     lib.chk_moddown_terms.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int64, ctypes.c_uint64,
                                                            ctypes.POINTER(ctypes.c_int64)]
-    lib.chk_fft_special.restype = None
-    lib.chk_fft_special.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_long, ctypes.c_int]
     lib.chk_table.restype = ctypes.c_int64
     lib.chk_table.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(ctypes.c_uint64)]
     _lib = lib

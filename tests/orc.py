@@ -50,6 +50,13 @@ def load():
         "orc_automorphism": (None, [_vp, _u64p, _u64p, ctypes.c_int, ctypes.c_int]),
         "orc_hommul": (None, [_vp] + [_u64p] * 6 + [ctypes.c_int, _u64p, _u64p, ctypes.c_int, ctypes.c_int]),
         "orc_rotate_left": (None, [_vp] + [_u64p] * 6 + [ctypes.c_int, ctypes.c_int]),
+        "orc_encode": (None, [_vp, _u64p, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_uint64,
+                              ctypes.c_int]),
+        "orc_decode": (None, [_vp, ctypes.POINTER(ctypes.c_double), _u64p, ctypes.c_int, ctypes.c_uint64]),
+        "orc_philox4x32_10": (None, [ctypes.POINTER(ctypes.c_uint32)] * 3),
+        "orc_gauss_cdt": (None, [_u64p]),
+        "orc_sample": (None, [_vp, _u64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                              ctypes.c_uint64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -57,6 +64,21 @@ def load():
         f.argtypes = args
     _lib = lib
     return lib
+
+
+def philox4x32_10(ctr, key):
+    c = np.array(ctr, dtype=np.uint32)
+    k = np.array(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    P = ctypes.POINTER(ctypes.c_uint32)
+    load().orc_philox4x32_10(c.ctypes.data_as(P), k.ctypes.data_as(P), out.ctypes.data_as(P))
+    return [int(x) for x in out]
+
+
+def gauss_cdt():
+    out = np.zeros(64, dtype=np.uint64)
+    load().orc_gauss_cdt(_p(out))
+    return out
 
 
 class Oracle:
@@ -113,6 +135,30 @@ class Oracle:
 
     def conjugate(self, out, inp, nres):
         self.lib.orc_conjugate(self.h, _p(out), _p(inp), nres)
+
+    def encode(self, vals, scale, nres=None):
+        """Residues (nres x N) of the slot values `vals` (complex, length a
+        power of two); the reference's ckks_encode with no empty slots."""
+        nres = self.k if nres is None else nres
+        v = np.ascontiguousarray(np.asarray(vals, dtype=np.complex128)).view(np.float64).copy()
+        out = np.zeros(nres * self.N, dtype=np.uint64)
+        self.lib.orc_encode(self.h, _p(out), v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(vals), scale,
+                            nres)
+        return out
+
+    def decode(self, limb0, slots, scale):
+        """Slot values from limb 0 (coefficient domain), the reference's ckks_decode."""
+        a = np.ascontiguousarray(limb0[:self.N], dtype=np.uint64)
+        v = np.zeros(2 * slots, dtype=np.float64)
+        self.lib.orc_decode(self.h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _p(a), slots, scale)
+        return v.view(np.complex128)
+
+    def sample(self, kind, seed, nonce, res0=0, nres=None):
+        """The product sampler's polynomial (limbs res0..res0+nres-1 of a QB-sized buffer)."""
+        nres = self.k + self.kl - res0 if nres is None else nres
+        out = np.zeros((self.k + self.kl) * self.N, dtype=np.uint64)
+        self.lib.orc_sample(self.h, _p(out), res0, nres, kind, seed, nonce)
+        return out
 
     def rescale(self, a, b, level):
         self.lib.orc_rescale(self.h, _p(a), _p(b), level)
