@@ -117,15 +117,79 @@ static int ilog2(int n) {
 
 static unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
 
+// Stages of length <= B inside aligned blocks of B values, in LDS: one
+// workgroup per block (B <= 4096, 64 KiB).  Forward: stages len = 2..B;
+// inverse: len = B..2.  With whole != 0 the block is the whole vector and the
+// bit reversal is folded into the load (forward) or, with the division by n,
+// into the store (inverse).
+constexpr int kFftThreads = 512;
+constexpr int kFftBlock = 4096;
+
+__global__ __launch_bounds__(kFftThreads) void k_enc_fft_block(DevParams P, cplx* v, int B, int logB, int inverse,
+                                                               int whole) {
+  extern __shared__ cplx buf[];
+  const int t = threadIdx.x;
+  cplx* blk = v + (size_t)blockIdx.x * B;
+  if (whole && !inverse) {
+    for (int i = t; i < B; i += kFftThreads) buf[logB ? (__brev((unsigned)i) >> (32 - logB)) : 0] = blk[i];
+  } else {
+    for (int i = t; i < B; i += kFftThreads) buf[i] = blk[i];
+  }
+  __syncthreads();
+  if (!inverse) {
+    for (int len = 2; len <= B; len <<= 1) {
+      const int lenh = len >> 1, lenq = len << 2;
+      const uint32_t step = (uint32_t)(2 * P.N / lenq);
+      for (int b = t; b < B / 2; b += kFftThreads) {
+        const int i = (b / lenh) * len, j = b % lenh;
+        const uint32_t tw = (P.rot_group[j] % (uint32_t)lenq) * step;
+        const cplx u = buf[i + j], w = cmul(buf[i + j + lenh], enc_root(P, tw));
+        buf[i + j] = cplx{u.re + w.re, u.im + w.im};
+        buf[i + j + lenh] = cplx{u.re - w.re, u.im - w.im};
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int len = B; len >= 2; len >>= 1) {
+      const int lenh = len >> 1, lenq = len << 2;
+      const uint32_t step = (uint32_t)(2 * P.N / lenq);
+      for (int b = t; b < B / 2; b += kFftThreads) {
+        const int i = (b / lenh) * len, j = b % lenh;
+        const uint32_t tw = ((uint32_t)lenq - P.rot_group[j] % (uint32_t)lenq) * step;
+        const cplx x = buf[i + j], y = buf[i + j + lenh];
+        buf[i + j] = cplx{x.re + y.re, x.im + y.im};
+        buf[i + j + lenh] = cmul(cplx{x.re - y.re, x.im - y.im}, enc_root(P, tw));
+      }
+      __syncthreads();
+    }
+  }
+  if (whole && inverse) {
+    const double dn = (double)B;
+    for (int i = t; i < B; i += kFftThreads)
+      blk[logB ? (__brev((unsigned)i) >> (32 - logB)) : 0] = cplx{buf[i].re / dn, buf[i].im / dn};
+  } else {
+    for (int i = t; i < B; i += kFftThreads) blk[i] = buf[i];
+  }
+}
+
+// The stages of length <= 4096 run in LDS (one launch); a vector longer than
+// 4096 adds one launch per remaining stage and a separate bit reversal.
 void launch_special_fft(const DevParams& P, double* vals, int n, int inverse, hipStream_t st) {
   cplx* v = reinterpret_cast<cplx*>(vals);
   const int logn = ilog2(n);
+  const int B = n < kFftBlock ? n : kFftBlock;
+  const int logB = ilog2(B);
+  const size_t lds = (size_t)B * sizeof(cplx);
+  const unsigned nblk = (unsigned)(n / B);
+  const int whole = n == B;
   if (!inverse) {
-    k_enc_bitrev<<<blocks(n), 256, 0, st>>>(v, n, logn, 0);
-    for (int len = 2; len <= n; len <<= 1) k_enc_stage_fwd<<<blocks(n / 2), 256, 0, st>>>(P, v, n, len);
+    if (!whole) k_enc_bitrev<<<blocks(n), 256, 0, st>>>(v, n, logn, 0);
+    k_enc_fft_block<<<nblk, kFftThreads, lds, st>>>(P, v, B, logB, 0, whole);
+    for (int len = 2 * B; len <= n; len <<= 1) k_enc_stage_fwd<<<blocks(n / 2), 256, 0, st>>>(P, v, n, len);
   } else {
-    for (int len = n; len >= 2; len >>= 1) k_enc_stage_inv<<<blocks(n / 2), 256, 0, st>>>(P, v, n, len);
-    k_enc_bitrev<<<blocks(n), 256, 0, st>>>(v, n, logn, 1);
+    for (int len = n; len > B; len >>= 1) k_enc_stage_inv<<<blocks(n / 2), 256, 0, st>>>(P, v, n, len);
+    k_enc_fft_block<<<nblk, kFftThreads, lds, st>>>(P, v, B, logB, 1, whole);
+    if (!whole) k_enc_bitrev<<<blocks(n), 256, 0, st>>>(v, n, logn, 1);
   }
 }
 

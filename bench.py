@@ -278,6 +278,72 @@ def serving_bench(args, rng):
             "resident_key_bytes": int(st["key_bytes"])}
 
 
+def c1_pipeline(args, stream):
+    """One plaintext product end to end at N=2^12, k=4 (prec 52), K=5 on the
+    device: encode -> encrypt -> HomMul+relin+rescale -> decrypt -> decode of
+    2048 slots (aoa_amd.pipeline), keys and randomness drawn on the device.
+    Plain (one C-ABI call per step, from Python) and replayed as one HIP graph;
+    beside it the oracle's restatement of the same sequence on the host cores,
+    whose decoded values must equal the device's bit for bit."""
+    import torch
+    from aoa_amd import Params
+    from aoa_amd.pipeline import Pipeline
+    Nc, kc, klc, precc, slots, seed = 4096, 4, 5, 52, 2048, 42
+    P = Params(Nc, kc, klc, precc)
+    pipe = Pipeline(P, seed, slots)
+    rng = np.random.default_rng(1)
+    z1 = rng.uniform(-1, 1, slots) + 1j * rng.uniform(-1, 1, slots)
+    z2 = rng.uniform(-1, 1, slots) + 1j * rng.uniform(-1, 1, slots)
+    pipe.set_inputs(z1, z2, 1 << precc)
+    nonce = 5
+    pipe.run(nonce)
+    eager = pipe.result()
+    plain_ms, _ = time_steps(lambda: pipe.run(nonce), max(20, args.steps), 3, stream)
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        pipe.run(nonce)
+    torch.cuda.current_stream().wait_stream(side)
+    with torch.cuda.graph(g, capture_error_mode="thread_local"):
+        pipe.run(nonce)
+    replays_ok = True
+    for _ in range(3):
+        pipe.out.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        replays_ok &= bool(np.array_equal(pipe.result().view(np.uint64), eager.view(np.uint64)))
+    graph_ms, _ = time_steps(g.replay, max(50, args.steps * 2), 5, stream)
+    res = {"workload": "encode -> encrypt -> HomMul+relin+rescale -> decrypt -> decode, 2048 slots, N=2^12, "
+                       "k=4 (prec 52), K=5, scale 2^52; keys, randomness and FFT on the device",
+           "plain_ms": round(plain_ms, 4), "graph_ms": round(graph_ms, 4),
+           "graph_pipelines_per_s": round(1000.0 / graph_ms, 1), "graph_replays_match_eager": replays_ok,
+           "max_err": float(np.max(np.abs(eager - z1 * z2))), "bound": 0.005}
+    if args.cpu_sample > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from orc import Oracle
+        O = Oracle(Nc, kc, klc, precc)
+        keys = O.c1_keys(seed)
+        threads = min(16, os.cpu_count() or 1)
+        ts = {}
+        for nt in (threads, 1):
+            Oracle.threads(nt)
+            ref, _ = O.c1_run(keys, z1, z2, 1 << precc, seed, nonce)
+            t = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                O.c1_run(keys, z1, z2, 1 << precc, seed, nonce)
+                t.append(time.perf_counter() - t0)
+            ts[nt] = float(np.median(t))
+        Oracle.threads(threads)
+        res["matches_oracle"] = bool(np.array_equal(ref.view(np.uint64), eager.view(np.uint64)))
+        res["cpu_ms"] = round(ts[threads] * 1e3, 3)
+        res["cpu_threads"] = threads
+        res["cpu_1t_ms"] = round(ts[1] * 1e3, 3)
+        res["cpu_sample"] = "oracle, same pipeline and inputs, median of 5 runs"
+    return res
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
@@ -452,6 +518,7 @@ def main():
     if not args.no_extra:
         res["mixed_levels_n15"] = mixed_n15(args, rng, stream)
         res["sumslots_n16"] = sumslots_n16(args, stream)
+        res["c1_pipeline_n12"] = c1_pipeline(args, stream)
 
     if not args.no_serving:
         res["serving"] = serving_bench(args, rng)

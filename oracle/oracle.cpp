@@ -807,6 +807,66 @@ static void sample(const Oracle& o, uint64_t* out, int res0, int nres, int kind,
   }
 }
 
+// ------------------------------------------------ C1 pipeline
+// One plaintext product end to end, on the sequence of the reference's
+// ckks_keygen / ckks_encrypt / ckks_mul / ckks_decrypt (ckkskeys.cpp:61-131,
+// ckks.cpp:87-105, :273-358, :107-132) with this build's sampler: keys, then
+// encode -> encrypt -> HomMul+relin+rescale -> decrypt -> decode.  Public
+// key on Q limbs: b = e - a s; relinearisation key on QB limbs:
+// b = rho(s^2) + e - a s; all in the NTT domain.  Nonces: s 0, pk a 1, pk e
+// 2, evk a 3, evk e 4; an encryption at nonce n draws u (ZO) at n, e0 at n+1,
+// e1 at n+2.
+struct C1Keys {
+  std::vector<uint64_t> s, pkb, pka, evkb, evka;
+};
+static void c1_keygen(const Oracle& o, C1Keys& K, uint64_t seed) {
+  const int k = o.k, R = o.k + o.kl;
+  const size_t N = o.N, qn = (size_t)k * N, qbn = (size_t)R * N;
+  K.s.assign(qbn, 0);
+  sample(o, K.s.data(), 0, R, S_HW, seed, 0);
+  transform(o, K.s.data(), 0, R, 0);
+  K.pka.assign(qn, 0);
+  sample(o, K.pka.data(), 0, k, S_UNIFORM, seed, 1);
+  std::vector<uint64_t> e(qbn), t(qbn);
+  sample(o, e.data(), 0, k, S_GAUSSIAN, seed, 2);
+  transform(o, e.data(), 0, k, 0);
+  vec_op(o, 2, t.data(), K.pka.data(), K.s.data(), k);
+  K.pkb.assign(qn, 0);
+  vec_op(o, 1, K.pkb.data(), e.data(), t.data(), k);
+  K.evka.assign(qbn, 0);
+  sample(o, K.evka.data(), 0, R, S_UNIFORM, seed, 3);
+  sample(o, e.data(), 0, R, S_GAUSSIAN, seed, 4);
+  transform(o, e.data(), 0, R, 0);
+  std::vector<uint64_t> s2(qbn, 0), rho(qbn, 0);
+  vec_op(o, 2, s2.data(), K.s.data(), K.s.data(), R);
+  for (int i = 0; i < k; ++i)  // rho: Q limbs times B mod q_i, special limbs 0 (manager.cu:149-176)
+    for (size_t c = 0; c < N; ++c) rho[i * N + c] = mulmod(o, s2[i * N + c], o.Bqi[i], i);
+  vec_op(o, 2, t.data(), K.evka.data(), K.s.data(), R);
+  K.evkb.assign(qbn, 0);
+  vec_op(o, 0, K.evkb.data(), rho.data(), e.data(), R);
+  vec_op(o, 1, K.evkb.data(), K.evkb.data(), t.data(), R);
+}
+static void c1_encrypt(const Oracle& o, const C1Keys& K, uint64_t* c0, uint64_t* c1, const std::complex<double>* z,
+                       int slots, uint64_t scale, uint64_t seed, uint64_t nonce) {
+  const int k = o.k;
+  const size_t qn = (size_t)k * o.N;
+  std::vector<uint64_t> m(qn), u(qn), e0(qn), e1(qn), t(qn);
+  std::vector<std::complex<double>> v(z, z + slots);
+  encode(o, m.data(), v.data(), slots, scale, k);
+  transform(o, m.data(), 0, k, 0);
+  sample(o, u.data(), 0, k, S_ZO, seed, nonce);
+  sample(o, e0.data(), 0, k, S_GAUSSIAN, seed, nonce + 1);
+  sample(o, e1.data(), 0, k, S_GAUSSIAN, seed, nonce + 2);
+  transform(o, u.data(), 0, k, 0);
+  transform(o, e0.data(), 0, k, 0);
+  transform(o, e1.data(), 0, k, 0);
+  vec_op(o, 2, t.data(), K.pkb.data(), u.data(), k);
+  vec_op(o, 0, c0, t.data(), e0.data(), k);
+  vec_op(o, 0, c0, c0, m.data(), k);
+  vec_op(o, 2, t.data(), K.pka.data(), u.data(), k);
+  vec_op(o, 0, c1, t.data(), e1.data(), k);
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -873,6 +933,50 @@ void orc_gauss_cdt(uint64_t* out) {
 }
 void orc_sample(void* h, uint64_t* out, int res0, int nres, int kind, uint64_t seed, uint64_t nonce) {
   sample(*(Oracle*)h, out, res0, nres, kind, seed, nonce);
+}
+void* orc_c1_keygen(void* h, uint64_t seed) {
+  C1Keys* K = new C1Keys;
+  c1_keygen(*(Oracle*)h, *K, seed);
+  return K;
+}
+void orc_c1_free(void* keys) { delete (C1Keys*)keys; }
+// Keys in the NTT domain: which 0 s (QB), 1 pk b, 2 pk a (Q), 3 evk b, 4 evk a (QB).
+int64_t orc_c1_key(void* keys, int which, uint64_t* out) {
+  const C1Keys& K = *(C1Keys*)keys;
+  const std::vector<uint64_t>* v[5] = {&K.s, &K.pkb, &K.pka, &K.evkb, &K.evka};
+  if (which < 0 || which > 4) return -1;
+  if (out) std::memcpy(out, v[which]->data(), v[which]->size() * 8);
+  return (int64_t)v[which]->size();
+}
+void orc_hommul(void* h, uint64_t* c30, uint64_t* c31, const uint64_t* c10, const uint64_t* c11,
+                const uint64_t* c20, const uint64_t* c21, int in_state, const uint64_t* evkb,
+                const uint64_t* evka, int level, int rescale);
+// z1, z2: slots complex values each (re, im interleaved); out: slots values;
+// c3 (optional, 2 x k limbs): the HomMul output in the coefficient domain.
+void orc_c1_run(void* h, void* keys, const double* z1, const double* z2, int slots, uint64_t scale, uint64_t seed,
+                uint64_t nonce, double* out, uint64_t* c3) {
+  const Oracle& o = *(Oracle*)h;
+  const C1Keys& K = *(C1Keys*)keys;
+  const int k = o.k, L = k - 1;
+  const size_t qn = (size_t)k * o.N;
+  std::vector<uint64_t> a0(qn), a1(qn), b0(qn), b1(qn), r0(qn), r1(qn), t(qn);
+  c1_encrypt(o, K, a0.data(), a1.data(), reinterpret_cast<const std::complex<double>*>(z1), slots, scale, seed, nonce);
+  c1_encrypt(o, K, b0.data(), b1.data(), reinterpret_cast<const std::complex<double>*>(z2), slots, scale, seed,
+             nonce + 3);
+  orc_hommul(h, r0.data(), r1.data(), a0.data(), a1.data(), b0.data(), b1.data(), TRANSSTATE, K.evkb.data(),
+             K.evka.data(), k - 1, 1);
+  if (c3) {
+    std::memcpy(c3, r0.data(), qn * 8);
+    std::memcpy(c3 + qn, r1.data(), qn * 8);
+  }
+  // decrypt on the L = k-1 remaining limbs: m = c0 + c1 s
+  transform(o, r0.data(), 0, L, 0);
+  transform(o, r1.data(), 0, L, 0);
+  vec_op(o, 2, t.data(), r1.data(), K.s.data(), L);
+  vec_op(o, 0, t.data(), t.data(), r0.data(), L);
+  transform(o, t.data(), 0, L, 1);
+  const uint64_t s2 = (uint64_t)((unsigned __int128)scale * scale / o.P[k - 1]);
+  decode(o, reinterpret_cast<std::complex<double>*>(out), t.data(), slots, s2);
 }
 int orc_roots(void* h, uint64_t* out) {
   Oracle* o = (Oracle*)h;

@@ -55,6 +55,12 @@ def load():
         "orc_decode": (None, [_vp, ctypes.POINTER(ctypes.c_double), _u64p, ctypes.c_int, ctypes.c_uint64]),
         "orc_philox4x32_10": (None, [ctypes.POINTER(ctypes.c_uint32)] * 3),
         "orc_gauss_cdt": (None, [_u64p]),
+        "orc_c1_keygen": (_vp, [_vp, ctypes.c_uint64]),
+        "orc_c1_free": (None, [_vp]),
+        "orc_c1_key": (ctypes.c_int64, [_vp, ctypes.c_int, _u64p]),
+        "orc_c1_run": (None, [_vp, _vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                              ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                              ctypes.POINTER(ctypes.c_double), _u64p]),
         "orc_sample": (None, [_vp, _u64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                               ctypes.c_uint64]),
     }
@@ -159,6 +165,30 @@ class Oracle:
         out = np.zeros((self.k + self.kl) * self.N, dtype=np.uint64)
         self.lib.orc_sample(self.h, _p(out), res0, nres, kind, seed, nonce)
         return out
+
+    def c1_keys(self, seed):
+        """Keys of the end-to-end pipeline (s on QB limbs, pk on Q, evk on QB; NTT domain)."""
+        kh = self.lib.orc_c1_keygen(self.h, seed)
+        keys = {}
+        for i, name in enumerate(("s", "pkb", "pka", "evkb", "evka")):
+            n = self.lib.orc_c1_key(kh, i, None)
+            a = np.zeros(n, dtype=np.uint64)
+            self.lib.orc_c1_key(kh, i, _p(a))
+            keys[name] = a
+        keys["_h"] = kh
+        return keys
+
+    def c1_run(self, keys, z1, z2, scale, seed, nonce):
+        """encode -> encrypt -> HomMul+relin+rescale -> decrypt -> decode; returns (slots, c3)."""
+        slots = len(z1)
+        P = ctypes.POINTER(ctypes.c_double)
+        a = np.ascontiguousarray(z1, dtype=np.complex128).view(np.float64).copy()
+        b = np.ascontiguousarray(z2, dtype=np.complex128).view(np.float64).copy()
+        out = np.zeros(2 * slots, dtype=np.float64)
+        c3 = np.zeros(2 * self.k * self.N, dtype=np.uint64)
+        self.lib.orc_c1_run(self.h, keys["_h"], a.ctypes.data_as(P), b.ctypes.data_as(P), slots, scale, seed, nonce,
+                            out.ctypes.data_as(P), _p(c3))
+        return out.view(np.complex128), c3
 
     def rescale(self, a, b, level):
         self.lib.orc_rescale(self.h, _p(a), _p(b), level)

@@ -83,3 +83,48 @@ def test_rotation_decrypts():
         else:
             exp[e] = v
     assert max(abs(int(a) - int(b)) for a, b in zip(dec, exp)) < 1 << 12
+
+
+def test_c1_pipeline_decrypts():
+    """The oracle's end-to-end sequence (keys from the sampler restatement,
+    encode, public-key encryption, HomMul+relin+rescale, decryption, decode)
+    gives the FP64 product within the reference's bound (tests.cpp:34) for two
+    encryption nonces; the keys satisfy their defining relations: pk.b + pk.a s
+    and evk.b + evk.a s - rho(s^2) are small errors (|e| <= 32)."""
+    import numpy as np
+    from orc import Oracle
+    o = Oracle(4096, 4, 5, 52)
+    keys = o.c1_keys(42)
+    rng = np.random.default_rng(1)
+    z1 = rng.uniform(-1, 1, 2048) + 1j * rng.uniform(-1, 1, 2048)
+    z2 = rng.uniform(-1, 1, 2048) + 1j * rng.uniform(-1, 1, 2048)
+    for nonce in (5, 100):
+        out, _ = o.c1_run(keys, z1, z2, 1 << 52, 42, nonce)
+        assert np.max(np.abs(out - z1 * z2)) < 0.005
+    N, k, R = o.N, o.k, o.k + o.kl
+
+    def small(x, nres):
+        y = x.copy()
+        o.ntt(y, 0, nres, 1)
+        for r in range(nres):
+            p = o.primes[r]
+            v = y[r * N:(r + 1) * N].astype(object)
+            c = np.array([int(a) - p if int(a) > p // 2 else int(a) for a in v])
+            assert np.abs(c).max() <= 32, r
+        return y
+
+    t = np.zeros(R * N, dtype=np.uint64)
+    o.eltwise(2, t, keys["pka"], keys["s"], k)
+    u = np.zeros(R * N, dtype=np.uint64)
+    o.eltwise(0, u, keys["pkb"], t, k)
+    small(u, k)
+    s2 = np.zeros(R * N, dtype=np.uint64)
+    o.eltwise(2, s2, keys["s"], keys["s"], R)
+    bq = o.table(7)
+    rho = np.zeros(R * N, dtype=np.uint64)
+    for r in range(k):
+        rho[r * N:(r + 1) * N] = [(int(a) * int(bq[r])) % o.primes[r] for a in s2[r * N:(r + 1) * N]]
+    o.eltwise(2, t, keys["evka"], keys["s"], R)
+    o.eltwise(0, u, keys["evkb"], t, R)
+    o.eltwise(1, u, u, rho, R)
+    small(u, R)
