@@ -49,6 +49,8 @@ struct BatchGroup {
   int n_keycol = 0;
   ModDownDesc* d_moddown = nullptr;
   int n_moddown = 0;
+  // leading descriptors whose prime is 2^61 or above (-1: not a prefix)
+  int ns_key2 = -1;
 };
 
 struct BatchLayer {
@@ -249,6 +251,20 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
       by_prime(fwd);
       by_prime(key2);
       by_prime(keycol);
+      // The butterflies of primes of 2^61 and above (the 63-bit q0) are not
+      // lazy; they sort first, and the key stage's first pass over the other
+      // limbs is launched in its lazy-only form (98 instead of 160 registers,
+      // four waves per SIMD instead of three).  The other stages gain nothing
+      // from the split (their occupancy is bound by LDS) and keep one launch.
+      auto strict_prefix = [&](const auto& v) -> int {
+        const std::vector<uint64_t>& pr = g.params->host.primes;
+        int n = 0;
+        while (n < (int)v.size() && pr[v[n].prime] >= (1ull << 61)) ++n;
+        for (size_t t = n; t < v.size(); ++t)
+          if (pr[v[t].prime] >= (1ull << 61)) return -1;
+        return n;
+      };
+      g.ns_key2 = strict_prefix(key2);
       g.n_pre_fwd = (int)prefwd.size();
       g.n_pre_inv = (int)preinv.size();
       g.n_tensor = (int)tens.size();
@@ -334,7 +350,7 @@ int batch_enqueue(aoa_batch* b, hipStream_t st, StageTimer* tm) {
     run(AOA_STAGE_NTT, [](BatchGroup& g) { return g.n_fwd > 0; },
         [&](BatchGroup& g) { return ntt_forward_desc(g.params->dev, g.d_fwd, g.n_fwd, 0, st); });
     run(AOA_STAGE_KEY_INTT, [](BatchGroup& g) { return g.n_key2 > 0; }, [&](BatchGroup& g) {
-      return ntt_inverse_key2_desc(g.params->dev, g.d_key2, g.n_key2, g.d_keycol, g.n_keycol, 0, st);
+      return ntt_inverse_key2_desc(g.params->dev, g.d_key2, g.n_key2, g.d_keycol, g.n_keycol, 0, st, g.ns_key2);
     });
     run(AOA_STAGE_MODDOWN, [](BatchGroup& g) { return g.n_moddown > 0; }, [&](BatchGroup& g) {
       launch_moddown_desc(g.params->dev, g.d_moddown, g.n_moddown, st);

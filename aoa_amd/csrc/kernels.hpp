@@ -81,12 +81,15 @@ void launch_rho(const DevParams& P, uint64_t* out, const uint64_t* in, hipStream
 
 // Descriptor-driven (stage-fused) forms used by batches: one launch covers
 // every descriptor of the table (device memory).
-int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st);
-int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st);
+// nstrict: the first nstrict descriptors use primes of 2^61 and above and
+// the rest do not (a prime-major list); -1 when not known.
+int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st, int nstrict = -1);
+int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st, int nstrict = -1);
 // Key-switching inverse NTT: first pass from KeyDesc2 entries (one input limb,
 // two keys, two outputs), second pass over the NttDesc list of outputs.
 int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const NttDesc* dcol, int ncol, int N1,
-                          hipStream_t st);
+                          hipStream_t st,
+                          int n2strict = -1, int ncolstrict = -1);
 void launch_tensor_desc(const DevParams& P, const TensorDesc* d, int ndesc, hipStream_t st);
 void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int max_level, hipStream_t st);
 void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, hipStream_t st);

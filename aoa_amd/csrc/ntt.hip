@@ -410,40 +410,49 @@ __global__ __launch_bounds__(256) void k_ntt_inv_col(DevParams P, LimbMap m, int
 
 // ----------------------------------------------- descriptor-driven kernels --
 
-template <int S>
+// MODE: 0 decides lazy or strict butterflies per descriptor; 1 (every
+// descriptor's prime is below 2^61) and 2 (none is) compile one form only,
+// which keeps the register count of the lazy form (launchers split a
+// prime-major descriptor list at the first lazy prime).
+template <int MODE>
+__device__ __forceinline__ bool use_lazy(const DevParams& P, int pr) {
+  return MODE == 1 ? true : (MODE == 2 ? false : lazy_prime(P, pr));
+}
+
+template <int S, int MODE>
 __global__ __launch_bounds__(256) void k_ntt_fwd_col_d(DevParams P, const NttDesc* d, int N2) {
   __shared__ uint64_t lds[Geo<S>::COL_LDS];
   const int tiles = N2 / Geo<S>::C;
   const NttDesc e = d[blockIdx.x / tiles];
-  if (lazy_prime(P, e.prime))
+  if (use_lazy<MODE>(P, e.prime))
     fwd_col_body<S, true>(P, e.src, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
-  else
+  else if (MODE != 1)
     fwd_col_body<S, false>(P, e.src, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
 }
 
-template <int S>
+template <int S, int MODE>
 __global__ __launch_bounds__(256) void k_ntt_fwd_row_d(DevParams P, const NttDesc* d, int g0) {
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
   const int tiles = (P.N / S) / Geo<S>::C;
   const NttDesc e = d[blockIdx.x / tiles];
-  if (lazy_prime(P, e.prime))
+  if (use_lazy<MODE>(P, e.prime))
     fwd_row_body<S, true>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, g0, lds);
-  else
+  else if (MODE != 1)
     fwd_row_body<S, false>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, g0, lds);
 }
 
-template <int S>
+template <int S, int MODE>
 __global__ __launch_bounds__(256) void k_ntt_inv_row_d(DevParams P, const NttDesc* d, int g0) {
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
   const int tiles = (P.N / S) / Geo<S>::C;
   const NttDesc e = d[blockIdx.x / tiles];
-  if (lazy_prime(P, e.prime))
+  if (use_lazy<MODE>(P, e.prime))
     inv_row_body<S, true>(P, e.src, e.dst, e.key, e.prime, blockIdx.x % tiles, g0, lds);
-  else
+  else if (MODE != 1)
     inv_row_body<S, false>(P, e.src, e.dst, e.key, e.prime, blockIdx.x % tiles, g0, lds);
 }
 
-template <int S>
+template <int S, int MODE>
 __global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const KeyDesc2* d, int g0) {
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
   const int tiles = (P.N / S) / Geo<S>::C;
@@ -452,20 +461,20 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const K
   uint64_t *d0 = e.dst[0], *d1 = e.dst[1];
   const uint64_t *k0 = e.key[0], *k1 = e.key[1];
   const int pr = e.prime;
-  if (lazy_prime(P, pr))
+  if (use_lazy<MODE>(P, pr))
     inv_row_key2_body<S, true>(P, src, d0, d1, k0, k1, pr, blockIdx.x % tiles, g0, lds);
-  else
+  else if (MODE != 1)
     inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, pr, blockIdx.x % tiles, g0, lds);
 }
 
-template <int S>
+template <int S, int MODE>
 __global__ __launch_bounds__(256) void k_ntt_inv_col_d(DevParams P, const NttDesc* d, int N2) {
   __shared__ uint64_t lds[Geo<S>::COL_LDS];
   const int tiles = N2 / Geo<S>::C;
   const NttDesc e = d[blockIdx.x / tiles];
-  if (lazy_prime(P, e.prime))
+  if (use_lazy<MODE>(P, e.prime))
     inv_col_body<S, true>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
-  else
+  else if (MODE != 1)
     inv_col_body<S, false>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
 }
 
@@ -784,7 +793,21 @@ int ntt_inverse(const DevParams& P, LimbMap m, int N1, const uint64_t* key0, con
   return 0;
 }
 
-int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st) {
+// Launches kernel K<S, MODE> over `n` descriptors of which the first
+// `strict` use primes of 2^61 and above (strict < 0: not known, one launch
+// deciding per descriptor).
+#define AOA_SPLIT_LAUNCH(K, S, GRIDPER, DESC, N_, STRICT, ...)                                          \
+  do {                                                                                                  \
+    if ((STRICT) < 0) {                                                                                 \
+      K<S, 0><<<dim3((GRIDPER) * (N_)), 256, 0, st>>>(P, DESC, __VA_ARGS__);                           \
+    } else {                                                                                            \
+      if ((STRICT) > 0) K<S, 2><<<dim3((GRIDPER) * (STRICT)), 256, 0, st>>>(P, DESC, __VA_ARGS__);     \
+      if ((N_) > (STRICT))                                                                              \
+        K<S, 1><<<dim3((GRIDPER) * ((N_) - (STRICT))), 256, 0, st>>>(P, (DESC) + (STRICT), __VA_ARGS__); \
+    }                                                                                                   \
+  } while (0)
+
+int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st, int nstrict) {
   N1 = pick_n1(P, N1);
   if (!N1 || ndesc <= 0) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
@@ -796,20 +819,20 @@ int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
   }
   {
     const int tiles = N2 / (4096 / N1);
-#define CALLC(S) k_ntt_fwd_col_d<S><<<dim3(tiles * ndesc), 256, 0, st>>>(P, d, N2)
+#define CALLC(S) AOA_SPLIT_LAUNCH(k_ntt_fwd_col_d, S, tiles, d, ndesc, nstrict, N2)
     AOA_DISPATCH_S(N1, CALLC)
 #undef CALLC
   }
   {
     const int tiles = N1 / (4096 / N2);
-#define CALLR(S) k_ntt_fwd_row_d<S><<<dim3(tiles * ndesc), 256, 0, st>>>(P, d, g0)
+#define CALLR(S) AOA_SPLIT_LAUNCH(k_ntt_fwd_row_d, S, tiles, d, ndesc, nstrict, g0)
     AOA_DISPATCH_S(N2, CALLR)
 #undef CALLR
   }
   return 0;
 }
 
-int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st) {
+int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st, int nstrict) {
   N1 = pick_n1(P, N1);
   if (!N1 || ndesc <= 0) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
@@ -821,13 +844,13 @@ int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
   }
   {
     const int tiles = N1 / (4096 / N2);
-#define CALLR(S) k_ntt_inv_row_d<S><<<dim3(tiles * ndesc), 256, 0, st>>>(P, d, g0)
+#define CALLR(S) AOA_SPLIT_LAUNCH(k_ntt_inv_row_d, S, tiles, d, ndesc, nstrict, g0)
     AOA_DISPATCH_S(N2, CALLR)
 #undef CALLR
   }
   {
     const int tiles = N2 / (4096 / N1);
-#define CALLC(S) k_ntt_inv_col_d<S><<<dim3(tiles * ndesc), 256, 0, st>>>(P, d, N2)
+#define CALLC(S) AOA_SPLIT_LAUNCH(k_ntt_inv_col_d, S, tiles, d, ndesc, nstrict, N2)
     AOA_DISPATCH_S(N1, CALLC)
 #undef CALLC
   }
@@ -835,7 +858,7 @@ int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
 }
 
 int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const NttDesc* dcol, int ncol, int N1,
-                          hipStream_t st) {
+                          hipStream_t st, int n2strict, int ncolstrict) {
   N1 = pick_n1(P, N1);
   if (!N1 || n2 <= 0 || ncol <= 0) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
@@ -847,13 +870,13 @@ int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const 
   }
   {
     const int tiles = N1 / (4096 / N2);
-#define CALLR(S) k_ntt_inv_row_key2_d<S><<<dim3(tiles * n2), 256, 0, st>>>(P, d2, g0)
+#define CALLR(S) AOA_SPLIT_LAUNCH(k_ntt_inv_row_key2_d, S, tiles, d2, n2, n2strict, g0)
     AOA_DISPATCH_S(N2, CALLR)
 #undef CALLR
   }
   {
     const int tiles = N2 / (4096 / N1);
-#define CALLC(S) k_ntt_inv_col_d<S><<<dim3(tiles * ncol), 256, 0, st>>>(P, dcol, N2)
+#define CALLC(S) AOA_SPLIT_LAUNCH(k_ntt_inv_col_d, S, tiles, dcol, ncol, ncolstrict, N2)
     AOA_DISPATCH_S(N1, CALLC)
 #undef CALLC
   }
