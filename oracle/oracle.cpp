@@ -18,7 +18,12 @@
 // FP64 round trips and by the canonical-embedding definition; encode/decode
 // by the embedding (the residues, read as a polynomial over the integers,
 // evaluate to scale * slot values at exp(i pi 5^j / N)), by agreement of the
-// signed value across limbs and by decode o encode = id.
+// signed value across limbs and by decode o encode = id.  The sampler's
+// restatement (Philox4x32-10 and the mapping of its output to each
+// distribution) is pinned by the generator's published known-answer vectors,
+// the Gaussian table against an independent normal CDF and the frequencies and
+// moments of every distribution; the end-to-end C1 sequence by decryption
+// within the reference's bound and by the defining relations of its keys.
 #include <omp.h>
 #include <stdint.h>
 

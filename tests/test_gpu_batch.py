@@ -167,3 +167,64 @@ def test_stage_times_runs_the_batch(gpu):
         lv = it[3]
         assert np.array_equal(P.to_host(it[2][0])[:lv * P.N], r[0])
         assert np.array_equal(P.to_host(it[2][1])[:lv * P.N], r[1])
+
+
+def test_batch_c3_full_size_equals_plain(gpu):
+    """The batched workload at its full size: 256 HomMul+relin+rescale at
+    N=2^15, k=20, levels uniform in 1..19, one graph; every output equals the
+    plain call's bit for bit (compared on the device)."""
+    import torch
+    from aoa_amd import Batch, Params
+    P = Params(32768, 20, 0, 48)
+    rng = np.random.default_rng(34)
+    evk = [P.to_device(_rand(P, rng, P.k + P.kl)) for _ in range(2)]
+    levels = [int(x) for x in rng.integers(1, 20, size=256)]
+    print("c3 full size: 256 items", flush=True)
+    items = _mul_items(P, rng, levels)
+    B = Batch()
+    B.add_mul(P, evk, items)
+    B.build()
+    B.launch(graph=True)
+    torch.cuda.synchronize()
+    ws = {}
+    for it in items:
+        c1, c2, out, lv = it
+        ref = [P.zeros(P.k), P.zeros(P.k)]
+        if lv not in ws:
+            ws[lv] = P.workspace(lv)
+        P.hommul(ref, c1, c2, evk, lv, ws[lv])
+        n = lv * P.N
+        assert torch.equal(out[0][:n], ref[0][:n]) and torch.equal(out[1][:n], ref[1][:n]), lv
+
+
+def test_batch_c4_full_size_equals_plain(gpu):
+    """Rotate-and-sum at its full size: 64 ciphertexts at N=2^16, k=24, 15
+    layers of rotate-accumulate (steps 2^0..2^14) with resident rotation keys;
+    every result equals the plain rotations plus additions bit for bit."""
+    import torch
+    from aoa_amd import Batch, Params
+    P = Params(65536, 24, 0, 48)
+    rng = np.random.default_rng(35)
+    level = P.k - 1
+    steps = [1 << i for i in range(15)]
+    print("c4 full size: 64 ciphertexts x 15 rotations", flush=True)
+    rtk = {r: [P.to_device(_rand(P, rng, P.k + P.kl)) for _ in range(2)] for r in steps}
+    cts = [[P.to_device(_rand(P, rng, P.k)) for _ in range(2)] for _ in range(64)]
+    work = [[t.clone() for t in c] for c in cts]
+    B = Batch()
+    for i, r in enumerate(steps):
+        if i:
+            B.barrier()
+        B.add_rotate(P, [(w, w, rtk[r], r, level, 1) for w in work])
+    B.launch(graph=True)
+    torch.cuda.synchronize()
+    ws = P.workspace(level)
+    n = (level + 1) * P.N
+    rot = [P.zeros(P.k), P.zeros(P.k)]
+    for c, w in zip(cts, work):
+        cur = [t.clone() for t in c]
+        for r in steps:
+            P.rotate_left(rot, cur, rtk[r], r, level, ws)
+            for y in range(2):
+                P.eltwise(0, cur[y], cur[y], rot[y], res0=0, nres=level + 1)
+        assert torch.equal(w[0][:n], cur[0][:n]) and torch.equal(w[1][:n], cur[1][:n])
