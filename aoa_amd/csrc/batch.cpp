@@ -38,6 +38,10 @@ struct BatchGroup {
   int n_auto = 0;
   NttDesc* d_intt3 = nullptr;  // mul: INTT of d0, d1, d2
   int n_intt3 = 0;
+  TensorRowDesc* d_trow = nullptr;  // mul (N >= 8192): tensor fused into the first inverse pass
+  int n_trow = 0, ns_trow = -1;
+  NttDesc* d_tcol = nullptr;  // ... and the column pass of d0, d1, d2
+  int n_tcol = 0;
   ModUpDesc* d_modup = nullptr;
   int n_modup = 0;
   int max_level = 0;
@@ -126,7 +130,8 @@ void batch_sizes(const aoa_batch* b, size_t* ws_bytes, size_t* desc_bytes) {
       for (auto& it : g.muls) {
         const size_t Lq = it.level + 1;
         s += mul_ws_limbs((int)Lq, (int)K) * N * 8;
-        desc += Lq * (sizeof(TensorDesc) + 3 * sizeof(NttDesc)) + sizeof(ModUpDesc) + K * sizeof(NttDesc) +
+        desc += Lq * (std::max(sizeof(TensorDesc), sizeof(TensorRowDesc)) + 3 * sizeof(NttDesc)) + sizeof(ModUpDesc) +
+                K * sizeof(NttDesc) +
                 (Lq + K) * sizeof(KeyDesc2) + 2 * (Lq + K) * sizeof(NttDesc) + 2 * sizeof(ModDownDesc);
       }
       for (auto& it : g.rots) {
@@ -167,7 +172,8 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
       const size_t N = (size_t)P.N;
       const int k = P.k, K = P.K;
       std::vector<TensorDesc> tens;
-      std::vector<NttDesc> prefwd, preinv, intt3, fwd, keycol;
+      std::vector<NttDesc> prefwd, preinv, intt3, fwd, keycol, tcol;
+      std::vector<TensorRowDesc> trow;
       std::vector<KeyDesc2> key2;
       std::vector<ModUpDesc> mu;
       std::vector<ModDownDesc> md;
@@ -189,7 +195,27 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
         uint64_t* P0 = D2 + (size_t)Lq * N;
         uint64_t* P1 = P0 + (size_t)(Lq + K) * N;
         w += mul_ws_limbs(Lq, K) * N;
-        for (int i = 0; i < Lq; ++i) {
+        // From N = 8192 up the transforms are two-pass and the tensor product
+        // runs inside the first inverse pass; below, a tensor kernel and the
+        // single-workgroup transforms.
+        const bool fused = P.N >= 8192;
+        for (int i = 0; i < Lq && fused; ++i) {
+          const size_t o = (size_t)i * N;
+          TensorRowDesc t;
+          t.a0 = it.c1[0] + o;
+          t.a1 = it.c1[1] + o;
+          t.b0 = it.c2[0] + o;
+          t.b1 = it.c2[1] + o;
+          t.x = X + o;
+          t.dst[0] = D0 + o;
+          t.dst[1] = D1 + o;
+          t.dst[2] = D2 + o;
+          t.prime = i;
+          t.pad = 0;
+          trow.push_back(t);
+          for (uint64_t* d : {D0 + o, D1 + o, D2 + o}) tcol.push_back({d, d, nullptr, i, 0});
+        }
+        for (int i = 0; i < Lq && !fused; ++i) {
           const size_t o = (size_t)i * N;
           TensorDesc t;
           t.a0 = it.c1[0] + o;
@@ -251,6 +277,8 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
       by_prime(fwd);
       by_prime(key2);
       by_prime(keycol);
+      by_prime(trow);
+      by_prime(tcol);
       // The butterflies of primes of 2^61 and above (the 63-bit q0) are not
       // lazy; they sort first, and the key stage's first pass over the other
       // limbs is launched in its lazy-only form (98 instead of 160 registers,
@@ -265,6 +293,11 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
         return n;
       };
       g.ns_key2 = strict_prefix(key2);
+      g.ns_trow = strict_prefix(trow);
+      g.n_trow = (int)trow.size();
+      g.n_tcol = (int)tcol.size();
+      g.d_trow = (TensorRowDesc*)(trow.empty() ? nullptr : put(trow.data(), trow.size() * sizeof(TensorRowDesc)));
+      g.d_tcol = (NttDesc*)(tcol.empty() ? nullptr : put(tcol.data(), tcol.size() * sizeof(NttDesc)));
       g.n_pre_fwd = (int)prefwd.size();
       g.n_pre_inv = (int)preinv.size();
       g.n_tensor = (int)tens.size();
@@ -341,8 +374,12 @@ int batch_enqueue(aoa_batch* b, hipStream_t st, StageTimer* tm) {
       launch_automorphism_desc(g.params->dev, g.d_auto, g.n_auto, st);
       return 0;
     });
-    run(AOA_STAGE_INTT, [](BatchGroup& g) { return g.n_intt3 > 0; },
-        [&](BatchGroup& g) { return ntt_inverse_desc(g.params->dev, g.d_intt3, g.n_intt3, 0, st); });
+    run(AOA_STAGE_INTT, [](BatchGroup& g) { return g.n_intt3 > 0 || g.n_trow > 0; }, [&](BatchGroup& g) {
+      if (g.n_trow > 0 &&
+          ntt_inverse_tensor_desc(g.params->dev, g.d_trow, g.n_trow, g.d_tcol, g.n_tcol, 0, st, g.ns_trow))
+        return -1;
+      return g.n_intt3 > 0 ? ntt_inverse_desc(g.params->dev, g.d_intt3, g.n_intt3, 0, st) : 0;
+    });
     run(AOA_STAGE_MODUP, [](BatchGroup& g) { return g.n_modup > 0; }, [&](BatchGroup& g) {
       launch_modup_desc(g.params->dev, g.d_modup, g.n_modup, g.max_level, st);
       return 0;

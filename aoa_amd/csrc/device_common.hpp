@@ -68,6 +68,21 @@ struct NttDesc {
   int pad;
 };
 
+// HomMul's tensor product fused into the first inverse pass, per Q limb:
+// d0 = a0 b0, d1 = a0 b1 + a1 b0, d2 = a1 b1 (NTT domain); d2 is also written
+// to x (canonical, NTT domain: the key product's input); dst[y] receives the
+// first inverse pass of d_y (the column pass then runs in place on dst[y]).
+struct TensorRowDesc {
+  const uint64_t* a0;
+  const uint64_t* a1;
+  const uint64_t* b0;
+  const uint64_t* b1;
+  uint64_t* x;
+  uint64_t* dst[3];
+  int prime;
+  int pad;
+};
+
 // Key switching's inverse first pass with both key halves: dst[y] = first
 // inverse pass of (src * key[y]) for y = 0, 1 (the column pass then runs on
 // dst[0] and dst[1] as ordinary NttDesc entries).

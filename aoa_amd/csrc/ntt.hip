@@ -289,6 +289,48 @@ __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint
   }
 }
 
+// HomMul's tensor product and the first inverse pass of one of its three
+// outputs (y): three workgroups per tile, adjacent in launch order, so the
+// operand tiles they share are read once from HBM and again from the L2
+// cache.  d1 is formed as a0 b1 + a1 b0, congruent to the reference's
+// (a0+a1)(b0+b1) - d0 - d2.
+template <int S, bool LZ>
+__device__ __forceinline__ void inv_row_tensor_body(const DevParams& P, const uint64_t* a0, const uint64_t* a1,
+                                                    const uint64_t* b0, const uint64_t* b1, uint64_t* xd,
+                                                    uint64_t* dst, int y, int pr, int tile, int g0, uint64_t* lds) {
+  using G = Geo<S>;
+  constexpr int C = G::C, TS = G::TS, PS = S + S / 16;
+  const DevPrime dp = P.primes[pr];
+  const uint64_t p = dp.p;
+  const uint64_t* T = P.tw_inv + (size_t)pr * P.N * 2;
+  const int rl = threadIdx.x / TS, sl = threadIdx.x % TS;
+  const int row = tile * C + rl;
+  const size_t off = (size_t)row * S;
+  uint64_t x[16];
+  if (y == 0) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(a0[off + sl + TS * k], b0[off + sl + TS * k], dp);
+  } else if (y == 1) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const size_t i = off + sl + TS * k;
+      const uint64_t t = key_mul<LZ>(a0[i], b1[i], dp) + key_mul<LZ>(a1[i], b0[i], dp);
+      // lazy: [0, 4p) -> [0, 2p); strict: canonical sum
+      x[k] = LZ ? (t >= 2 * p ? t - 2 * p : t) : (t >= p ? t - p : t);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const size_t i = off + sl + TS * k;
+      uint64_t c = key_mul<LZ>(a1[i], b1[i], dp);
+      if (LZ) c = c >= p ? c - p : c;
+      xd[i] = c;
+      x[k] = c;
+    }
+  }
+  inv_row_core<S, LZ>(P, x, dst + off, p, T, row, sl, g0, lds + rl * PS);
+}
+
 // Column pass of the inverse: global stages log2(S)-1 .. 0, N^-1 folded into
 // stage 0.
 template <int S, bool LZ>
@@ -465,6 +507,22 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const K
     inv_row_key2_body<S, true>(P, src, d0, d1, k0, k1, pr, blockIdx.x % tiles, g0, lds);
   else if (MODE != 1)
     inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, pr, blockIdx.x % tiles, g0, lds);
+}
+
+template <int S, int MODE>
+__global__ __launch_bounds__(256) void k_ntt_inv_row_tensor_d(DevParams P, const TensorRowDesc* d, int g0) {
+  __shared__ uint64_t lds[Geo<S>::ROW_LDS];
+  const int tiles = (P.N / S) / Geo<S>::C;
+  const int y = blockIdx.x % 3, b = blockIdx.x / 3;
+  const TensorRowDesc& e = d[b / tiles];
+  const uint64_t *a0 = e.a0, *a1 = e.a1, *b0 = e.b0, *b1 = e.b1;
+  uint64_t* xd = e.x;
+  uint64_t* dst = y == 0 ? e.dst[0] : (y == 1 ? e.dst[1] : e.dst[2]);
+  const int pr = e.prime;
+  if (use_lazy<MODE>(P, pr))
+    inv_row_tensor_body<S, true>(P, a0, a1, b0, b1, xd, dst, y, pr, b % tiles, g0, lds);
+  else if (MODE != 1)
+    inv_row_tensor_body<S, false>(P, a0, a1, b0, b1, xd, dst, y, pr, b % tiles, g0, lds);
 }
 
 template <int S, int MODE>
@@ -877,6 +935,26 @@ int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const 
   {
     const int tiles = N2 / (4096 / N1);
 #define CALLC(S) AOA_SPLIT_LAUNCH(k_ntt_inv_col_d, S, tiles, dcol, ncol, ncolstrict, N2)
+    AOA_DISPATCH_S(N1, CALLC)
+#undef CALLC
+  }
+  return 0;
+}
+
+int ntt_inverse_tensor_desc(const DevParams& P, const TensorRowDesc* dt, int nt, const NttDesc* dcol, int ncol,
+                            int N1, hipStream_t st, int ntstrict) {
+  N1 = pick_n1(P, N1);
+  if (!N1 || N1 == P.N || nt <= 0 || ncol <= 0) return -1;  // two-pass transforms only
+  const int N2 = P.N / N1, g0 = log2i(N1);
+  {
+    const int tiles = N1 / (4096 / N2);
+#define CALLR(S) AOA_SPLIT_LAUNCH(k_ntt_inv_row_tensor_d, S, 3 * tiles, dt, nt, ntstrict, g0)
+    AOA_DISPATCH_S(N2, CALLR)
+#undef CALLR
+  }
+  {
+    const int tiles = N2 / (4096 / N1);
+#define CALLC(S) AOA_SPLIT_LAUNCH(k_ntt_inv_col_d, S, tiles, dcol, ncol, -1, N2)
     AOA_DISPATCH_S(N1, CALLC)
 #undef CALLC
   }
