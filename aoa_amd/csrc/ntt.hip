@@ -509,11 +509,20 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const K
     inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, pr, blockIdx.x % tiles, g0, lds);
 }
 
+// Workgroups are dispatched round-robin over the chip's 8 XCDs (workgroup g
+// runs on XCD g % 8), each with its own L2.  The three workgroups of a tile
+// take ids g, g+8 and g+16, so they share an XCD and the operand tiles they
+// all read stay in one L2.  `ntiles` = descriptors x tiles per limb; the grid
+// is rounded up to a multiple of 24.
 template <int S, int MODE>
-__global__ __launch_bounds__(256) void k_ntt_inv_row_tensor_d(DevParams P, const TensorRowDesc* d, int g0) {
+__global__ __launch_bounds__(256) void k_ntt_inv_row_tensor_d(DevParams P, const TensorRowDesc* d, int g0,
+                                                              int ntiles) {
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
   const int tiles = (P.N / S) / Geo<S>::C;
-  const int y = blockIdx.x % 3, b = blockIdx.x / 3;
+  const int slot = blockIdx.x >> 3;
+  const int b = (slot / 3) * 8 + (blockIdx.x & 7);
+  const int y = slot % 3;
+  if (b >= ntiles) return;
   const TensorRowDesc& e = d[b / tiles];
   const uint64_t *a0 = e.a0, *a1 = e.a1, *b0 = e.b0, *b1 = e.b1;
   uint64_t* xd = e.x;
@@ -948,7 +957,16 @@ int ntt_inverse_tensor_desc(const DevParams& P, const TensorRowDesc* dt, int nt,
   const int N2 = P.N / N1, g0 = log2i(N1);
   {
     const int tiles = N1 / (4096 / N2);
-#define CALLR(S) AOA_SPLIT_LAUNCH(k_ntt_inv_row_tensor_d, S, 3 * tiles, dt, nt, ntstrict, g0)
+    auto grid = [&](int n) { return dim3((unsigned)((3 * tiles * n + 23) / 24 * 24)); };
+    const int nlazy = ntstrict < 0 ? 0 : nt - ntstrict;
+#define CALLR(S)                                                                                          \
+  if (ntstrict < 0) {                                                                                     \
+    k_ntt_inv_row_tensor_d<S, 0><<<grid(nt), 256, 0, st>>>(P, dt, g0, tiles * nt);                       \
+  } else {                                                                                                \
+    if (ntstrict > 0) k_ntt_inv_row_tensor_d<S, 2><<<grid(ntstrict), 256, 0, st>>>(P, dt, g0, tiles * ntstrict); \
+    if (nlazy > 0)                                                                                        \
+      k_ntt_inv_row_tensor_d<S, 1><<<grid(nlazy), 256, 0, st>>>(P, dt + ntstrict, g0, tiles * nlazy);      \
+  }
     AOA_DISPATCH_S(N2, CALLR)
 #undef CALLR
   }
