@@ -292,8 +292,11 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
           if (pr[v[t].prime] >= (1ull << 61)) return -1;
         return n;
       };
-      g.ns_key2 = strict_prefix(key2);
-      g.ns_trow = strict_prefix(trow);
+      // Small stages (a serving step of a few requests) keep one launch: the
+      // split pays only where a launch fills the GPU.
+      constexpr size_t kSplitMin = 256;
+      g.ns_key2 = key2.size() >= kSplitMin ? strict_prefix(key2) : -1;
+      g.ns_trow = trow.size() >= kSplitMin ? strict_prefix(trow) : -1;
       g.n_trow = (int)trow.size();
       g.n_tcol = (int)tcol.size();
       g.d_trow = (TensorRowDesc*)(trow.empty() ? nullptr : put(trow.data(), trow.size() * sizeof(TensorRowDesc)));
