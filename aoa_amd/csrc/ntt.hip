@@ -160,12 +160,28 @@ __device__ __forceinline__ void fwd_row_body(const DevParams& P, const uint64_t*
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) L[swz(sl + TS * k)] = x[k];
-  __syncthreads();
   const int sh = sl;
+  // The next round's twiddle is requested before the barrier, so its latency
+  // overlaps the exchange.
+  const tw2 t4 = load_tw(T, ((size_t)1 << (g0 + 4)) + ((size_t)row << 4) + ((size_t)sh << (4 - (LOGS - 4))));
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < 16; ++k) x[k] = L[swz(16 * sh + k)];
+  {
+    constexpr int half = 1 << (LOGS - 5);
 #pragma unroll
-  for (int r = 4; r < LOGS; ++r) {
+    for (int k = 0; k < 16; ++k) {
+      if (k & half) continue;
+      if (LOGS == 8) {
+        ct_bfly<LZ>(x[k], x[k + half], t4, p);
+      } else {
+        const size_t base = ((size_t)1 << (g0 + 4)) + ((size_t)row << 4) + ((size_t)sh << (4 - (LOGS - 4)));
+        ct_bfly<LZ>(x[k], x[k + half], load_tw(T, base + (k >> (LOGS - 4))), p);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 5; r < LOGS; ++r) {
     const int half = 1 << (LOGS - 1 - r);
     const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r) + ((size_t)sh << (r - (LOGS - 4)));
 #pragma unroll
@@ -198,12 +214,22 @@ __device__ __forceinline__ void inv_row_core(const DevParams& P, uint64_t (&x)[1
   constexpr int TS = G::TS, LOGS = G::LOGS;
 #pragma unroll
   for (int k = 0; k < 16; ++k) L[swz(sl + TS * k)] = x[k];
-  __syncthreads();
   const int sh = sl;
+  // The first round's eight twiddles are requested before the barrier, so
+  // their latency overlaps the exchange.
+  tw2 t0[8];
+  {
+    const size_t base = ((size_t)1 << (g0 + LOGS - 1)) + ((size_t)row << (LOGS - 1)) + ((size_t)sh << 3);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t0[q] = load_tw(T, base + q);
+  }
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < 16; ++k) x[k] = L[swz(16 * sh + k)];
 #pragma unroll
-  for (int r = LOGS - 1; r >= 4; --r) {
+  for (int k = 0; k < 16; k += 2) gs_bfly<LZ>(x[k], x[k + 1], t0[k >> 1], p);
+#pragma unroll
+  for (int r = LOGS - 2; r >= 4; --r) {
     const int half = 1 << (LOGS - 1 - r);
     const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r) + ((size_t)sh << (r - (LOGS - 4)));
 #pragma unroll
@@ -215,11 +241,19 @@ __device__ __forceinline__ void inv_row_core(const DevParams& P, uint64_t (&x)[1
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) L[swz(16 * sh + k)] = x[k];
+  // likewise the eight twiddles of the round after the second exchange
+  {
+    const size_t base = ((size_t)1 << (g0 + 3)) + ((size_t)row << 3);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t0[q] = load_tw(T, base + q);
+  }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < 16; ++k) x[k] = L[swz(sl + TS * k)];
 #pragma unroll
-  for (int r = 3; r >= 0; --r) {
+  for (int k = 0; k < 16; k += 2) gs_bfly<LZ>(x[k], x[k + 1], t0[k >> 1], p);
+#pragma unroll
+  for (int r = 2; r >= 0; --r) {
     const int half = 8 >> r;
     const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r);
 #pragma unroll
