@@ -74,8 +74,10 @@ def test_batch_mixed_ring_dimensions(gpu):
     rng = np.random.default_rng(32)
     B = Batch()
     checks = []
-    for N, k in ((8192, 4), (16384, 8), (65536, 3)):
-        P = Params(N, k, 0, 48)
+    # N <= 4096: tensor kernel + single-workgroup transforms; from 8192 up: the
+    # tensor product fused into the two-pass inverse transform
+    for N, k, prec in ((2048, 2, 55), (4096, 3, 52), (8192, 4, 48), (16384, 8, 48), (65536, 3, 48)):
+        P = Params(N, k, 0, prec)
         evk = [P.to_device(_rand(P, rng, P.k + P.kl)) for _ in range(2)]
         items = _mul_items(P, rng, [k - 1, 1])
         checks.append((P, evk, items, [_plain_mul(P, evk, it) for it in items]))
