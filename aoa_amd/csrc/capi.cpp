@@ -73,6 +73,7 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
       {H.enc_w.data(), H.enc_w.size() * 8, 0},
       {H.rot_group.data(), H.rot_group.size() * 4, 0},
       {H.gauss_cdt.data(), H.gauss_cdt.size() * 8, 0},
+      {H.md_const.data(), H.md_const.size() * sizeof(MdConst), 0},
   };
   const int nsec = sizeof(secs) / sizeof(secs[0]);
   size_t total = 0;
@@ -111,6 +112,7 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
   D.enc_w = (const double*)(b + secs[10].off);
   D.rot_group = (const uint32_t*)(b + secs[11].off);
   D.gauss_cdt = (const uint64_t*)(b + secs[12].off);
+  D.md_const = (const uint32_t*)(b + secs[13].off);
   *out = h;
   return AOA_OK;
 }
@@ -167,6 +169,10 @@ extern "C" int64_t aoa_params_table(const aoa_params* h, int which, uint64_t* ou
       for (uint32_t g : H.rot_group) v.push_back(g);
       break;
     case 13: v = H.gauss_cdt; break;
+    case 14:  // ModDown constants, eight 32-bit words per (i, j) widened to 64 bits
+      for (const MdConst& m : H.md_const)
+        for (uint32_t x : {m.cl, m.ch, m.cs, m.w0, m.w1, m.w2, m.thr, m.pad}) v.push_back(x);
+      break;
     default: return -fail(AOA_ERR_ARG, "unknown table");
   }
   if (!out) return (int64_t)v.size();

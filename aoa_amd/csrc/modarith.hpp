@@ -185,6 +185,26 @@ AOA_HD uint64_t ref_mulmod_shoup_lazy(uint64_t a, uint64_t w, uint64_t wq, uint6
   return r;
 }
 
+// Screening a product a*c mod p for the deviation window without computing
+// its remainder (a < 2^56, c < p < 2^63).  With W = floor(c 2^96 / p) =
+// w2 2^64 + w1 2^32 + w0 and r = a*c mod p, the top 32-bit word of
+// (a W mod 2^96) is f = floor(r 2^32 / p) or f - 1 (mod 2^32): a W =
+// m 2^96 + r 2^96 / p - a theta with 0 <= theta < 1 and a theta < 2^64.
+// moddown_screen sums the four partial products that reach that word, drops
+// the carry out of the lowest one (a0 w0, at most one more) and adds 2, so it
+// returns f, f + 1 or f + 2 (mod 2^32).  Hence r < thr implies
+// moddown_screen(...) <= moddown_screen_thr(thr, p).
+// bias must be 2^33; it is an argument so that a kernel can hold it in
+// registers as the first multiply-add's addend.
+AOA_HD uint32_t moddown_screen(uint32_t a0, uint32_t a1, uint32_t w0, uint32_t w1, uint32_t w2,
+                               uint64_t bias = 2ull << 32) {
+  // a0 w1 + a1 w0 sits at bit 32: its overflow past 2^64 is 2^96, which drops
+  uint64_t t = (uint64_t)a0 * w1 + bias;
+  t += (uint64_t)a1 * w0;
+  return a1 * w1 + a0 * w2 + (uint32_t)(t >> 32);
+}
+
+
 // Bit length of a nonzero 64-bit value.
 AOA_HD int bitlen64(uint64_t x) {
   int n = 0;

@@ -76,6 +76,24 @@ static std::string select_primes(int k, int kl, int prec, std::vector<aoa_prime_
   return "";
 }
 
+// The words (w0, w1, w2) of floor(c 2^96 / p) for c < p < 2^63, by long
+// division one 32-bit word at a time.
+void moddown_screen_words(uint64_t c, uint64_t p, uint32_t w[3]) {
+  unsigned __int128 r = c;
+  for (int x = 2; x >= 0; --x) {
+    r <<= 32;
+    w[x] = (uint32_t)(r / p);
+    r %= p;
+  }
+}
+
+// Screen threshold for remainders below thr: floor(thr 2^32 / p) + 2, or
+// 2^32 - 1 (every product screened in) when that does not fit.
+uint32_t moddown_screen_thr(uint64_t thr, uint64_t p) {
+  const unsigned __int128 t = (((unsigned __int128)thr << 32) / p) + 2;
+  return t > 0xFFFFFFFFu ? 0xFFFFFFFFu : (uint32_t)t;
+}
+
 std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
   if (N < 2048 || N > 65536 || (N & (N - 1)) != 0) return "NTT is not working for this N";
   if (kl <= 0) kl = k + 1;
@@ -186,6 +204,7 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
     P.inv_bhat[2 * j + 1] = shoup_quot(inv, bj);
   }
   P.bhat_mod_q.assign((size_t)k * kl * 2, 0);
+  P.md_const.assign((size_t)k * kl, MdConst{});
   P.inv_b_mod_q.assign((size_t)k * 2, 0);
   P.b_mod_q.assign((size_t)k * 2, 0);
   uint64_t bmax = 0;
@@ -215,6 +234,26 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
     } else {
       d.quirk_lo = 0;
       d.quirk_thr = ~(uint64_t)0;
+    }
+    // Karatsuba split and screen words of every c, for Q primes below 2^56
+    // (the 63-bit q0 reduces each product instead).
+    if (d.nb <= 56) {
+      const uint64_t wthr = d.quirk_thr < qi ? d.quirk_thr : qi;
+      const uint32_t sthr = moddown_screen_thr(wthr, qi);
+      for (int j = 0; j < kl; ++j) {
+        const uint64_t c = P.bhat_mod_q[((size_t)i * kl + j) * 2];
+        MdConst& m = P.md_const[(size_t)i * kl + j];
+        uint32_t w[3];
+        moddown_screen_words(c, qi, w);
+        m.cl = (uint32_t)(c & ((1u << 28) - 1));
+        m.ch = (uint32_t)(c >> 28);
+        m.cs = m.cl + m.ch;
+        m.w0 = w[0];
+        m.w1 = w[1];
+        m.w2 = w[2];
+        m.thr = sthr;
+        m.pad = 2;  // moddown_screen's bias 2^33, as its high word
+      }
     }
     uint64_t inv = inv_mod(bfull, qi);
     P.inv_b_mod_q[2 * i] = inv;

@@ -71,6 +71,16 @@ struct DevParams {
   const double* enc_w;           // [2N][2]: exp(2 pi i t / 2N), the encoder's roots of unity
   const uint32_t* rot_group;     // [N/2]: 5^j mod 2N
   const uint64_t* gauss_cdt;     // [2 kGaussHalf]: round(2^64 P(e <= j - kGaussHalf)), e = round(3.2 z)
+  const uint32_t* md_const;      // [k][K][8]: ModDown product constants (MdConst), Q primes below 2^56
+};
+
+// ModDown constants of one (q_i, b_j) pair for c = (B/b_j) mod q_i < 2^56:
+// c split at bit 28 (cl, ch) and their sum cs for the three-product
+// (Karatsuba) exact accumulation, the words of floor(c 2^96 / q_i) for
+// moddown_screen, q_i's screen threshold (the same in every j) and the high
+// word of moddown_screen's bias (2).
+struct MdConst {
+  uint32_t cl, ch, cs, w0, w1, w2, thr, pad;
 };
 
 struct HostParams {
@@ -83,6 +93,7 @@ struct HostParams {
   std::vector<double> enc_w;        // 2N complex roots, interleaved (re, im)
   std::vector<uint32_t> rot_group;  // N/2 entries
   std::vector<uint64_t> gauss_cdt;  // 2 kGaussHalf entries
+  std::vector<MdConst> md_const;    // k * K entries
 };
 
 // Builds every table.  Returns an empty string on success, else the reason
@@ -93,5 +104,12 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out);
 uint64_t inv_mod(uint64_t a, uint64_t p);
 
 uint32_t bit_reverse(uint32_t x, int bits);
+
+// The 32-bit words (w0, w1, w2) of floor(c 2^96 / p), c < p < 2^63
+// (moddown_screen's constant).
+void moddown_screen_words(uint64_t c, uint64_t p, uint32_t w[3]);
+
+// moddown_screen threshold for remainders below thr < p.
+uint32_t moddown_screen_thr(uint64_t thr, uint64_t p);
 
 }  // namespace aoa

@@ -283,6 +283,80 @@ __device__ __forceinline__ uint64_t lazy_window(const DevPrime& d, int exact) {
   return exact ? 0 : (d.quirk_thr < d.p ? d.quirk_thr : d.p);
 }
 
+typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
+
+// Sums over j in [j0, j1) of the ModDown products for NL limbs li[] with
+// primes below 2^56, reduced mod q_i, into res[].  Each product a c
+// (a < 2^56 from LDS, c = (B/b_j) mod q_i) is accumulated exactly from the
+// 28-bit halves, a c = al cl + (as cs - al cl - ah ch) 2^28 + ah ch 2^56
+// (as = al + ah, cs = cl + ch): three 32x32->64 multiply-adds into 64-bit
+// sums that cannot overflow for up to 64 terms, one 128-bit reduction per
+// limb.  Exact sums equal the reference's term-by-term modular sum whenever
+// no term deviates; SCREEN runs moddown_screen on every product, and a lane
+// whose limb passed it for any j (about 1 term in 130 000 at 48-bit q_i)
+// recomputes that limb's sum with the reference formula per term.
+template <int NL, bool SCREEN>
+__device__ __forceinline__ void moddown_conv(const DevParams& P, const uint64_t* aux, int c, const int* li,
+                                             int j0, int j1, uint64_t* res) {
+  constexpr uint32_t M28 = (1u << 28) - 1;
+  const int K = P.K;
+  const cptr<u32x8> tab = as_const(reinterpret_cast<const u32x8*>(P.md_const));
+  uint64_t s0[NL], s2[NL], sm[NL];
+  bool hit[NL];
+  uint32_t thr[NL];
+  // the screen's bias comes from the table so that it stays a register
+  // operand of the multiply-add instead of a separate addition
+  const uint64_t bias = (uint64_t)tab[(size_t)li[0] * K][7] << 32;
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    s0[n] = s2[n] = sm[n] = 0;
+    hit[n] = false;
+    thr[n] = tab[(size_t)li[n] * K][6];
+  }
+#pragma unroll 2
+  for (int j = j0; j < j1; ++j) {
+    const uint64_t a = aux[j * BC_COEF + c];
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    const uint32_t al = a0 & M28, ah = (uint32_t)(a >> 28), as = al + ah;
+#pragma unroll
+    for (int n = 0; n < NL; ++n) {
+      const u32x8 e = tab[(size_t)li[n] * K + j];
+      s0[n] += (uint64_t)al * e[0];
+      s2[n] += (uint64_t)ah * e[1];
+      sm[n] += (uint64_t)as * e[2];
+      if (SCREEN) hit[n] |= moddown_screen(a0, a1, e[3], e[4], e[5], bias) <= thr[n];
+    }
+  }
+  const tw2* bq = reinterpret_cast<const tw2*>(P.bhat_mod_q);
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    const DevPrime dq = ld_prime(P.primes, li[n]);
+    if (SCREEN && hit[n]) {
+      const uint64_t wthr = lazy_window(dq, 0);
+      const tw2* ci = bq + (size_t)li[n] * K;
+      uint64_t acc = 0;
+      for (int j = j0; j < j1; ++j) {
+        const tw2 w = ld_tw2(ci, j);
+        acc += ref_mulmod_shoup_lazy(aux[j * BC_COEF + c], w.w, w.q, dq.p, dq.mu, (int)dq.nb, wthr);
+      }
+      res[n] = barrett_reduce(0, acc, dq.p, dq.mu, (int)dq.nb);
+    } else {
+      uint64_t h, l;
+      join_28(s0[n], sm[n] - s0[n] - s2[n], s2[n], &h, &l);
+      res[n] = reduce128(h, l, dq);
+    }
+  }
+}
+
+template <int NL>
+__device__ __forceinline__ void moddown_convs(const DevParams& P, const uint64_t* aux, int c, const int* li,
+                                              int j0, int j1, uint64_t* res) {
+  if (P.moddown_exact)
+    moddown_conv<NL, false>(P, aux, c, li, j0, j1, res);
+  else
+    moddown_conv<NL, true>(P, aux, c, li, j0, j1, res);
+}
+
 __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t* inQ, const uint64_t* inB,
                                              const uint64_t* add, uint64_t* out, int level, int rescale,
                                              int blk, uint64_t* aux) {
@@ -301,81 +375,18 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   __syncthreads();
   const tw2* ibq = reinterpret_cast<const tw2*>(P.inv_b_mod_q);
   const tw2* bq = reinterpret_cast<const tw2*>(P.bhat_mod_q);
-  // Sum over j of the reference's products for limb i, reduced mod q_i.
-  // Terms of primes below 2^57 are exact Shoup products left in [0, 2p) and
-  // summed without reduction (K <= 64 terms).  A term whose remainder falls
-  // in the deviation window only raises a flag; a flagged lane (about one
-  // coefficient-limb in 4000 at 48-bit q_i) recomputes its sum with the
-  // reference formula per term.  The 63-bit prime reduces every term.
+  // Sum over j in [j0, j1) of the reference's products for limb i of a prime
+  // wider than 56 bits (q0), reduced term by term.
   auto conv = [&](int i, int j0, int j1) -> uint64_t {
     const DevPrime dq = ld_prime(P.primes, i);
     const tw2* ci = bq + (size_t)i * K;
     uint64_t acc = 0;
-    if (dq.nb <= 56) {
-      const uint64_t thr = lazy_window(dq, P.moddown_exact);
-      const uint64_t pthr = dq.p + thr;
-      bool hit = false;
-#pragma unroll 4
-      for (int j = j0; j < j1; ++j) {
-        const tw2 w = ld_tw2(ci, j);
-        const uint64_t r = mul_shoup_lazy(aux[j * BC_COEF + c], w.w, w.q, dq.p);
-        // r in [0, 2p): its canonical value is below thr iff r < thr or
-        // p <= r < p + thr (three compares, no subtraction)
-        hit |= (r < thr) | ((r >= dq.p) & (r < pthr));
-        acc += r;
-      }
-      if (hit) {
-        acc = 0;
-        for (int j = j0; j < j1; ++j) {
-          const tw2 w = ld_tw2(ci, j);
-          acc += ref_mulmod_shoup_lazy(aux[j * BC_COEF + c], w.w, w.q, dq.p, dq.mu, (int)dq.nb, thr);
-        }
-      }
-      return barrett_reduce(0, acc, dq.p, dq.mu, (int)dq.nb);
-    }
 #pragma unroll 4
     for (int j = j0; j < j1; ++j) {
       const tw2 w = ld_tw2(ci, j);
       acc = add_mod(acc, moddown_term(aux[j * BC_COEF + c], w, dq, P.moddown_exact), dq.p);
     }
     return acc;
-  };
-  // Two limbs at once: shared aux reads, two independent accumulation chains.
-  auto conv2 = [&](int i, int i2, uint64_t* r1, uint64_t* r2) {
-    const DevPrime d1 = ld_prime(P.primes, i), d2 = ld_prime(P.primes, i2);
-    if (d1.nb > 56 || d2.nb > 56) {
-      *r1 = conv(i, 0, K);
-      *r2 = conv(i2, 0, K);
-      return;
-    }
-    const tw2* c1 = bq + (size_t)i * K;
-    const tw2* c2 = bq + (size_t)i2 * K;
-    const uint64_t t1 = lazy_window(d1, P.moddown_exact);
-    const uint64_t t2 = lazy_window(d2, P.moddown_exact);
-    const uint64_t pt1 = d1.p + t1, pt2 = d2.p + t2;
-    uint64_t a1 = 0, a2 = 0;
-    bool hit = false;
-#pragma unroll 4
-    for (int j = 0; j < K; ++j) {
-      const uint64_t a = aux[j * BC_COEF + c];
-      const tw2 w1 = ld_tw2(c1, j), w2 = ld_tw2(c2, j);
-      const uint64_t x1 = mul_shoup_lazy(a, w1.w, w1.q, d1.p);
-      const uint64_t x2 = mul_shoup_lazy(a, w2.w, w2.q, d2.p);
-      hit |= (x1 < t1) | ((x1 >= d1.p) & (x1 < pt1)) | (x2 < t2) | ((x2 >= d2.p) & (x2 < pt2));
-      a1 += x1;
-      a2 += x2;
-    }
-    if (hit) {
-      a1 = a2 = 0;
-      for (int j = 0; j < K; ++j) {
-        const uint64_t a = aux[j * BC_COEF + c];
-        const tw2 w1 = ld_tw2(c1, j), w2 = ld_tw2(c2, j);
-        a1 += ref_mulmod_shoup_lazy(a, w1.w, w1.q, d1.p, d1.mu, (int)d1.nb, t1);
-        a2 += ref_mulmod_shoup_lazy(a, w2.w, w2.q, d2.p, d2.mu, (int)d2.nb, t2);
-      }
-    }
-    *r1 = barrett_reduce(0, a1, d1.p, d1.mu, (int)d1.nb);
-    *r2 = barrett_reduce(0, a2, d2.p, d2.mu, (int)d2.nb);
   };
   auto finish = [&](int i, uint64_t acc) -> uint64_t {
     const uint64_t p = P.primes[i].p;
@@ -384,15 +395,43 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
     if (add) v = add_mod(v, add[(size_t)i * N + cid], p);
     return v;
   };
-  if (!rescale) {
+  // This wave's limbs g, g+4, ... below iend, in passes of four, two or one
+  // (limbs of primes wider than 56 bits, i.e. q0, one at a time with
+  // conv).  store() writes limb i from its sum.
+  auto limbs = [&](int iend, auto store) {
     int i = g;
-    for (; i + BC_GROUPS <= level; i += 2 * BC_GROUPS) {
-      uint64_t r1, r2;
-      conv2(i, i + BC_GROUPS, &r1, &r2);
-      out[(size_t)i * N + cid] = finish(i, r1);
-      out[(size_t)(i + BC_GROUPS) * N + cid] = finish(i + BC_GROUPS, r2);
+    while (i < iend) {
+      const int left = (iend - i + BC_GROUPS - 1) / BC_GROUPS;
+      if (P.primes[i].nb > 56) {
+        store(i, conv(i, 0, K));
+        i += BC_GROUPS;
+        continue;
+      }
+      int li[4] = {i, i + BC_GROUPS, i + 2 * BC_GROUPS, i + 3 * BC_GROUPS};
+      // the first n limbs of li all have primes below 2^57
+      auto narrow = [&](int n) {
+        bool ok = true;
+        for (int t = 1; t < n; ++t) ok &= P.primes[li[t]].nb <= 56;
+        return ok;
+      };
+      uint64_t r[4];
+      if (left >= 4 && narrow(4)) {
+        moddown_convs<4>(P, aux, c, li, 0, K, r);
+        for (int n = 0; n < 4; ++n) store(li[n], r[n]);
+        i += 4 * BC_GROUPS;
+      } else if (left >= 2 && narrow(2)) {
+        moddown_convs<2>(P, aux, c, li, 0, K, r);
+        for (int n = 0; n < 2; ++n) store(li[n], r[n]);
+        i += 2 * BC_GROUPS;
+      } else {
+        moddown_convs<1>(P, aux, c, li, 0, K, r);
+        store(i, r[0]);
+        i += BC_GROUPS;
+      }
     }
-    if (i <= level) out[(size_t)i * N + cid] = finish(i, conv(i, 0, K));
+  };
+  if (!rescale) {
+    limbs(level + 1, [&](int i, uint64_t acc) { out[(size_t)i * N + cid] = finish(i, acc); });
     return;
   }
   // Rescale needs limb `level` everywhere: its sum over j is split across the
@@ -400,7 +439,16 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   uint64_t* part = aux + (size_t)K * BC_COEF;
   const int jq = (K + BC_GROUPS - 1) / BC_GROUPS;
   const int j0 = g * jq, j1 = min(K, j0 + jq);
-  part[g * BC_COEF + c] = conv(level, j0, j1 > j0 ? j1 : j0);
+  {
+    uint64_t r = 0;
+    if (P.primes[level].nb > 56) {
+      r = conv(level, j0, j1 > j0 ? j1 : j0);
+    } else if (j1 > j0) {
+      const int li[1] = {level};
+      moddown_convs<1>(P, aux, c, li, j0, j1, &r);
+    }
+    part[g * BC_COEF + c] = r;
+  }
   __syncthreads();
   const uint64_t pl = P.primes[level].p;
   uint64_t acc = 0;
@@ -408,20 +456,12 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   for (int t = 0; t < BC_GROUPS; ++t) acc = add_mod(acc, part[t * BC_COEF + c], pl);
   const uint64_t top = finish(level, acc);
   const tw2* iql = reinterpret_cast<const tw2*>(P.inv_ql_mod_qi) + (size_t)level * k;
-  auto emit = [&](int i, uint64_t acc) {
+  limbs(level, [&](int i, uint64_t acc) {
     const uint64_t p = P.primes[i].p;
     const uint64_t v = finish(i, acc);
     const tw2 s = iql[i];
     out[(size_t)i * N + cid] = mul_shoup(sub_mod(v, top, p), s.w, s.q, p);
-  };
-  int i = g;
-  for (; i + BC_GROUPS < level; i += 2 * BC_GROUPS) {
-    uint64_t r1, r2;
-    conv2(i, i + BC_GROUPS, &r1, &r2);
-    emit(i, r1);
-    emit(i + BC_GROUPS, r2);
-  }
-  if (i < level) emit(i, conv(i, 0, K));
+  });
 }
 
 __global__ __launch_bounds__(256) void k_moddown(DevParams P, ModDownArgs A) {

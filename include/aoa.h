@@ -65,7 +65,11 @@ int aoa_params_primes(const aoa_params* params, uint64_t* out, int n);
  * 4 (B/b_j)^-1 mod b_j [kl], 5 (B/b_j) mod q_i [k][kl], 6 B^-1 mod q_i [k],
  * 7 B mod q_i [k], 8 q_l^-1 mod q_i [k][k], 9 Barrett MU floor(4^nb/p) [k+kl],
  * 11 encoder roots exp(2 pi i t / 2N) [2N][re, im] (IEEE double bit patterns),
- * 12 rotation group 5^j mod 2N [N/2], 13 Gaussian table of aoa_sample [64].
+ * 12 rotation group 5^j mod 2N [N/2], 13 Gaussian table of aoa_sample [64],
+ * 14 ModDown product constants [k][kl][8] for c = (B/b_j) mod q_i (c mod 2^28,
+ * c >> 28, their sum, the 32-bit words w0 w1 w2 of floor(c 2^96 / q_i), q_i's
+ * screen threshold, 2 (the high word of its bias 2^33); all zero for primes
+ * of 57 bits or more).
  * Returns the number of words written (or needed, if out is NULL). */
 int64_t aoa_params_table(const aoa_params* params, int which, uint64_t* out, int64_t n);
 

@@ -61,12 +61,14 @@ int64_t chk_exact_mults(int N, int k, int kl, int prec, int64_t trials, uint64_t
 // Random operands plus operands built so that the exact remainder sits at
 // the edges of the deviation window [eps, thr).  out[0] = mismatches,
 // out[1] = products where ref_barrett differs from the exact remainder,
-// out[2] = products that took the literal path.
+// out[2] = products that took the literal path, out[3] = products that
+// moddown_screen passed (Q primes below 2^56), which must include every
+// product with its remainder below the window bound.
 int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t seed, int64_t* out) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
   std::mt19937_64 rng(seed);
-  int64_t bad = 0, quirks = 0, slow = 0;
+  int64_t bad = 0, quirks = 0, slow = 0, screened = 0;
   for (int i = 0; i < k; ++i) {
     const DevPrime& d = H.dprimes[i];
     const uint64_t p = d.p;
@@ -102,12 +104,92 @@ int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t s
       if (lazy >= 2 * p || lazy % p != ref) ++bad;
       if (ref != exact) ++quirks;
       if (exact >= d.quirk_lo && exact < d.quirk_thr) ++slow;
+      if (d.nb <= 56) {
+        // the screen must pass every product whose remainder is below the
+        // lazy window bound, in particular every deviating one
+        const MdConst& m = H.md_const[(size_t)i * kl + j];
+        const uint32_t f = moddown_screen((uint32_t)a, (uint32_t)(a >> 32), m.w0, m.w1, m.w2);
+        const bool in = f <= m.thr;
+        if ((exact < thr || ref != exact) && !in) ++bad;
+        if (in) ++screened;
+      }
     }
   }
   out[0] = bad;
   out[1] = quirks;
   out[2] = slow;
+  out[3] = screened;
   return 0;
+}
+
+// moddown_screen against its bound on the Q primes below 2^56 of three
+// parameter sets: for random and extreme a < 2^56 and c < p, the returned
+// word minus floor(r 2^32 / p) (r = a c mod p) is 0, 1 or 2 modulo 2^32.
+// Returns the number of violations.
+int64_t chk_moddown_screen_words(int64_t trials, uint64_t seed) {
+  std::mt19937_64 rng(seed);
+  int64_t bad = 0;
+  const int shapes[3][4] = {{65536, 30, 31, 48}, {4096, 4, 5, 52}, {16384, 5, 5, 55}};
+  for (const auto& sh : shapes) {
+    HostParams H;
+    if (!build_host_params(sh[0], sh[1], sh[2], sh[3], &H).empty()) return -1;
+    for (int i = 0; i < sh[1]; ++i) {
+      const uint64_t p = H.primes[i];
+      if (H.dprimes[i].nb > 56) continue;
+      for (int64_t t = 0; t < trials; ++t) {
+        uint64_t c = rng() % p, a = rng() >> 8;
+        if (t < 9) {
+          c = (t % 3 == 0) ? p - 1 : (t % 3 == 1 ? 1 : 0);
+          a = (t / 3 == 0) ? (1ull << 56) - 1 : (t / 3 == 1 ? 0 : p);
+        }
+        if (t >= 9 && t < 4096 + 9) a = (uint64_t)(t - 9);  // every a < 2^12
+        uint32_t w[3];
+        moddown_screen_words(c, p, w);
+        const uint64_t r = (uint64_t)(((unsigned __int128)a * c) % p);
+        const uint32_t f = (uint32_t)((((unsigned __int128)r) << 32) / p);
+        const uint32_t got = moddown_screen((uint32_t)a, (uint32_t)(a >> 32), w[0], w[1], w[2]);
+        const uint32_t d = got - f;
+        if (d > 2) ++bad;
+      }
+    }
+  }
+  return bad;
+}
+
+// The three-product accumulation of ModDown: for random and extreme a_j <
+// b_j, sum_j a_j c_ij = s0 + (sm - s0 - s2) 2^28 + s2 2^56 with s0, s2, sm
+// summed in 64 bits from the 28-bit halves of a_j and the MdConst halves of
+// c_ij, compared with the 128-bit sum, for every Q prime below 2^56.
+// Returns the number of mismatches (or -1 on bad params).
+int64_t chk_moddown_accumulate(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
+  HostParams H;
+  if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  std::mt19937_64 rng(seed);
+  int64_t bad = 0;
+  const uint32_t M28 = (1u << 28) - 1;
+  for (int i = 0; i < k; ++i) {
+    if (H.dprimes[i].nb > 56) continue;
+    for (int64_t t = 0; t < trials; ++t) {
+      uint64_t s0 = 0, s2 = 0, sm = 0;
+      unsigned __int128 want = 0;
+      for (int j = 0; j < kl; ++j) {
+        const uint64_t bj = H.primes[k + j];
+        const uint64_t a = t == 0 ? bj - 1 : (t == 1 ? (bj - 1) & ~(uint64_t)M28 : rng() % bj);
+        const MdConst& m = H.md_const[(size_t)i * kl + j];
+        const uint64_t c = H.bhat_mod_q[((size_t)i * kl + j) * 2];
+        if ((uint64_t)m.cl + ((uint64_t)m.ch << 28) != c || m.cs != m.cl + m.ch) ++bad;
+        const uint32_t al = (uint32_t)a & M28, ah = (uint32_t)(a >> 28), as = al + ah;
+        s0 += (uint64_t)al * m.cl;
+        s2 += (uint64_t)ah * m.ch;
+        sm += (uint64_t)as * m.cs;
+        want += (unsigned __int128)a * c;
+      }
+      const unsigned __int128 got =
+          (unsigned __int128)s0 + ((unsigned __int128)(sm - s0 - s2) << 28) + ((unsigned __int128)s2 << 56);
+      if (got != want) ++bad;
+    }
+  }
+  return bad;
 }
 
 // The product's host tables (same numbering as aoa_params_table), built
@@ -141,6 +223,10 @@ int64_t chk_table(int N, int k, int kl, int prec, int which, uint64_t* out) {
       for (uint32_t g : H.rot_group) v.push_back(g);
       break;
     case 13: v = H.gauss_cdt; break;
+    case 14:
+      for (const MdConst& m : H.md_const)
+        for (uint32_t x : {m.cl, m.ch, m.cs, m.w0, m.w1, m.w2, m.thr, m.pad}) v.push_back(x);
+      break;
     default: return -1;
   }
   if (out)

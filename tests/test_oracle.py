@@ -110,10 +110,56 @@ def test_moddown_fast_term_matches_reference_formula():
     chk = native_chk.load()
     for shape, expect_quirk in [((65536, 30, 31, 48), True), ((4096, 4, 5, 52), True),
                                 ((16384, 5, 5, 55), False), ((4096, 10, 11, 45), False)]:
-        out = (ctypes.c_int64 * 3)()
+        out = (ctypes.c_int64 * 4)()
         assert chk.chk_moddown_terms(*shape, 40000, 11, out) == 0
         assert out[0] == 0, "fast ModDown term differs from the reference formula %s" % (shape,)
         assert (out[1] > 0) == expect_quirk
+        # the screen passes the aimed window products (3 in 4 trials) and
+        # few of the random ones
+        assert out[3] >= out[1]
+
+
+def test_moddown_screen_bound_exhaustive_small():
+    """moddown_screen's claim, checked by brute force over every a < 2^12 and
+    a few c for small primes, plus random 55-bit a: f = floor(r 2^32 / p) and
+    the returned word lies in {f, f+1, f+2} mod 2^32."""
+    chk = native_chk.load()
+    assert chk.chk_moddown_screen_words(2000, 7) == 0
+
+
+def test_moddown_constant_table():
+    """Table 14 against Python big integers: the 28-bit split of
+    c = (B/b_j) mod q_i, floor(c 2^96 / q_i) and the screen threshold
+    floor(thr 2^32 / q_i) + 2 with thr = delta_max eps (delta_max =
+    floor(b_max q_i / 2^(2nb)) + 3, eps = 2^nb - q_i) when that is below
+    2^(nb-1), else every product screened in."""
+    for N, k, kl, prec in [(65536, 30, 31, 48), (4096, 4, 5, 52), (16384, 5, 5, 55)]:
+        primes = [int(v) for v in native_chk.table(N, k, kl, prec, 10)]
+        c = native_chk.table(N, k, kl, prec, 5).reshape(k, kl)
+        t = native_chk.table(N, k, kl, prec, 14).reshape(k, kl, 8)
+        Q, B = primes[:k], primes[k:]
+        bmax = max(B)
+        for i, q in enumerate(Q):
+            nb = q.bit_length()
+            if nb > 56:
+                assert not t[i].any()
+                continue
+            eps = (1 << nb) - q
+            dmax = (bmax * q >> (2 * nb)) + 3
+            thr = dmax * eps if dmax * eps < (1 << (nb - 1)) else None
+            sthr = 0xFFFFFFFF if thr is None else min(0xFFFFFFFF, (min(thr, q) << 32) // q + 2)
+            for j in range(kl):
+                cij = int(c[i, j])
+                W = (cij << 96) // q
+                row = [int(x) for x in t[i, j]]
+                assert row == [cij & (2**28 - 1), cij >> 28, (cij & (2**28 - 1)) + (cij >> 28),
+                               W & 0xFFFFFFFF, (W >> 32) & 0xFFFFFFFF, W >> 64, sthr, 2], (N, k, i, j)
+
+
+def test_moddown_three_product_accumulation():
+    chk = native_chk.load()
+    for shape in [(65536, 30, 31, 48), (4096, 4, 5, 52), (16384, 5, 5, 55), (4096, 10, 11, 45)]:
+        assert chk.chk_moddown_accumulate(*shape, 300, 3) == 0, shape
 
 
 def test_product_exact_mults():
