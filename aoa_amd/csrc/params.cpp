@@ -220,14 +220,25 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
       P.bhat_mod_q[((size_t)i * kl + j) * 2] = pr;
       P.bhat_mod_q[((size_t)i * kl + j) * 2 + 1] = shoup_quot(pr, qi);
     }
-    // Barrett quotient error bound for products below bmax * q_i:
-    // delta < (bmax q_i) / 2^(2nb) + 2.  The reference result deviates from
-    // the exact remainder only for eps <= r < delta * eps, which needs
-    // delta * eps < 2^nb; otherwise every product takes the literal formula.
+    // Bound on the reference's Barrett quotient error delta = floor(x/q) - qhat
+    // for x = a c < bmax q_i, where qhat = floor(X1 MU / 2^64) with
+    // X1 = floor(x / 2^s), s = 2nb - 64 and MU = (2^(2nb) - rho) / q_i,
+    // rho = 2^(2nb) mod q_i.  Then x/q - X1 MU / 2^64 = (x mod 2^s)/q +
+    // X1 rho / (q 2^64) < 1 + X1 rho / (q 2^64) (as 2^s < q), so
+    // delta < 2 + X1 rho / (q 2^64) and delta <= floor(X1max rho / (q 2^64)) + 2.
+    // The reference result deviates from the exact remainder only for
+    // eps <= r < delta * eps, which needs delta * eps < 2^nb; otherwise every
+    // product takes the literal formula.
     DevPrime& d = P.dprimes[i];
-    unsigned __int128 pmax = (unsigned __int128)bmax * qi;
-    unsigned __int128 dmax = (pmax >> (2 * d.nb)) + 3;
-    unsigned __int128 thr = dmax * d.eps;
+    unsigned __int128 dmax;
+    if (d.nb >= 33) {
+      const unsigned __int128 x1max = ((unsigned __int128)(bmax - 1) * (qi - 1)) >> (2 * d.nb - 64);
+      const unsigned __int128 rho = (((unsigned __int128)1) << (2 * d.nb)) % qi;
+      dmax = x1max * rho / (((unsigned __int128)qi) << 64) + 2;
+    } else {
+      dmax = ~(unsigned __int128)0 >> 1;
+    }
+    unsigned __int128 thr = dmax > ((unsigned __int128)1 << 64) ? ~(unsigned __int128)0 >> 1 : dmax * d.eps;
     if (thr < ((unsigned __int128)1 << (d.nb - 1))) {
       d.quirk_lo = d.eps;
       d.quirk_thr = (uint64_t)thr;
@@ -236,7 +247,9 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
       d.quirk_thr = ~(uint64_t)0;
     }
     // Karatsuba split and screen words of every c, for Q primes below 2^56
-    // (the 63-bit q0 reduces each product instead).
+    // (the 63-bit q0 reduces each product instead).  One screen threshold per
+    // prime: a bound per (i, j) would pass fewer products but costs the
+    // kernel's inner loop one more scalar operand per product.
     if (d.nb <= 56) {
       const uint64_t wthr = d.quirk_thr < qi ? d.quirk_thr : qi;
       const uint32_t sthr = moddown_screen_thr(wthr, qi);

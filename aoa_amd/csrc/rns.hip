@@ -293,8 +293,9 @@ typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
 // sums that cannot overflow for up to 64 terms, one 128-bit reduction per
 // limb.  Exact sums equal the reference's term-by-term modular sum whenever
 // no term deviates; SCREEN runs moddown_screen on every product, and a lane
-// whose limb passed it for any j (about 1 term in 130 000 at 48-bit q_i)
-// recomputes that limb's sum with the reference formula per term.
+// whose limb passed it for any j (one term in 10 000 to 300 000 at 48-bit
+// q_i, growing with 2^48 - q_i) recomputes that limb's sum with the
+// reference formula per term.
 template <int NL, bool SCREEN>
 __device__ __forceinline__ void moddown_conv(const DevParams& P, const uint64_t* aux, int c, const int* li,
                                              int j0, int j1, uint64_t* res) {

@@ -68,8 +68,8 @@ int aoa_params_primes(const aoa_params* params, uint64_t* out, int n);
  * 12 rotation group 5^j mod 2N [N/2], 13 Gaussian table of aoa_sample [64],
  * 14 ModDown product constants [k][kl][8] for c = (B/b_j) mod q_i (c mod 2^28,
  * c >> 28, their sum, the 32-bit words w0 w1 w2 of floor(c 2^96 / q_i), q_i's
- * screen threshold, 2 (the high word of its bias 2^33); all zero for primes
- * of 57 bits or more).
+ * screen threshold, 2 (the high word of the screen's bias 2^33); all zero for
+ * primes of 57 bits or more).
  * Returns the number of words written (or needed, if out is NULL). */
 int64_t aoa_params_table(const aoa_params* params, int which, uint64_t* out, int64_t n);
 

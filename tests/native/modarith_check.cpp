@@ -63,7 +63,7 @@ int64_t chk_exact_mults(int N, int k, int kl, int prec, int64_t trials, uint64_t
 // out[1] = products where ref_barrett differs from the exact remainder,
 // out[2] = products that took the literal path, out[3] = products that
 // moddown_screen passed (Q primes below 2^56), which must include every
-// product with its remainder below the window bound.
+// deviating product.
 int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t seed, int64_t* out) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
@@ -92,7 +92,8 @@ int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t s
         const uint64_t cinv = inv_mod(c, p);
         uint64_t base = (uint64_t)(((unsigned __int128)target * cinv) % p);
         const uint64_t mmax = bj > base ? (bj - 1 - base) / p : 0;
-        a = base + (mmax ? (rng() % (mmax + 1)) : 0) * p;
+        // half of them with the largest such a (the largest quotient error)
+        a = base + (mmax ? ((t & 8) ? mmax : rng() % (mmax + 1)) : 0) * p;
         if (a >= bj) a = base % bj;
       }
       const uint64_t ref = ref_barrett(mulhi64(a, c), a * c, p, d.mu, (int)d.nb);
@@ -105,12 +106,11 @@ int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t s
       if (ref != exact) ++quirks;
       if (exact >= d.quirk_lo && exact < d.quirk_thr) ++slow;
       if (d.nb <= 56) {
-        // the screen must pass every product whose remainder is below the
-        // lazy window bound, in particular every deviating one
+        // the screen must pass every product that deviates
         const MdConst& m = H.md_const[(size_t)i * kl + j];
         const uint32_t f = moddown_screen((uint32_t)a, (uint32_t)(a >> 32), m.w0, m.w1, m.w2);
         const bool in = f <= m.thr;
-        if ((exact < thr || ref != exact) && !in) ++bad;
+        if (ref != exact && !in) ++bad;
         if (in) ++screened;
       }
     }
@@ -122,7 +122,47 @@ int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t s
   return 0;
 }
 
-// moddown_screen against its bound on the Q primes below 2^56 of three
+// The quotient error delta = floor(x / q) - qhat of the reference's Barrett
+// reduction (qhat as ref_barrett computes it) against the bound behind the
+// deviation window, quirk_thr = delta_max * eps, for products x = a c with
+// a below the largest special prime and c < q: random x, x at the top of the
+// range and x with the largest possible floor(x / 2^(2nb - 64)).  Returns the
+// number of products with delta * eps >= quirk_thr, or -1 on bad params.
+int64_t chk_barrett_delta(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
+  HostParams H;
+  if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  std::mt19937_64 rng(seed);
+  uint64_t bmax = 0;
+  for (int j = 0; j < kl; ++j) bmax = H.primes[k + j] > bmax ? H.primes[k + j] : bmax;
+  int64_t bad = 0;
+  for (int i = 0; i < k; ++i) {
+    const DevPrime& d = H.dprimes[i];
+    if (d.quirk_thr == ~(uint64_t)0) continue;
+    const uint64_t q = d.p;
+    const int nb = (int)d.nb, s = 2 * nb - 64;
+    const unsigned __int128 xmax = (unsigned __int128)(bmax - 1) * (q - 1);
+    for (int64_t t = 0; t < trials; ++t) {
+      unsigned __int128 x;
+      if (t % 3 == 0) {
+        x = xmax - (rng() % 4096);
+      } else if (t % 3 == 1) {
+        // the largest X1 with its smallest low part
+        x = ((xmax >> s) << s) + (rng() % 16);
+        if (x > xmax) x = xmax;
+      } else {
+        x = (((unsigned __int128)rng() << 64) | rng()) % (xmax + 1);
+      }
+      const uint64_t hi = (uint64_t)(x >> 64), lo = (uint64_t)x;
+      const uint64_t xlo = (lo >> s) | (hi << (64 - s)), xhi = hi >> s;
+      const uint64_t qhat = mulhi64(xlo, d.mu) + xhi * d.mu;
+      const unsigned __int128 delta = x / q - qhat;
+      if (delta * d.eps >= d.quirk_thr) ++bad;
+    }
+  }
+  return bad;
+}
+
+// moddown_screen against its bound// moddown_screen against its bound on the Q primes below 2^56 of three
 // parameter sets: for random and extreme a < 2^56 and c < p, the returned
 // word minus floor(r 2^32 / p) (r = a c mod p) is 0, 1 or 2 modulo 2^32.
 // Returns the number of violations.

@@ -33,6 +33,8 @@ def load():
     lib.chk_moddown_terms.restype = ctypes.c_int
     lib.chk_moddown_terms.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int64, ctypes.c_uint64,
                                                            ctypes.POINTER(ctypes.c_int64)]
+    lib.chk_barrett_delta.restype = ctypes.c_int64
+    lib.chk_barrett_delta.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int64, ctypes.c_uint64]
     lib.chk_moddown_screen_words.restype = ctypes.c_int64
     lib.chk_moddown_screen_words.argtypes = [ctypes.c_int64, ctypes.c_uint64]
     lib.chk_moddown_accumulate.restype = ctypes.c_int64

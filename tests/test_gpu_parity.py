@@ -149,8 +149,9 @@ def _window_aimed_qb(O, rng, level, frac=4):
     """A QB polynomial whose special residues put many ModDown products
     a_j c_ij (a_j = [x_bj (B/b_j)^-1]_bj, c_ij = (B/b_j) mod q_i) at
     remainders inside the reference's deviation window [eps, delta_max eps)
-    of q_i, or at its edges: for one coefficient in `frac`, every j aims at
-    a random limb i <= level."""
+    of q_i, at its edges, or below a wider bound (delta_max from b_max q_i
+    alone): for one coefficient in `frac`, every j aims at a random limb
+    i <= level."""
     N, k, kl = O.N, O.k, O.kl
     inv_bhat = [int(v) for v in O.table(4)]
     c = O.table(5).reshape(k, kl)
@@ -162,17 +163,21 @@ def _window_aimed_qb(O, rng, level, frac=4):
         q = Q[i]
         nb = q.bit_length()
         eps = (1 << nb) - q
-        thr = ((bmax * q >> (2 * nb)) + 3) * eps
+        # the window bound delta_max eps, and a wider one
+        x1 = (bmax - 1) * (q - 1) >> (2 * nb - 64)
+        thr = (x1 * ((1 << (2 * nb)) % q) // (q << 64) + 2) * eps
+        wide = ((bmax * q >> (2 * nb)) + 3) * eps
         cinv = [pow(int(c[i, j]), -1, q) if int(c[i, j]) else 0 for j in range(kl)]
-        win.append((q, eps, thr, cinv))
+        win.append((q, eps, thr, wide, cinv))
     qb = O.random_poly(rng, k + kl).reshape(k + kl, N)
     for cid in range(0, N, frac):
         for j in range(kl):
-            q, eps, thr, cinv = win[int(rng.integers(0, level + 1))]
-            if thr >= q or cinv[j] == 0:
+            q, eps, thr, wide, cinv = win[int(rng.integers(0, level + 1))]
+            if wide >= q or cinv[j] == 0:
                 continue
-            pick = int(rng.integers(0, 4))
-            target = [eps, thr - 1, eps - 1, int(rng.integers(eps, thr))][pick] % q
+            pick = int(rng.integers(0, 6))
+            target = [eps, thr - 1, thr, eps - 1, int(rng.integers(eps, thr)),
+                      int(rng.integers(eps, wide))][pick] % q
             a = target * cinv[j] % q
             m = (B[j] - 1 - a) // q
             a += int(rng.integers(0, m + 1)) * q if m > 0 else 0

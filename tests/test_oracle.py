@@ -131,29 +131,39 @@ def test_moddown_constant_table():
     """Table 14 against Python big integers: the 28-bit split of
     c = (B/b_j) mod q_i, floor(c 2^96 / q_i) and the screen threshold
     floor(thr 2^32 / q_i) + 2 with thr = delta_max eps (delta_max =
-    floor(b_max q_i / 2^(2nb)) + 3, eps = 2^nb - q_i) when that is below
-    2^(nb-1), else every product screened in."""
+    floor(X1 rho / (q_i 2^64)) + 2, X1 = floor((b_max - 1)(q_i - 1) / 2^(2nb-64)),
+    rho = 2^(2nb) mod q_i, eps = 2^nb - q_i) when that is below 2^(nb-1),
+    else every product screened in."""
     for N, k, kl, prec in [(65536, 30, 31, 48), (4096, 4, 5, 52), (16384, 5, 5, 55)]:
         primes = [int(v) for v in native_chk.table(N, k, kl, prec, 10)]
         c = native_chk.table(N, k, kl, prec, 5).reshape(k, kl)
         t = native_chk.table(N, k, kl, prec, 14).reshape(k, kl, 8)
         Q, B = primes[:k], primes[k:]
-        bmax = max(B)
         for i, q in enumerate(Q):
             nb = q.bit_length()
             if nb > 56:
                 assert not t[i].any()
                 continue
             eps = (1 << nb) - q
-            dmax = (bmax * q >> (2 * nb)) + 3
-            thr = dmax * eps if dmax * eps < (1 << (nb - 1)) else None
-            sthr = 0xFFFFFFFF if thr is None else min(0xFFFFFFFF, (min(thr, q) << 32) // q + 2)
+            x1 = (max(B) - 1) * (q - 1) >> (2 * nb - 64)
+            thr = (x1 * ((1 << (2 * nb)) % q) // (q << 64) + 2) * eps
+            sthr = 0xFFFFFFFF if thr >= (1 << (nb - 1)) else min(0xFFFFFFFF, (thr << 32) // q + 2)
             for j in range(kl):
                 cij = int(c[i, j])
                 W = (cij << 96) // q
                 row = [int(x) for x in t[i, j]]
                 assert row == [cij & (2**28 - 1), cij >> 28, (cij & (2**28 - 1)) + (cij >> 28),
                                W & 0xFFFFFFFF, (W >> 32) & 0xFFFFFFFF, W >> 64, sthr, 2], (N, k, i, j)
+
+
+def test_barrett_quotient_error_bound():
+    """The reference's Barrett quotient error stays within the bound that
+    sets the deviation window, for products at the top of the ModDown range
+    and random ones."""
+    chk = native_chk.load()
+    for shape in [(65536, 30, 31, 48), (4096, 4, 5, 52), (16384, 5, 5, 55), (4096, 10, 11, 45),
+                  (32768, 20, 21, 48)]:
+        assert chk.chk_barrett_delta(*shape, 30000, 13) == 0, shape
 
 
 def test_moddown_three_product_accumulation():
