@@ -173,6 +173,62 @@ constexpr int BC_GROUPS = 4;
 // mod b_j (aoa-ntt/src/cuda/manager.cu:180-223).  The per-i values are shared
 // through LDS; the sum over i is accumulated exactly in 128 bits and reduced
 // once, which equals the reference's term-by-term modular sum.
+
+// Sums over i <= level of v_i (Q_l/q_i mod b_j) for NL special limbs jl[],
+// v_i = [x_i (Q_l/q_i)^-1]_{q_i} from LDS, each reduced mod b_j first where
+// q_i > b_j (always for q0).  With v, w < 2^56 split at bit 28 and
+// vs = vl + vh, ws = wl + wh: v w = vl wl + (vs ws - vl wl - vh wh) 2^28 +
+// vh wh 2^56, three 32x32->64 multiply-adds per term into 64-bit sums that
+// cannot overflow for up to 64 terms, one 128-bit reduction per limb.
+template <int NL>
+__device__ __forceinline__ void modup_pass(const DevParams& P, const uint64_t* tl, int c, const int* jl, int level,
+                                           uint64_t* outB, size_t cid) {
+  constexpr uint64_t M28 = (1ull << 28) - 1;
+  const int L = level + 1;
+  DevPrime d[NL];
+  const uint64_t* cw[NL];
+  uint32_t red = 0;
+  uint64_t s0[NL], s2[NL], sm[NL];
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    d[n] = ld_prime(P.primes, P.k + jl[n]);
+    cw[n] = P.qhat_mod_b + ((size_t)level * P.K + jl[n]) * P.k;
+    red |= d[n].modup_reduce;
+    s0[n] = s2[n] = sm[n] = 0;
+  }
+  auto acc = [&](int n, uint64_t v, uint64_t w) {
+    const uint32_t vl = (uint32_t)(v & M28), vh = (uint32_t)(v >> 28);
+    const uint32_t wl = (uint32_t)(w & M28), wh = (uint32_t)(w >> 28);
+    s0[n] += (uint64_t)vl * wl;
+    s2[n] += (uint64_t)vh * wh;
+    sm[n] += (uint64_t)(vl + vh) * (wl + wh);
+  };
+  auto reduced = [&](int i) {
+    const uint64_t v = tl[i * BC_COEF + c];
+    const uint64_t qi = ld_prime(P.primes, i).p;
+#pragma unroll
+    for (int n = 0; n < NL; ++n)
+      acc(n, qi > d[n].p ? barrett_reduce(0, v, d[n].p, d[n].mu, (int)d[n].nb) : v, ld_u64(cw[n], i));
+  };
+  reduced(0);  // q0 is wider than every special prime
+  if (red) {
+    for (int i = 1; i < L; ++i) reduced(i);
+  } else {
+#pragma unroll 2
+    for (int i = 1; i < L; ++i) {
+      const uint64_t v = tl[i * BC_COEF + c];
+#pragma unroll
+      for (int n = 0; n < NL; ++n) acc(n, v, ld_u64(cw[n], i));
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    uint64_t h, l;
+    join_28(s0[n], sm[n] - s0[n] - s2[n], s2[n], &h, &l);
+    outB[(size_t)jl[n] * P.N + cid] = reduce128(h, l, d[n]);
+  }
+}
+
 __device__ __forceinline__ void modup_body(const DevParams& P, const uint64_t* x, uint64_t* outB, int level,
                                            int blk, uint64_t* tl) {
   const int L = level + 1;
@@ -188,54 +244,20 @@ __device__ __forceinline__ void modup_body(const DevParams& P, const uint64_t* x
     tl[i * BC_COEF + c] = mul_shoup(x[(size_t)i * N + cid], w.w, w.q, p);
   }
   __syncthreads();
-  // Two special limbs per pass over i (shared LDS reads, two independent
-  // accumulation chains).  With v, w < 2^56 split at bit 28, v*w =
-  // s0 + s1 2^28 + s2 2^56 where each partial sum over at most 64 terms fits
-  // in 64 bits: four 32x32->64 multiply-adds per term, no carries.
-  constexpr uint64_t M28 = (1ull << 28) - 1;
-  for (int j = g; j < P.K; j += 2 * BC_GROUPS) {
-    const int j2 = j + BC_GROUPS;
-    const bool two = j2 < P.K;
-    const DevPrime d1 = ld_prime(P.primes, P.k + j);
-    const DevPrime d2 = ld_prime(P.primes, P.k + (two ? j2 : j));
-    const uint64_t* c1 = P.qhat_mod_b + ((size_t)level * P.K + j) * P.k;
-    const uint64_t* c2 = P.qhat_mod_b + ((size_t)level * P.K + (two ? j2 : j)) * P.k;
-    uint64_t a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
-    auto term = [&](int i, bool reduce) {
-      const uint64_t v = tl[i * BC_COEF + c];
-      uint64_t v1 = v, v2 = v;
-      if (reduce) {
-        const uint64_t qi = ld_prime(P.primes, i).p;
-        if (qi > d1.p) v1 = barrett_reduce(0, v, d1.p, d1.mu, (int)d1.nb);
-        if (qi > d2.p) v2 = barrett_reduce(0, v, d2.p, d2.mu, (int)d2.nb);
-      }
-      const uint64_t w1 = ld_u64(c1, i), w2 = ld_u64(c2, i);
-      const uint32_t x1l = (uint32_t)(v1 & M28), x1h = (uint32_t)(v1 >> 28);
-      const uint32_t x2l = (uint32_t)(v2 & M28), x2h = (uint32_t)(v2 >> 28);
-      const uint32_t w1l = (uint32_t)(w1 & M28), w1h = (uint32_t)(w1 >> 28);
-      const uint32_t w2l = (uint32_t)(w2 & M28), w2h = (uint32_t)(w2 >> 28);
-      a0 += (uint64_t)x1l * w1l;
-      a1 += (uint64_t)x1l * w1h;
-      a1 += (uint64_t)x1h * w1l;
-      a2 += (uint64_t)x1h * w1h;
-      b0 += (uint64_t)x2l * w2l;
-      b1 += (uint64_t)x2l * w2h;
-      b1 += (uint64_t)x2h * w2l;
-      b2 += (uint64_t)x2h * w2h;
-    };
-    term(0, true);  // q0 is wider than every special prime
-    if (d1.modup_reduce | d2.modup_reduce) {
-      for (int i = 1; i < L; ++i) term(i, true);
+  // This wave's special limbs g, g+4, ... in passes of four, two or one.
+  int j = g;
+  while (j < P.K) {
+    const int left = (P.K - j + BC_GROUPS - 1) / BC_GROUPS;
+    const int jl[4] = {j, j + BC_GROUPS, j + 2 * BC_GROUPS, j + 3 * BC_GROUPS};
+    if (left >= 4) {
+      modup_pass<4>(P, tl, c, jl, level, outB, cid);
+      j += 4 * BC_GROUPS;
+    } else if (left >= 2) {
+      modup_pass<2>(P, tl, c, jl, level, outB, cid);
+      j += 2 * BC_GROUPS;
     } else {
-#pragma unroll 4
-      for (int i = 1; i < L; ++i) term(i, false);
-    }
-    uint64_t h, l;
-    join_28(a0, a1, a2, &h, &l);
-    outB[(size_t)j * N + cid] = reduce128(h, l, d1);
-    if (two) {
-      join_28(b0, b1, b2, &h, &l);
-      outB[(size_t)j2 * N + cid] = reduce128(h, l, d2);
+      modup_pass<1>(P, tl, c, jl, level, outB, cid);
+      j += BC_GROUPS;
     }
   }
 }
