@@ -40,7 +40,7 @@ struct BatchGroup {
   int n_intt3 = 0;
   TensorRowDesc* d_trow = nullptr;  // mul (N >= 8192): tensor fused into the first inverse pass
   int n_trow = 0, ns_trow = -1;
-  NttDesc* d_tcol = nullptr;  // ... and the column pass of d0, d1, d2
+  NttDesc* d_tcol = nullptr;  // ... and the column pass of d2
   int n_tcol = 0;
   ModUpDesc* d_modup = nullptr;
   int n_modup = 0;
@@ -96,10 +96,12 @@ static BatchGroup& group_for(aoa_batch* b, const aoa_params* p, const uint64_t* 
 
 // Key-switching product descriptors of one item: X (Lq + K limbs, NTT
 // domain) times both key halves into P0 / P1, one KeyDesc2 per limb for the
-// first inverse pass and one NttDesc per output limb for the second.
+// first inverse pass and one NttDesc per output limb for the second.  A0 / A1
+// (Lq limbs, NTT domain, or null) join the Q limbs of P0 / P1 times B mod q,
+// so that ModDown adds them (two-pass transforms only).
 static void add_key_descs(std::vector<KeyDesc2>& key2, std::vector<NttDesc>& keycol, const uint64_t* X,
-                          uint64_t* P0, uint64_t* P1, const uint64_t* kb, const uint64_t* ka, int Lq, int K, int k,
-                          size_t N) {
+                          uint64_t* P0, uint64_t* P1, const uint64_t* kb, const uint64_t* ka, const uint64_t* A0,
+                          const uint64_t* A1, int Lq, int K, int k, size_t N) {
   for (int m = 0; m < Lq + K; ++m) {
     const int pr = m < Lq ? m : k + (m - Lq);
     const size_t o = (size_t)m * N;
@@ -109,6 +111,8 @@ static void add_key_descs(std::vector<KeyDesc2>& key2, std::vector<NttDesc>& key
     e.dst[1] = P1 + o;
     e.key[0] = kb + (size_t)pr * N;
     e.key[1] = ka + (size_t)pr * N;
+    e.add[0] = A0 && m < Lq ? A0 + o : nullptr;
+    e.add[1] = A1 && m < Lq ? A1 + o : nullptr;
     e.prime = pr;
     e.pad = 0;
     key2.push_back(e);
@@ -196,8 +200,12 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
         uint64_t* P1 = P0 + (size_t)(Lq + K) * N;
         w += mul_ws_limbs(Lq, K) * N;
         // From N = 8192 up the transforms are two-pass and the tensor product
-        // runs inside the first inverse pass; below, a tensor kernel and the
-        // single-workgroup transforms.
+        // runs inside the first inverse pass of d2, while d0 and d1 stay in
+        // the NTT domain and join the key products as B d0, B d1: ModDown is
+        // linear in its Q limbs and its B-limb sum does not see them, so
+        // ModDown(X + B d) = ModDown(X) + d, residue for residue, and d0, d1
+        // need no inverse transform.  Below, a tensor kernel, the
+        // single-workgroup transforms, and the additions in ModDown.
         const bool fused = P.N >= 8192;
         for (int i = 0; i < Lq && fused; ++i) {
           const size_t o = (size_t)i * N;
@@ -213,7 +221,7 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
           t.prime = i;
           t.pad = 0;
           trow.push_back(t);
-          for (uint64_t* d : {D0 + o, D1 + o, D2 + o}) tcol.push_back({d, d, nullptr, i, 0});
+          tcol.push_back({D2 + o, D2 + o, nullptr, i, 0});
         }
         for (int i = 0; i < Lq && !fused; ++i) {
           const size_t o = (size_t)i * N;
@@ -235,9 +243,11 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
         mu.push_back({D2, X + (size_t)Lq * N, lv, 0});
         for (int j = 0; j < K; ++j)
           fwd.push_back({X + (size_t)(Lq + j) * N, X + (size_t)(Lq + j) * N, nullptr, k + j, 0});
-        add_key_descs(key2, keycol, X, P0, P1, g.evk_b, g.evk_a, Lq, K, k, N);
+        add_key_descs(key2, keycol, X, P0, P1, g.evk_b, g.evk_a, fused ? D0 : nullptr, fused ? D1 : nullptr, Lq,
+                      K, k, N);
         for (int y = 0; y < 2; ++y)
-          md.push_back({y ? P1 : P0, (y ? P1 : P0) + (size_t)Lq * N, y ? D1 : D0, it.out[y], lv, 1});
+          md.push_back({y ? P1 : P0, (y ? P1 : P0) + (size_t)Lq * N, fused ? nullptr : (y ? D1 : D0), it.out[y],
+                        lv, 1});
       }
       for (auto& it : g.rots) {
         const int lv = it.level, Lq = lv + 1;
@@ -258,7 +268,7 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
         mu.push_back({R1, X + (size_t)Lq * N, lv, 0});
         for (int j = 0; j < K; ++j)
           fwd.push_back({X + (size_t)(Lq + j) * N, X + (size_t)(Lq + j) * N, nullptr, k + j, 0});
-        add_key_descs(key2, keycol, X, P0, P1, it.rtk_b, it.rtk_a, Lq, K, k, N);
+        add_key_descs(key2, keycol, X, P0, P1, it.rtk_b, it.rtk_a, nullptr, nullptr, Lq, K, k, N);
         for (int y = 0; y < 2; ++y) {
           const uint64_t* add = y == 0 ? R0 : (it.accumulate ? it.in[1] : nullptr);
           md.push_back({y ? P1 : P0, (y ? P1 : P0) + (size_t)Lq * N, add, it.out[y], lv, 0});

@@ -69,9 +69,10 @@ struct NttDesc {
 };
 
 // HomMul's tensor product fused into the first inverse pass, per Q limb:
-// d0 = a0 b0, d1 = a0 b1 + a1 b0, d2 = a1 b1 (NTT domain); d2 is also written
-// to x (canonical, NTT domain: the key product's input); dst[y] receives the
-// first inverse pass of d_y (the column pass then runs in place on dst[y]).
+// d0 = a0 b0, d1 = a0 b1 + a1 b0, d2 = a1 b1 (NTT domain).  dst[0] and dst[1]
+// receive d0 and d1 (NTT domain, in [0, 2p)), x receives d2 (canonical, NTT
+// domain: the key product's input) and dst[2] the first inverse pass of d2
+// (the column pass then runs in place on dst[2]).
 struct TensorRowDesc {
   const uint64_t* a0;
   const uint64_t* a1;
@@ -84,12 +85,15 @@ struct TensorRowDesc {
 };
 
 // Key switching's inverse first pass with both key halves: dst[y] = first
-// inverse pass of (src * key[y]) for y = 0, 1 (the column pass then runs on
-// dst[0] and dst[1] as ordinary NttDesc entries).
+// inverse pass of (src * key[y] + (B mod q) add[y]) for y = 0, 1 (the column
+// pass then runs on dst[0] and dst[1] as ordinary NttDesc entries).  add is
+// null, or (Q limbs of a two-pass transform only) NTT-domain limbs whose
+// ModDown-scaled sum joins the product: ModDown(X + B d) = ModDown(X) + d.
 struct KeyDesc2 {
   const uint64_t* src;
   uint64_t* dst[2];
   const uint64_t* key[2];
+  const uint64_t* add[2];
   int prime;
   int pad;
 };

@@ -298,16 +298,19 @@ __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t*
 }
 
 // Key switching's two products from one input limb: dst0 = first inverse
-// pass of (src * key0), dst1 = same with key1.  The workgroup reads its src
-// tile twice; the second read is served by the L2 cache, so HBM sees the
-// input once while the registers hold one tile at a time.
+// pass of (src * key0 + (B mod q) add0), dst1 = same with key1 and add1 (the
+// adds may be null).  The workgroup reads its src tile twice; the second read
+// is served by the L2 cache, so HBM sees the input once while the registers
+// hold one tile at a time.
 template <int S, bool LZ>
 __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint64_t* src, uint64_t* dst0,
                                                   uint64_t* dst1, const uint64_t* key0, const uint64_t* key1,
-                                                  int pr, int tile, int g0, uint64_t* lds) {
+                                                  const uint64_t* add0, const uint64_t* add1, int pr, int tile,
+                                                  int g0, uint64_t* lds) {
   using G = Geo<S>;
   constexpr int C = G::C, TS = G::TS, PS = S + S / 16;
   const DevPrime dp = P.primes[pr];
+  const uint64_t p = dp.p;
   const uint64_t* T = P.tw_inv + (size_t)pr * P.N * 2;
   const int rl = threadIdx.x / TS, sl = threadIdx.x % TS;
   const int row = tile * C + rl;
@@ -315,23 +318,36 @@ __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint
 #pragma unroll
   for (int y = 0; y < 2; ++y) {
     const uint64_t* kp = (y ? key1 : key0) + off;
+    const uint64_t* ap = y ? add1 : add0;
     uint64_t x[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(src[off + sl + TS * k], kp[sl + TS * k], dp);
+    if (ap) {
+      // add is only set for Q limbs, whose B mod q is in the table
+      const tw2 bq = reinterpret_cast<const tw2*>(P.b_mod_q)[pr];
+      ap += off;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const uint64_t t = x[k] + mul_shoup(ap[sl + TS * k], bq.w, bq.q, p);
+        // lazy: [0, 3p) -> [0, 2p); strict: canonical sum
+        x[k] = LZ ? (t >= 2 * p ? t - 2 * p : t) : (t >= p ? t - p : t);
+      }
+    }
     if (y) __syncthreads();  // the row's LDS block is reused
-    inv_row_core<S, LZ>(P, x, (y ? dst1 : dst0) + off, dp.p, T, row, sl, g0, lds + rl * PS);
+    inv_row_core<S, LZ>(P, x, (y ? dst1 : dst0) + off, p, T, row, sl, g0, lds + rl * PS);
   }
 }
 
-// HomMul's tensor product and the first inverse pass of one of its three
-// outputs (y): three workgroups per tile, adjacent in launch order, so the
-// operand tiles they share are read once from HBM and again from the L2
-// cache.  d1 is formed as a0 b1 + a1 b0, congruent to the reference's
-// (a0+a1)(b0+b1) - d0 - d2.
+// HomMul's tensor product and the first inverse pass of d2, one workgroup per
+// tile: d0 = a0 b0 and d1 = a0 b1 + a1 b0 (congruent to the reference's
+// (a0+a1)(b0+b1) - d0 - d2) go to d0out / d1out in the NTT domain, in
+// [0, 2p); d2 = a1 b1 goes canonical to xd and through the inverse row pass
+// to dst.
 template <int S, bool LZ>
 __device__ __forceinline__ void inv_row_tensor_body(const DevParams& P, const uint64_t* a0, const uint64_t* a1,
                                                     const uint64_t* b0, const uint64_t* b1, uint64_t* xd,
-                                                    uint64_t* dst, int y, int pr, int tile, int g0, uint64_t* lds) {
+                                                    uint64_t* d0out, uint64_t* d1out, uint64_t* dst, int pr,
+                                                    int tile, int g0, uint64_t* lds) {
   using G = Geo<S>;
   constexpr int C = G::C, TS = G::TS, PS = S + S / 16;
   const DevPrime dp = P.primes[pr];
@@ -341,26 +357,20 @@ __device__ __forceinline__ void inv_row_tensor_body(const DevParams& P, const ui
   const int row = tile * C + rl;
   const size_t off = (size_t)row * S;
   uint64_t x[16];
-  if (y == 0) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(a0[off + sl + TS * k], b0[off + sl + TS * k], dp);
-  } else if (y == 1) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const size_t i = off + sl + TS * k;
-      const uint64_t t = key_mul<LZ>(a0[i], b1[i], dp) + key_mul<LZ>(a1[i], b0[i], dp);
-      // lazy: [0, 4p) -> [0, 2p); strict: canonical sum
-      x[k] = LZ ? (t >= 2 * p ? t - 2 * p : t) : (t >= p ? t - p : t);
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const size_t i = off + sl + TS * k;
-      uint64_t c = key_mul<LZ>(a1[i], b1[i], dp);
-      if (LZ) c = c >= p ? c - p : c;
-      xd[i] = c;
-      x[k] = c;
-    }
+  for (int k = 0; k < 16; ++k) {
+    const size_t i = off + sl + TS * k;
+    const uint64_t A0 = a0[i], A1 = a1[i], B0 = b0[i], B1 = b1[i];
+    uint64_t c0 = key_mul<LZ>(A0, B0, dp);
+    const uint64_t t = key_mul<LZ>(A0, B1, dp) + key_mul<LZ>(A1, B0, dp);
+    uint64_t c2 = key_mul<LZ>(A1, B1, dp);
+    // lazy: products in [0, 2p), the sum in [0, 4p) -> [0, 2p); strict: canonical
+    const uint64_t c1 = LZ ? (t >= 2 * p ? t - 2 * p : t) : (t >= p ? t - p : t);
+    if (LZ) c2 = c2 >= p ? c2 - p : c2;
+    d0out[i] = c0;
+    d1out[i] = c1;
+    xd[i] = c2;
+    x[k] = c2;
   }
   inv_row_core<S, LZ>(P, x, dst + off, p, T, row, sl, g0, lds + rl * PS);
 }
@@ -535,37 +545,26 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const K
   const KeyDesc2& e = d[blockIdx.x / tiles];
   const uint64_t* src = e.src;
   uint64_t *d0 = e.dst[0], *d1 = e.dst[1];
-  const uint64_t *k0 = e.key[0], *k1 = e.key[1];
+  const uint64_t *k0 = e.key[0], *k1 = e.key[1], *a0 = e.add[0], *a1 = e.add[1];
   const int pr = e.prime;
   if (use_lazy<MODE>(P, pr))
-    inv_row_key2_body<S, true>(P, src, d0, d1, k0, k1, pr, blockIdx.x % tiles, g0, lds);
+    inv_row_key2_body<S, true>(P, src, d0, d1, k0, k1, a0, a1, pr, blockIdx.x % tiles, g0, lds);
   else if (MODE != 1)
-    inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, pr, blockIdx.x % tiles, g0, lds);
+    inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, a0, a1, pr, blockIdx.x % tiles, g0, lds);
 }
 
-// Workgroups are dispatched round-robin over the chip's 8 XCDs (workgroup g
-// runs on XCD g % 8), each with its own L2.  The three workgroups of a tile
-// take ids g, g+8 and g+16, so they share an XCD and the operand tiles they
-// all read stay in one L2.  `ntiles` = descriptors x tiles per limb; the grid
-// is rounded up to a multiple of 24.
 template <int S, int MODE>
-__global__ __launch_bounds__(256) void k_ntt_inv_row_tensor_d(DevParams P, const TensorRowDesc* d, int g0,
-                                                              int ntiles) {
+__global__ __launch_bounds__(256) void k_ntt_inv_row_tensor_d(DevParams P, const TensorRowDesc* d, int g0) {
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
   const int tiles = (P.N / S) / Geo<S>::C;
-  const int slot = blockIdx.x >> 3;
-  const int b = (slot / 3) * 8 + (blockIdx.x & 7);
-  const int y = slot % 3;
-  if (b >= ntiles) return;
-  const TensorRowDesc& e = d[b / tiles];
+  const TensorRowDesc& e = d[blockIdx.x / tiles];
   const uint64_t *a0 = e.a0, *a1 = e.a1, *b0 = e.b0, *b1 = e.b1;
-  uint64_t* xd = e.x;
-  uint64_t* dst = y == 0 ? e.dst[0] : (y == 1 ? e.dst[1] : e.dst[2]);
+  uint64_t *xd = e.x, *o0 = e.dst[0], *o1 = e.dst[1], *o2 = e.dst[2];
   const int pr = e.prime;
   if (use_lazy<MODE>(P, pr))
-    inv_row_tensor_body<S, true>(P, a0, a1, b0, b1, xd, dst, y, pr, b % tiles, g0, lds);
+    inv_row_tensor_body<S, true>(P, a0, a1, b0, b1, xd, o0, o1, o2, pr, blockIdx.x % tiles, g0, lds);
   else if (MODE != 1)
-    inv_row_tensor_body<S, false>(P, a0, a1, b0, b1, xd, dst, y, pr, b % tiles, g0, lds);
+    inv_row_tensor_body<S, false>(P, a0, a1, b0, b1, xd, o0, o1, o2, pr, blockIdx.x % tiles, g0, lds);
 }
 
 template <int S, int MODE>
@@ -991,16 +990,7 @@ int ntt_inverse_tensor_desc(const DevParams& P, const TensorRowDesc* dt, int nt,
   const int N2 = P.N / N1, g0 = log2i(N1);
   {
     const int tiles = N1 / (4096 / N2);
-    auto grid = [&](int n) { return dim3((unsigned)((3 * tiles * n + 23) / 24 * 24)); };
-    const int nlazy = ntstrict < 0 ? 0 : nt - ntstrict;
-#define CALLR(S)                                                                                          \
-  if (ntstrict < 0) {                                                                                     \
-    k_ntt_inv_row_tensor_d<S, 0><<<grid(nt), 256, 0, st>>>(P, dt, g0, tiles * nt);                       \
-  } else {                                                                                                \
-    if (ntstrict > 0) k_ntt_inv_row_tensor_d<S, 2><<<grid(ntstrict), 256, 0, st>>>(P, dt, g0, tiles * ntstrict); \
-    if (nlazy > 0)                                                                                        \
-      k_ntt_inv_row_tensor_d<S, 1><<<grid(nlazy), 256, 0, st>>>(P, dt + ntstrict, g0, tiles * nlazy);      \
-  }
+#define CALLR(S) AOA_SPLIT_LAUNCH(k_ntt_inv_row_tensor_d, S, tiles, dt, nt, ntstrict, g0)
     AOA_DISPATCH_S(N2, CALLR)
 #undef CALLR
   }
