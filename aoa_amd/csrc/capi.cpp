@@ -421,18 +421,31 @@ static int hommul_core(const aoa_params* h, void* stream, uint64_t* c3_0, uint64
   uint64_t* D2 = D1 + (size_t)L * N;       // L: d2 in the coefficient domain
   uint64_t* P0 = D2 + (size_t)L * N;       // L+K
   uint64_t* P1 = P0 + (size_t)(L + K) * N; // L+K
+  // With two-pass transforms d0 and d1 stay in the NTT domain and join the
+  // key products as (B mod q) d0, (B mod q) d1: ModDown is linear in its Q
+  // limbs and its special-limb sum does not see them, so
+  // ModDown(X + B d) = ModDown(X) + d residue for residue, and only d2 is
+  // transformed back.  The single-workgroup transform (N <= 4096) keeps the
+  // additions in ModDown.
+  const bool fold = (P.ntt_n1 ? P.ntt_n1 : ntt_default_n1(P.N)) != P.N;
   launch_tensor(P, D0, D1, X, a0, a1, b0, b1, L, st);
   LimbMap mi;
   std::memset(&mi, 0, sizeof(mi));
-  mi.base[0] = D0;
-  mi.base[1] = D1;
-  mi.base[2] = D2;
-  mi.src[0] = D0;
-  mi.src[1] = D1;
-  mi.src[2] = X;
+  if (fold) {
+    mi.base[0] = D2;
+    mi.src[0] = X;
+    mi.npolys = 1;
+  } else {
+    mi.base[0] = D0;
+    mi.base[1] = D1;
+    mi.base[2] = D2;
+    mi.src[0] = D0;
+    mi.src[1] = D1;
+    mi.src[2] = X;
+    mi.npolys = 3;
+  }
   mi.nlimbs = L;
   mi.qlimbs = L;
-  mi.npolys = 3;
   if (ntt_inverse(P, mi, 0, nullptr, nullptr, st)) return fail(AOA_ERR_UNSUPPORTED, "no NTT split");
   launch_modup(P, D2, X + (size_t)L * N, level, st);
   LimbMap mb;
@@ -450,15 +463,16 @@ static int hommul_core(const aoa_params* h, void* stream, uint64_t* c3_0, uint64
   mk.nlimbs = L + K;
   mk.qlimbs = L;
   mk.npolys = 2;
-  if (ntt_inverse(P, mk, 0, evk_b, evk_a, st)) return fail(AOA_ERR_UNSUPPORTED, "no NTT split");
+  if (ntt_inverse(P, mk, 0, evk_b, evk_a, st, fold ? D0 : nullptr, fold ? D1 : nullptr))
+    return fail(AOA_ERR_UNSUPPORTED, "no NTT split");
   ModDownArgs a;
   std::memset(&a, 0, sizeof(a));
   a.inQ[0] = P0;
   a.inQ[1] = P1;
   a.inB[0] = P0 + (size_t)L * N;
   a.inB[1] = P1 + (size_t)L * N;
-  a.add[0] = D0;
-  a.add[1] = D1;
+  a.add[0] = fold ? nullptr : D0;
+  a.add[1] = fold ? nullptr : D1;
   a.out[0] = c3_0;
   a.out[1] = c3_1;
   a.npolys = 2;

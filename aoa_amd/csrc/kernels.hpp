@@ -17,10 +17,12 @@ int ntt_default_n1(int N);
 bool ntt_valid_split(int N, int N1);
 // Forward / inverse NTT of every limb in `m`.  N1 = 0 picks the default split.
 // With key0/key1 set, the inverse multiplies limb li of m.src[0] by limb
-// prime_of(li) of key y before transforming into m.base[y] (y = 0, 1).
+// prime_of(li) of key y before transforming into m.base[y] (y = 0, 1); with
+// add0/add1 also set (two-pass splits only; -1 otherwise), (B mod q) times
+// limb li of add y joins the product of each Q limb (li < m.qlimbs).
 int ntt_forward(const DevParams& P, LimbMap m, int N1, hipStream_t st);
 int ntt_inverse(const DevParams& P, LimbMap m, int N1, const uint64_t* key0, const uint64_t* key1,
-                hipStream_t st);
+                hipStream_t st, const uint64_t* add0 = nullptr, const uint64_t* add1 = nullptr);
 
 // out = a op b (out = a*b + c for MULADD) over nlimbs limbs mapped to primes
 // as in LimbMap (qlimbs Q limbs from prime0, then special limbs).

@@ -274,11 +274,27 @@ __device__ __forceinline__ uint64_t key_mul(uint64_t x, uint64_t k, const DevPri
   return LZ ? mul_barrett62_lazy(x, k, dp.p, dp.mu2, (int)dp.nb) : mul_barrett(x, k, dp.p, dp.mu, (int)dp.nb);
 }
 
+// x (products in [0, 2p) when LZ, else canonical) plus (B mod q) times the
+// NTT-domain addend of a Q limb pr, left in the same range.
+template <bool LZ>
+__device__ __forceinline__ void add_bscaled(uint64_t (&x)[16], const uint64_t* ap, int stride, const DevParams& P,
+                                            int pr, uint64_t p) {
+  const tw2 bq = reinterpret_cast<const tw2*>(P.b_mod_q)[pr];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint64_t t = x[k] + mul_shoup(ap[stride * k], bq.w, bq.q, p);
+    // lazy: [0, 3p) -> [0, 2p); strict: canonical sum
+    x[k] = LZ ? (t >= 2 * p ? t - 2 * p : t) : (t >= p ? t - p : t);
+  }
+}
+
 // Row pass of the inverse: global stages g0+log2(S)-1 .. g0.  With a key, the
-// input limb is first multiplied by the key limb.
+// input limb is first multiplied by the key limb, and then (if add is set,
+// for Q limbs) (B mod q) times the add limb joins the product.
 template <int S, bool LZ>
 __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t* src, uint64_t* dst,
-                                             const uint64_t* key, int pr, int tile, int g0, uint64_t* lds) {
+                                             const uint64_t* key, const uint64_t* add, int pr, int tile, int g0,
+                                             uint64_t* lds) {
   using G = Geo<S>;
   constexpr int C = G::C, TS = G::TS, PS = S + S / 16;
   const DevPrime dp = P.primes[pr];
@@ -293,6 +309,7 @@ __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t*
     const uint64_t* kp = key + (size_t)row * S;
 #pragma unroll
     for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(x[k], kp[sl + TS * k], dp);
+    if (add) add_bscaled<LZ>(x, add + (size_t)row * S + sl, TS, P, pr, dp.p);
   }
   inv_row_core<S, LZ>(P, x, dst + (size_t)row * S, dp.p, T, row, sl, g0, lds + rl * PS);
 }
@@ -322,17 +339,7 @@ __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint
     uint64_t x[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(src[off + sl + TS * k], kp[sl + TS * k], dp);
-    if (ap) {
-      // add is only set for Q limbs, whose B mod q is in the table
-      const tw2 bq = reinterpret_cast<const tw2*>(P.b_mod_q)[pr];
-      ap += off;
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const uint64_t t = x[k] + mul_shoup(ap[sl + TS * k], bq.w, bq.q, p);
-        // lazy: [0, 3p) -> [0, 2p); strict: canonical sum
-        x[k] = LZ ? (t >= 2 * p ? t - 2 * p : t) : (t >= p ? t - p : t);
-      }
-    }
+    if (ap) add_bscaled<LZ>(x, ap + off + sl, TS, P, pr, p);  // set for Q limbs only
     if (y) __syncthreads();  // the row's LDS block is reused
     inv_row_core<S, LZ>(P, x, (y ? dst1 : dst0) + off, p, T, row, sl, g0, lds + rl * PS);
   }
@@ -466,7 +473,8 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(DevParams P, LimbMap m, int
 
 template <int S>
 __global__ __launch_bounds__(256) void k_ntt_inv_row(DevParams P, LimbMap m, int g0, const uint64_t* key0,
-                                                      const uint64_t* key1) {
+                                                      const uint64_t* key1, const uint64_t* add0,
+                                                      const uint64_t* add1) {
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
   const int tiles = (P.N / S) / Geo<S>::C;
   const int li = blockIdx.x / tiles;
@@ -474,11 +482,13 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row(DevParams P, LimbMap m, int
   const size_t off = (size_t)li * P.N;
   const int pr = limb_prime(m, P.k, li);
   const uint64_t* key = key0 ? (y == 0 ? key0 : key1) + (size_t)pr * P.N : nullptr;
+  const uint64_t* addy = y == 0 ? add0 : add1;
+  const uint64_t* add = addy && li < m.qlimbs ? addy + off : nullptr;
   const uint64_t* src = (key0 ? m.src[0] : m.src[y]) + off;
   if (lazy_prime(P, pr))
-    inv_row_body<S, true>(P, src, m.base[y] + off, key, pr, blockIdx.x % tiles, g0, lds);
+    inv_row_body<S, true>(P, src, m.base[y] + off, key, add, pr, blockIdx.x % tiles, g0, lds);
   else
-    inv_row_body<S, false>(P, src, m.base[y] + off, key, pr, blockIdx.x % tiles, g0, lds);
+    inv_row_body<S, false>(P, src, m.base[y] + off, key, add, pr, blockIdx.x % tiles, g0, lds);
 }
 
 template <int S>
@@ -533,9 +543,9 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_d(DevParams P, const NttDes
   const int tiles = (P.N / S) / Geo<S>::C;
   const NttDesc e = d[blockIdx.x / tiles];
   if (use_lazy<MODE>(P, e.prime))
-    inv_row_body<S, true>(P, e.src, e.dst, e.key, e.prime, blockIdx.x % tiles, g0, lds);
+    inv_row_body<S, true>(P, e.src, e.dst, e.key, nullptr, e.prime, blockIdx.x % tiles, g0, lds);
   else if (MODE != 1)
-    inv_row_body<S, false>(P, e.src, e.dst, e.key, e.prime, blockIdx.x % tiles, g0, lds);
+    inv_row_body<S, false>(P, e.src, e.dst, e.key, nullptr, e.prime, blockIdx.x % tiles, g0, lds);
 }
 
 template <int S, int MODE>
@@ -861,9 +871,10 @@ int ntt_forward(const DevParams& P, LimbMap m, int N1, hipStream_t st) {
 }
 
 int ntt_inverse(const DevParams& P, LimbMap m, int N1, const uint64_t* key0, const uint64_t* key1,
-                hipStream_t st) {
+                hipStream_t st, const uint64_t* add0, const uint64_t* add1) {
   N1 = pick_n1(P, N1);
   if (!N1) return -1;
+  if ((add0 || add1) && (N1 == P.N || !key0)) return -1;  // adds: two-pass key products only
   const int N2 = P.N / N1, g0 = log2i(N1);
   for (int y = 0; y < m.npolys; ++y)
     if (!m.src[y]) m.src[y] = m.base[y];
@@ -877,7 +888,7 @@ int ntt_inverse(const DevParams& P, LimbMap m, int N1, const uint64_t* key0, con
   {
     const int tiles = N1 / (4096 / N2);
     dim3 grid(tiles * m.nlimbs, m.npolys);
-#define CALLR(S) k_ntt_inv_row<S><<<grid, 256, 0, st>>>(P, m, g0, key0, key1)
+#define CALLR(S) k_ntt_inv_row<S><<<grid, 256, 0, st>>>(P, m, g0, key0, key1, add0, add1)
     AOA_DISPATCH_S(N2, CALLR)
 #undef CALLR
   }
