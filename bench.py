@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # `bench.py --profile` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0; the factors make a
 # 1 GiB device copy read 1 GiB and write 1 GiB); the run is summarised in
 # profiles/moddown_pmc_traffic_n16_k30.md.
-MODDOWN_TRAFFIC_BYTES = {32: 4032248853}
+MODDOWN_TRAFFIC_BYTES = {32: 3024239242}
 
 
 def op_bytes(N, L, K):
@@ -51,11 +51,12 @@ def rot_bytes(N, L, K):
     return 8 * N * (6 * L + 2 * K)
 
 
-def moddown_bytes(N, L, K, rescale):
-    """One ModDown of a QB polynomial with the HomMul epilogue fused: reads
-    L + K input limbs and the L-limb addend, writes L (or L-1 after the fused
-    rescale) limbs."""
-    return 8 * N * (L + K + L + (L - 1 if rescale else L))
+def moddown_bytes(N, L, K, rescale, addend=False):
+    """One ModDown of a QB polynomial: reads L + K input limbs (and an L-limb
+    addend if it adds one), writes L (or L-1 after the fused rescale) limbs.
+    HomMul with two-pass transforms folds its additions into the key
+    products, so its ModDown reads no addend."""
+    return 8 * N * (L + K + (L if addend else 0) + (L - 1 if rescale else L))
 
 
 def ntt_bytes(N, R):
@@ -475,7 +476,7 @@ def main():
         "wall_s": round(wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(md_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(md_gbs / HBM_PEAK_GBS, 4), "traffic": MODDOWN_TRAFFIC_BYTES.get(B),
-                     "kernel": "aoa::k_moddown_d (ModDown + rescale + HomMul additions, %d items x 2 polys)" % B,
+                     "kernel": "aoa::k_moddown_d (ModDown + rescale, %d items x 2 polys)" % B,
                      "launch_ms": round(md_launch_ms, 4), "algorithmic_bytes": md_bytes},
     }
 

@@ -299,11 +299,11 @@ enum {
   AOA_STAGE_PRE_INV = 1,      /* inverse NTT of inputs given in the NTT domain */
   AOA_STAGE_TENSOR = 2,       /* d0, d1, d2 of HomMul (N <= 4096; from N = 8192 up inside AOA_STAGE_INTT) */
   AOA_STAGE_AUTOMORPHISM = 3, /* rotation's Galois map */
-  AOA_STAGE_INTT = 4,         /* inverse NTT of d0, d1, d2 (from N = 8192 up with the tensor product) */
+  AOA_STAGE_INTT = 4,         /* inverse NTT of d0, d1, d2 (from N = 8192 up: of d2 only, with the tensor product) */
   AOA_STAGE_MODUP = 5,
   AOA_STAGE_NTT = 6,          /* forward NTT of the ModUp output */
   AOA_STAGE_KEY_INTT = 7,     /* key product fused into the inverse NTT */
-  AOA_STAGE_MODDOWN = 8,      /* ModDown (+ rescale, + additions) */
+  AOA_STAGE_MODDOWN = 8,      /* ModDown (+ rescale, + the additions of rotations and of HomMul below N = 8192) */
   AOA_NSTAGES = 9
 };
 
