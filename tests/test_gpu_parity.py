@@ -230,6 +230,29 @@ def test_hommul_parity(gpu, shape):
         assert np.array_equal(P.to_host(out[1])[:n], ref1[:n]), "c1 level %d" % level
 
 
+@pytest.mark.parametrize("n1", [64, 4096])
+def test_hommul_split_parity(gpu, n1):
+    """HomMul at N=4096 with the two-pass split (the additions of d0 and d1
+    join the key products before the inverse NTT) and with the
+    single-workgroup transform (the additions stay in ModDown): both give the
+    oracle's residues, at the top level and at level 1."""
+    P, O = _params(4096, 4, 5, 52)
+    P.set_ntt_split(n1)
+    rng = np.random.default_rng(17)
+    k, kl, N = O.k, O.kl, O.N
+    for level in (1, k - 1):
+        c1 = [O.random_poly(rng, k) for _ in range(2)]
+        c2 = [O.random_poly(rng, k) for _ in range(2)]
+        evk = [O.random_poly(rng, k + kl) for _ in range(2)]
+        ref0, ref1 = O.hommul(c1, c2, evk, level)
+        d = [P.to_device(x) for x in c1 + c2 + evk]
+        out = [P.zeros(k), P.zeros(k)]
+        P.hommul(out, (d[0], d[1]), (d[2], d[3]), (d[4], d[5]), level, P.workspace(level))
+        n = level * N
+        assert np.array_equal(P.to_host(out[0])[:n], ref0[:n]), "c0 split %d level %d" % (n1, level)
+        assert np.array_equal(P.to_host(out[1])[:n], ref1[:n]), "c1 split %d level %d" % (n1, level)
+
+
 @pytest.mark.parametrize("shape", SMALL[:3])
 def test_rotate_parity(gpu, shape):
     P, O = _params(*shape)
