@@ -33,6 +33,9 @@ SIGNATURES = {
     "aoa_sample": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                   ctypes.c_uint64]),
     "aoa_rescale": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int]),
+    "aoa_encrypt": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_uint64, ctypes.c_uint64,
+                                   c_vp, ctypes.c_size_t]),
+    "aoa_decrypt": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int]),
     "aoa_modup": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int]),
     "aoa_moddown": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int]),
     "aoa_automorphism": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int, ctypes.c_int]),

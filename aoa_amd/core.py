@@ -108,6 +108,20 @@ class Params:
         """vals (float64 device buffer, 2*slots) <- slot values of limb 0 of inp (coefficient domain)."""
         check(self.lib.aoa_decode(self.h, _stream(stream), _ptr(vals), _ptr(inp), slots, int(scale)))
 
+    def encrypt(self, c0, c1, m, pk_b, pk_a, seed, nonce, ws, stream=None):
+        """Public-key encryption of the plaintext m (coefficient domain, Q
+        limbs; left in the NTT domain): u (ZO), e0, e1 (Gaussian) drawn at
+        nonces nonce, nonce+1, nonce+2; c0 = pk_b u + e0 + m, c1 = pk_a u + e1
+        in the NTT domain.  ws: at least 3 k N words."""
+        w = _ptr(ws)
+        check(self.lib.aoa_encrypt(self.h, _stream(stream), _ptr(c0), _ptr(c1), _ptr(m), _ptr(pk_b), _ptr(pk_a),
+                                   int(seed), int(nonce), w, ws.numel() * ws.element_size()))
+
+    def decrypt(self, m, c0, c1, s, level, stream=None):
+        """m = INTT(c1 s + c0) on limbs 0..level, with c0 and c1 transformed
+        to the NTT domain in place; s is the NTT-domain secret key."""
+        check(self.lib.aoa_decrypt(self.h, _stream(stream), _ptr(m), _ptr(c0), _ptr(c1), _ptr(s), level))
+
     def sample(self, out, kind, seed, nonce, res0=0, nres=None, stream=None):
         """Limbs res0..res0+nres-1 of out drawn from distribution `kind`
         (SAMPLE_UNIFORM, _GAUSSIAN, _ZO, _HW, _NARROW) keyed by (seed, nonce)."""

@@ -288,6 +288,56 @@ extern "C" int aoa_sample(const aoa_params* h, void* stream, uint64_t* out, int 
   return check_launch("aoa_sample");
 }
 
+extern "C" int aoa_encrypt(const aoa_params* h, void* stream, uint64_t* c0, uint64_t* c1, uint64_t* m,
+                           const uint64_t* pk_b, const uint64_t* pk_a, uint64_t seed, uint64_t nonce, void* ws,
+                           size_t ws_bytes) {
+  if (!h || !c0 || !c1 || !m || !pk_b || !pk_a || !ws) return fail(AOA_ERR_ARG, "null argument");
+  const int k = h->host.k;
+  const size_t N = (size_t)h->host.N;
+  if (ws_bytes < 3 * (size_t)k * N * sizeof(uint64_t)) return fail(AOA_ERR_WORKSPACE, "workspace too small");
+  const DevParams& P = h->dev;
+  hipStream_t st = (hipStream_t)stream;
+  uint64_t* u = (uint64_t*)ws;
+  uint64_t* e0 = u + (size_t)k * N;
+  uint64_t* e1 = e0 + (size_t)k * N;
+  launch_sample(P, u, 0, k, SAMPLE_ZO, seed, nonce, st);
+  launch_sample(P, e0, 0, k, SAMPLE_GAUSSIAN, seed, nonce + 1, st);
+  launch_sample(P, e1, 0, k, SAMPLE_GAUSSIAN, seed, nonce + 2, st);
+  LimbMap mm;
+  std::memset(&mm, 0, sizeof(mm));
+  mm.base[0] = m;
+  mm.base[1] = u;
+  mm.base[2] = e0;
+  mm.base[3] = e1;
+  mm.nlimbs = k;
+  mm.qlimbs = k;
+  mm.npolys = 4;
+  if (ntt_forward(P, mm, 0, st)) return fail(AOA_ERR_UNSUPPORTED, "no NTT split for this N");
+  launch_encrypt_combine(P, c0, c1, pk_b, pk_a, u, e0, e1, m, k, st);
+  return check_launch("aoa_encrypt");
+}
+
+extern "C" int aoa_decrypt(const aoa_params* h, void* stream, uint64_t* m, uint64_t* c0, uint64_t* c1,
+                           const uint64_t* s, int level) {
+  if (!h || !m || !c0 || !c1 || !s) return fail(AOA_ERR_ARG, "null argument");
+  if (level < 0 || level >= h->host.k) return fail(AOA_ERR_ARG, "level out of range");
+  const DevParams& P = h->dev;
+  hipStream_t st = (hipStream_t)stream;
+  const int L = level + 1;
+  LimbMap mc;
+  std::memset(&mc, 0, sizeof(mc));
+  mc.base[0] = c0;
+  mc.base[1] = c1;
+  mc.nlimbs = L;
+  mc.qlimbs = L;
+  mc.npolys = 2;
+  if (ntt_forward(P, mc, 0, st)) return fail(AOA_ERR_UNSUPPORTED, "no NTT split for this N");
+  launch_eltwise(P, ELT_MULADD, m, c1, s, c0, L, L, 0, st);
+  LimbMap mo = limbs_of(m, 0, L, h->host.k);
+  if (ntt_inverse(P, mo, 0, nullptr, nullptr, st)) return fail(AOA_ERR_UNSUPPORTED, "no NTT split for this N");
+  return check_launch("aoa_decrypt");
+}
+
 extern "C" int aoa_rescale(const aoa_params* h, void* stream, uint64_t* c0, uint64_t* c1, int level) {
   if (!h || !c0 || !c1) return fail(AOA_ERR_ARG, "null argument");
   if (level == 0) return fail(AOA_ERR_LEVEL, "The ciphertext level is 0. No more rescaling is possible.");

@@ -26,6 +26,9 @@ int ntt_inverse(const DevParams& P, LimbMap m, int N1, const uint64_t* key0, con
 
 // out = a op b (out = a*b + c for MULADD) over nlimbs limbs mapped to primes
 // as in LimbMap (qlimbs Q limbs from prime0, then special limbs).
+void launch_encrypt_combine(const DevParams& P, uint64_t* c0, uint64_t* c1, const uint64_t* pkb,
+                            const uint64_t* pka, const uint64_t* u, const uint64_t* e0, const uint64_t* e1,
+                            const uint64_t* m, int nlimbs, hipStream_t st);
 void launch_eltwise(const DevParams& P, int op, uint64_t* out, const uint64_t* a, const uint64_t* b,
                     const uint64_t* c, int nlimbs, int qlimbs, int prime0, hipStream_t st);
 

@@ -93,6 +93,27 @@ void launch_eltwise_scalar(const DevParams& P, int op, uint64_t* out, const uint
   k_eltwise_scalar<<<dim3((unsigned)(total / 512)), 256, 0, st>>>(P, op, out, a, b, prime0);
 }
 
+// Public-key encryption's combination (aoa-ntt/src/ckks/ckks.cpp:87-105),
+// NTT domain, limbs 0..nlimbs-1: c0 = pk_b u + e0 + m, c1 = pk_a u + e1.
+__global__ __launch_bounds__(256) void k_encrypt_combine(DevParams P, uint64_t* c0, uint64_t* c1,
+                                                         const uint64_t* pkb, const uint64_t* pka,
+                                                         const uint64_t* u, const uint64_t* e0,
+                                                         const uint64_t* e1, const uint64_t* m) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const DevPrime d = P.primes[(int)(e >> P.logN)];
+  const uint64_t uu = u[e];
+  const uint64_t b = add_mod(mul_var(pkb[e], uu, d.p, d.mu, d.mu2, (int)d.nb), e0[e], d.p);
+  c0[e] = add_mod(b, m[e], d.p);
+  c1[e] = add_mod(mul_var(pka[e], uu, d.p, d.mu, d.mu2, (int)d.nb), e1[e], d.p);
+}
+
+void launch_encrypt_combine(const DevParams& P, uint64_t* c0, uint64_t* c1, const uint64_t* pkb,
+                            const uint64_t* pka, const uint64_t* u, const uint64_t* e0, const uint64_t* e1,
+                            const uint64_t* m, int nlimbs, hipStream_t st) {
+  const size_t total = (size_t)nlimbs * P.N;
+  k_encrypt_combine<<<dim3((unsigned)(total / 256)), 256, 0, st>>>(P, c0, c1, pkb, pka, u, e0, e1, m);
+}
+
 // Conjugation in the NTT domain: out[cid] = in[N-1-cid] per limb
 // (aoa-ntt/src/ckks/encoder.cu:391-401).  With the bit-reversed evaluation
 // order, reversing the index maps the evaluation at psi^e to psi^-e, i.e. the

@@ -4,15 +4,16 @@ decode.
 
 The sequence is the one the newckks API runs (ckks_keygen, ckks_encrypt,
 ckks_mul, ckks_decrypt: aoa-ntt/src/ckks/ckkskeys.cpp:61-131,
-aoa-ntt/src/ckks/ckks.cpp:87-132, :273-358), written out call by call so that
-it can be replayed as one HIP graph.  Keys live in the NTT domain: the secret
+aoa-ntt/src/ckks/ckks.cpp:87-132, :273-358), written out call by call (keys
+from aoa_sample / aoa_ntt / aoa_eltwise, then aoa_encode, aoa_encrypt,
+aoa_hommul_relin_rescale, aoa_decrypt, aoa_decode) so that it can be
+replayed as one HIP graph.  Keys live in the NTT domain: the secret
 s on the QB limbs, the public key (b = e - a s) on the Q limbs, the
 relinearisation key (b = rho(s^2) + e - a s) on the QB limbs.  aoa_sample
 nonces: s 0, public a 1, public e 2, relinearisation a 3 and e 4; an
 encryption at nonce n draws u (ZO) at n, e0 at n+1 and e1 at n+2.
 """
-from .core import (ADD, FORWARD, INVERSE, MUL, MULADD, SAMPLE_GAUSSIAN, SAMPLE_HW, SAMPLE_UNIFORM, SAMPLE_ZO,
-                   SUB)
+from .core import ADD, FORWARD, MUL, SAMPLE_GAUSSIAN, SAMPLE_HW, SAMPLE_UNIFORM, SUB
 
 
 class Pipeline:
@@ -53,7 +54,8 @@ class Pipeline:
         self.z = [f64(), f64()]      # inputs (complex, re/im interleaved)
         self.v = [f64(), f64()]      # encoder work buffers
         self.out = f64()             # decoded product
-        self.m, self.u, self.e0, self.e1, self.t = z(k), z(k), z(k), z(k), z(k)
+        self.m, self.t = z(k), z(k)
+        self.ews = z(3 * k)          # encryption's u, e0, e1
         self.c = [[z(k), z(k)], [z(k), z(k)]]
         self.c3 = [z(k), z(k)]
         self.ws = P.workspace(k - 1)
@@ -70,19 +72,11 @@ class Pipeline:
         self.scale_out = self.scale * self.scale // q
 
     def _encrypt(self, j, nonce):
-        P, k, seed = self.P, self.P.k, self.seed
+        P, k = self.P, self.P.k
         self.v[j].copy_(self.z[j])
         P.encode(self.m, self.v[j], self.slots, self.scale, nres=k)
-        P.ntt(self.m, 0, k, FORWARD)
-        P.sample(self.u, SAMPLE_ZO, seed, nonce, 0, k)
-        P.sample(self.e0, SAMPLE_GAUSSIAN, seed, nonce + 1, 0, k)
-        P.sample(self.e1, SAMPLE_GAUSSIAN, seed, nonce + 2, 0, k)
-        for x in (self.u, self.e0, self.e1):
-            P.ntt(x, 0, k, FORWARD)
-        c0, c1 = self.c[j]
-        P.eltwise(MULADD, c0, self.pkb, self.u, self.e0, res0=0, nres=k)
-        P.eltwise(ADD, c0, c0, self.m, res0=0, nres=k)
-        P.eltwise(MULADD, c1, self.pka, self.u, self.e1, res0=0, nres=k)
+        # u (ZO), e0, e1 (Gaussian) at nonce, nonce+1, nonce+2; c0 = pk_b u + e0 + m, c1 = pk_a u + e1
+        P.encrypt(self.c[j][0], self.c[j][1], self.m, self.pkb, self.pka, self.seed, nonce, self.ews)
 
     def run(self, nonce):
         """Encrypt both inputs (nonces `nonce` and `nonce + 3`), multiply,
@@ -92,11 +86,7 @@ class Pipeline:
         self._encrypt(0, nonce)
         self._encrypt(1, nonce + 3)
         P.hommul(self.c3, self.c[0], self.c[1], (self.evkb, self.evka), k - 1, self.ws)
-        c0, c1 = self.c3
-        P.ntt(c0, 0, L, FORWARD)
-        P.ntt(c1, 0, L, FORWARD)
-        P.eltwise(MULADD, self.t, c1, self.s, c0, res0=0, nres=L)
-        P.ntt(self.t, 0, L, INVERSE)
+        P.decrypt(self.t, self.c3[0], self.c3[1], self.s, L - 1)
         P.decode(self.out, self.t, self.slots, self.scale_out)
 
     def result(self):

@@ -131,6 +131,24 @@ int aoa_decode(const aoa_params* params, void* stream, double* vals, const uint6
  * sample_hw and sample_narrow (aoa-ntt/include/newckks/cuda/sampler.h:109-119). */
 enum { AOA_SAMPLE_UNIFORM = 0, AOA_SAMPLE_GAUSSIAN = 1, AOA_SAMPLE_ZO = 2, AOA_SAMPLE_HW = 3, AOA_SAMPLE_NARROW = 4 };
 
+/* Public-key encryption of one plaintext (ckks_encrypt_poly,
+ * aoa-ntt/src/ckks/ckks.cpp:87-105): m holds the plaintext on the k Q limbs in
+ * the coefficient domain and is left in the NTT domain; u (ZO), e0 and e1
+ * (Gaussian) are drawn with aoa_sample's generator at nonces nonce, nonce+1
+ * and nonce+2 on limbs 0..k-1; then c0 = pk_b u + e0 + m and
+ * c1 = pk_a u + e1 in the NTT domain.  The same as aoa_sample, aoa_ntt and
+ * aoa_eltwise called in that order, in five launches.  ws: at least
+ * 3 k N 8 bytes (u, e0, e1). */
+int aoa_encrypt(const aoa_params* params, void* stream, uint64_t* c0, uint64_t* c1, uint64_t* m,
+                const uint64_t* pk_b, const uint64_t* pk_a, uint64_t seed, uint64_t nonce, void* ws, size_t ws_bytes);
+
+/* Decryption to the plaintext polynomial (ckks_decrypt_poly,
+ * aoa-ntt/src/ckks/ckks.cpp:179-192): c0 and c1 (coefficient domain, limbs
+ * 0..level) are transformed to the NTT domain in place, then
+ * m = INTT(c1 s + c0) on limbs 0..level, s being the NTT-domain secret key. */
+int aoa_decrypt(const aoa_params* params, void* stream, uint64_t* m, uint64_t* c0, uint64_t* c1,
+                const uint64_t* s, int level);
+
 /* Draw limbs res0 .. res0+nres-1 of the polynomial at `out` (limb r at
  * out + r*N, prime r) on the device.  Replaces the Sampler methods
  * (aoa-ntt/src/cuda/sampler.cu:21-225), which draw from cuRAND and host rand().

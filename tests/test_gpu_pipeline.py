@@ -77,3 +77,51 @@ def test_pipeline_graph_replays(gpu):
         assert all(np.array_equal(P.to_host(t), c) for t, c in zip(pipe.c3, c3)), rep
         pipe.run(7)  # eager work between replays
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("N,k,kl,prec", [(4096, 4, 5, 52), (16384, 5, 5, 55)])
+def test_encrypt_decrypt_entry_points_match_primitives(gpu, N, k, kl, prec):
+    """aoa_encrypt and aoa_decrypt equal the primitive calls they replace
+    (aoa_sample, aoa_ntt, aoa_eltwise), bit for bit, with the single-workgroup
+    transform (N = 4096) and the two-pass one."""
+    import torch
+    from aoa_amd import Params
+    from aoa_amd.core import ADD, FORWARD, INVERSE, MULADD, SAMPLE_GAUSSIAN, SAMPLE_ZO
+    P, O = Params(N, k, kl, prec), Oracle(N, k, kl, prec)
+    rng = np.random.default_rng(N)
+    seed, nonce = 7, 123
+    m = P.to_device(O.random_poly(rng, k))
+    pkb, pka = P.to_device(O.random_poly(rng, k)), P.to_device(O.random_poly(rng, k))
+    # primitives
+    m1 = m.clone()
+    u, e0, e1 = P.zeros(k), P.zeros(k), P.zeros(k)
+    P.ntt(m1, 0, k, FORWARD)
+    P.sample(u, SAMPLE_ZO, seed, nonce, 0, k)
+    P.sample(e0, SAMPLE_GAUSSIAN, seed, nonce + 1, 0, k)
+    P.sample(e1, SAMPLE_GAUSSIAN, seed, nonce + 2, 0, k)
+    for x in (u, e0, e1):
+        P.ntt(x, 0, k, FORWARD)
+    r0, r1 = P.zeros(k), P.zeros(k)
+    P.eltwise(MULADD, r0, pkb, u, e0, res0=0, nres=k)
+    P.eltwise(ADD, r0, r0, m1, res0=0, nres=k)
+    P.eltwise(MULADD, r1, pka, u, e1, res0=0, nres=k)
+    # entry point
+    m2 = m.clone()
+    c0, c1, ws = P.zeros(k), P.zeros(k), P.zeros(3 * k)
+    P.encrypt(c0, c1, m2, pkb, pka, seed, nonce, ws)
+    assert torch.equal(m1, m2) and torch.equal(r0, c0) and torch.equal(r1, c1)
+    # decryption at level k-2 with a random NTT-domain key
+    s = P.to_device(O.random_poly(rng, k))
+    level = k - 2
+    L = level + 1
+    a0, a1 = P.to_device(O.random_poly(rng, k)), P.to_device(O.random_poly(rng, k))
+    b0, b1, t = a0.clone(), a1.clone(), P.zeros(k)
+    P.ntt(b0, 0, L, FORWARD)
+    P.ntt(b1, 0, L, FORWARD)
+    P.eltwise(MULADD, t, b1, s, b0, res0=0, nres=L)
+    P.ntt(t, 0, L, INVERSE)
+    got = P.zeros(k)
+    P.decrypt(got, a0, a1, s, level)
+    n = L * N
+    assert torch.equal(got[:n], t[:n])
+    assert torch.equal(a0[:n], b0[:n]) and torch.equal(a1[:n], b1[:n])
