@@ -300,9 +300,7 @@ extern "C" int aoa_encrypt(const aoa_params* h, void* stream, uint64_t* c0, uint
   uint64_t* u = (uint64_t*)ws;
   uint64_t* e0 = u + (size_t)k * N;
   uint64_t* e1 = e0 + (size_t)k * N;
-  launch_sample(P, u, 0, k, SAMPLE_ZO, seed, nonce, st);
-  launch_sample(P, e0, 0, k, SAMPLE_GAUSSIAN, seed, nonce + 1, st);
-  launch_sample(P, e1, 0, k, SAMPLE_GAUSSIAN, seed, nonce + 2, st);
+  launch_sample_encrypt(P, u, e0, e1, k, seed, nonce, st);
   LimbMap mm;
   std::memset(&mm, 0, sizeof(mm));
   mm.base[0] = m;

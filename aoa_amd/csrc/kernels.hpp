@@ -50,6 +50,10 @@ void launch_conjugate(const DevParams& P, uint64_t* out, const uint64_t* in, int
 // out + r*N) from distribution `kind`, keyed by (seed, nonce).
 void launch_sample(const DevParams& P, uint64_t* out, int res0, int nres, int kind, uint64_t seed, uint64_t nonce,
                    hipStream_t st);
+// u (ZO) at nonce, e0 and e1 (Gaussian) at nonce + 1 and nonce + 2, limbs
+// 0..nres-1, in one launch.
+void launch_sample_encrypt(const DevParams& P, uint64_t* u, uint64_t* e0, uint64_t* e1, int nres, uint64_t seed,
+                           uint64_t nonce, hipStream_t st);
 
 void launch_special_fft(const DevParams& P, double* vals, int n, int inverse, hipStream_t st);
 void launch_encode_pack(const DevParams& P, uint64_t* out, const double* vals, int slots, double scale, int k,

@@ -73,10 +73,8 @@ __global__ __launch_bounds__(256) void k_sample_uniform(DevParams P, uint64_t* o
 }
 
 // Gaussian, ZO and narrow: one value per coefficient, written to every limb.
-__global__ __launch_bounds__(256) void k_sample_small(DevParams P, uint64_t* out, int res0, int nres, int kind,
-                                                      uint64_t seed, uint64_t nonce) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= (uint32_t)P.N) return;
+__device__ __forceinline__ void sample_small_at(const DevParams& P, uint64_t* out, int res0, int nres, int kind,
+                                                uint64_t seed, uint64_t nonce, uint32_t i) {
   const u32x4 c = draw(seed, nonce, ((uint32_t)kind << 24) | 0xFFFFFFu, i);
   int64_t v;
   if (kind == SAMPLE_GAUSSIAN) {
@@ -94,6 +92,30 @@ __global__ __launch_bounds__(256) void k_sample_small(DevParams P, uint64_t* out
   }
   for (int l = res0; l < res0 + nres; ++l)
     out[(size_t)l * P.N + i] = small_residue(v, ld_prime(P.primes, l).p);
+}
+
+__global__ __launch_bounds__(256) void k_sample_small(DevParams P, uint64_t* out, int res0, int nres, int kind,
+                                                      uint64_t seed, uint64_t nonce) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= (uint32_t)P.N) return;
+  sample_small_at(P, out, res0, nres, kind, seed, nonce, i);
+}
+
+// An encryption's three draws in one launch (blockIdx.y = 0, 1, 2): u (ZO) at
+// nonce, e0 and e1 (Gaussian) at nonce + 1 and nonce + 2, limbs 0..nres-1;
+// the same values as three aoa_sample calls.
+__global__ __launch_bounds__(256) void k_sample_encrypt(DevParams P, uint64_t* u, uint64_t* e0, uint64_t* e1,
+                                                        int nres, uint64_t seed, uint64_t nonce) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= (uint32_t)P.N) return;
+  const int y = blockIdx.y;
+  sample_small_at(P, y == 0 ? u : (y == 1 ? e0 : e1), 0, nres, y == 0 ? SAMPLE_ZO : SAMPLE_GAUSSIAN, seed,
+                  nonce + (uint64_t)y, i);
+}
+
+void launch_sample_encrypt(const DevParams& P, uint64_t* u, uint64_t* e0, uint64_t* e1, int nres, uint64_t seed,
+                           uint64_t nonce, hipStream_t st) {
+  k_sample_encrypt<<<dim3((unsigned)((P.N + 255) / 256), 3), 256, 0, st>>>(P, u, e0, e1, nres, seed, nonce);
 }
 
 __global__ __launch_bounds__(256) void k_fill_zero(uint64_t* out, size_t n) {

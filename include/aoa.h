@@ -137,7 +137,7 @@ enum { AOA_SAMPLE_UNIFORM = 0, AOA_SAMPLE_GAUSSIAN = 1, AOA_SAMPLE_ZO = 2, AOA_S
  * (Gaussian) are drawn with aoa_sample's generator at nonces nonce, nonce+1
  * and nonce+2 on limbs 0..k-1; then c0 = pk_b u + e0 + m and
  * c1 = pk_a u + e1 in the NTT domain.  The same as aoa_sample, aoa_ntt and
- * aoa_eltwise called in that order, in five launches.  ws: at least
+ * aoa_eltwise called in that order, in three launches.  ws: at least
  * 3 k N 8 bytes (u, e0, e1). */
 int aoa_encrypt(const aoa_params* params, void* stream, uint64_t* c0, uint64_t* c1, uint64_t* m,
                 const uint64_t* pk_b, const uint64_t* pk_a, uint64_t seed, uint64_t nonce, void* ws, size_t ws_bytes);
