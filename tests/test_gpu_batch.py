@@ -231,3 +231,31 @@ def test_batch_c4_full_size_equals_plain(gpu):
             for y in range(2):
                 P.eltwise(0, cur[y], cur[y], rot[y], res0=0, nres=level + 1)
         assert torch.equal(w[0][:n], cur[0][:n]) and torch.equal(w[1][:n], cur[1][:n])
+
+
+def test_batch_fused_every_level_equals_oracle(gpu):
+    """The batched HomMul at the smallest two-pass size (N=2^13, k=8, prec 48),
+    one item at every level 1..k-1: the tensor pass of d2, d0 and d1 folded
+    into the key products and ModDown without addends give the oracle's
+    residues for every item."""
+    import torch
+    from aoa_amd import Batch, Params
+    N, k = 8192, 8
+    P, O = Params(N, k, 0, 48), Oracle(N, k, 0, 48)
+    rng = np.random.default_rng(43)
+    evk = [P.to_device(_rand(P, rng, P.k + P.kl)) for _ in range(2)]
+    levels = list(range(1, k))
+    items = _mul_items(P, rng, levels)
+    B = Batch()
+    B.add_mul(P, evk, items)
+    B.build()
+    B.launch(graph=False)
+    torch.cuda.synchronize()
+    he = [P.to_host(t) for t in evk]
+    for it in items:
+        lv = it[3]
+        h = [P.to_host(t) for t in it[0] + it[1]]
+        o0, o1 = O.hommul((h[0], h[1]), (h[2], h[3]), (he[0], he[1]), lv)
+        n = lv * N
+        assert np.array_equal(P.to_host(it[2][0])[:n], o0[:n]), "c0 level %d" % lv
+        assert np.array_equal(P.to_host(it[2][1])[:n], o1[:n]), "c1 level %d" % lv
