@@ -307,16 +307,18 @@ void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int ma
 
 // --------------------------------------------------------------- ModDown ----
 // out_i = (x_i - sum_j ref([x_bj (B/b_j)^-1]_bj * (B/b_j mod q_i))) B^-1 mod q_i
-// (aoa-ntt/src/cuda/manager.cu:312-341), optionally + add_i, optionally
-// rescaled: out_i <- (out_i - out_l) q_l^-1 mod q_i (manager.cu:90-118).
+// (aoa-ntt/src/cuda/manager.cu:312-341), optionally + add_i (rotations, and
+// HomMul with the single-workgroup transform), optionally rescaled:
+// out_i <- (out_i - out_l) q_l^-1 mod q_i (manager.cu:90-118).
 //
 // ref(a*c) is the reference's Barrett reduction of the product (ref_barrett).
 // With A = a*c - qhat*p = delta*p + r (r the exact remainder, delta the
 // Barrett quotient error, 0 <= delta*eps < 2^nb, eps = 2^nb - p), its result
-// is r - eps when eps <= r < delta*eps and r otherwise.  So the exact Shoup
-// remainder is used unless r falls in [eps, delta_max*eps), a window of
-// probability about 2^-17 per term at 48-bit q_i, where the literal formula
-// is evaluated.
+// is r - eps when eps <= r < delta*eps and r otherwise; delta_max (from the
+// largest product and 2^(2nb) mod p, params.cpp) bounds delta.  moddown_term
+// evaluates the literal formula only when r falls in [eps, delta_max*eps)
+// (the 63-bit q0, and the per-limb recomputation of moddown_conv); primes
+// below 2^57 take the exact sums of moddown_conv.
 // exact != 0 selects the exact remainder for every term (an empty window).
 __device__ __forceinline__ uint64_t moddown_term(uint64_t a, const tw2 w, const DevPrime& dq, int exact) {
   return ref_mulmod_shoup(a, w.w, w.q, dq.p, dq.mu, (int)dq.nb, exact ? 0 : dq.quirk_lo, exact ? 0 : dq.quirk_thr);
