@@ -589,13 +589,13 @@ __global__ __launch_bounds__(256) void k_ntt_inv_col_d(DevParams P, const NttDes
 }
 
 // ------------------------------------------- single-workgroup transform --
-// N <= 4096: one workgroup holds a whole limb (V = N/256 values per thread,
+// N <= 4096: one workgroup holds a whole limb (V values per thread,
 // 2^R = V) and runs every stage, exchanging through LDS between rounds of R
 // stages, so each limb is read and written once.  In a round whose Rr active
 // bits sit at positions [lo, lo + Rr) of the element index, value j of thread
 // t is element sp_index(t, j): the active bits come from j, the rest from t
 // (and, when Rr < R, from the high bits of j).  Loads and stores go through
-// LDS in the order i = t + 256 m, so every global access is coalesced.
+// LDS in the order i = t + TH m, so every global access is coalesced.
 
 template <int R>
 __device__ __forceinline__ int sp_index(int t, int j, int lo, int Rr) {
@@ -607,20 +607,24 @@ __device__ __forceinline__ int sp_index(int t, int j, int lo, int Rr) {
 
 __device__ __forceinline__ int spz(int i) { return i + (i >> 5); }
 
+// TH = N/4 threads (16 waves at N = 4096), four values each: a limb's
+// transform is one workgroup, so the threads per limb set its latency, and
+// four waves per SIMD hide the twiddle and LDS waits that one wave per SIMD
+// (256 threads, 16 values each) left exposed.
 template <int LOGN>
 struct SGeo {
-  static constexpr int N = 1 << LOGN, V = N / 256, R = (V == 16) ? 4 : 3, LDS = N + N / 32;
+  static constexpr int N = 1 << LOGN, TH = N / 4, V = 4, R = 2, LDS = N + N / 32;
 };
 
 // Coalesced load of a limb (optionally times a key limb) into LDS.
 template <int LOGN, bool LZ>
 __device__ __forceinline__ void sp_load(const uint64_t* src, const uint64_t* key, const DevPrime& dp,
                                         uint64_t* lds) {
-  constexpr int V = SGeo<LOGN>::V;
+  constexpr int V = SGeo<LOGN>::V, TH = SGeo<LOGN>::TH;
   const int t = threadIdx.x;
 #pragma unroll
   for (int m = 0; m < V; ++m) {
-    const int i = t + 256 * m;
+    const int i = t + TH * m;
     uint64_t v = src[i];
     if (key) v = key_mul<LZ>(v, key[i], dp);
     lds[spz(i)] = v;
@@ -630,10 +634,10 @@ __device__ __forceinline__ void sp_load(const uint64_t* src, const uint64_t* key
 
 template <int LOGN>
 __device__ __forceinline__ void sp_store(uint64_t* dst, const uint64_t* lds) {
-  constexpr int V = SGeo<LOGN>::V;
+  constexpr int V = SGeo<LOGN>::V, TH = SGeo<LOGN>::TH;
   const int t = threadIdx.x;
 #pragma unroll
-  for (int m = 0; m < V; ++m) dst[t + 256 * m] = lds[spz(t + 256 * m)];
+  for (int m = 0; m < V; ++m) dst[t + TH * m] = lds[spz(t + TH * m)];
 }
 
 template <int LOGN, bool LZ>
@@ -722,7 +726,7 @@ __device__ __forceinline__ void inv_single_body(const DevParams& P, const uint64
 }
 
 template <int LOGN>
-__global__ __launch_bounds__(256) void k_ntt_fwd_single(DevParams P, LimbMap m) {
+__global__ __launch_bounds__(SGeo<LOGN>::TH) void k_ntt_fwd_single(DevParams P, LimbMap m) {
   __shared__ uint64_t lds[SGeo<LOGN>::LDS];
   const int li = blockIdx.x, y = blockIdx.y, pr = limb_prime(m, P.k, li);
   const size_t off = (size_t)li * P.N;
@@ -733,7 +737,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_single(DevParams P, LimbMap m) 
 }
 
 template <int LOGN>
-__global__ __launch_bounds__(256) void k_ntt_inv_single(DevParams P, LimbMap m, const uint64_t* key0,
+__global__ __launch_bounds__(SGeo<LOGN>::TH) void k_ntt_inv_single(DevParams P, LimbMap m, const uint64_t* key0,
                                                          const uint64_t* key1) {
   __shared__ uint64_t lds[SGeo<LOGN>::LDS];
   const int li = blockIdx.x, y = blockIdx.y, pr = limb_prime(m, P.k, li);
@@ -747,7 +751,7 @@ __global__ __launch_bounds__(256) void k_ntt_inv_single(DevParams P, LimbMap m, 
 }
 
 template <int LOGN>
-__global__ __launch_bounds__(256) void k_ntt_fwd_single_d(DevParams P, const NttDesc* d) {
+__global__ __launch_bounds__(SGeo<LOGN>::TH) void k_ntt_fwd_single_d(DevParams P, const NttDesc* d) {
   __shared__ uint64_t lds[SGeo<LOGN>::LDS];
   const NttDesc& e = d[blockIdx.x];
   const uint64_t* src = e.src;
@@ -760,7 +764,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_single_d(DevParams P, const Ntt
 }
 
 template <int LOGN>
-__global__ __launch_bounds__(256) void k_ntt_inv_single_d(DevParams P, const NttDesc* d) {
+__global__ __launch_bounds__(SGeo<LOGN>::TH) void k_ntt_inv_single_d(DevParams P, const NttDesc* d) {
   __shared__ uint64_t lds[SGeo<LOGN>::LDS];
   const NttDesc& e = d[blockIdx.x];
   const uint64_t* src = e.src;
@@ -775,7 +779,7 @@ __global__ __launch_bounds__(256) void k_ntt_inv_single_d(DevParams P, const Ntt
 
 // Both key products of one input limb, one workgroup per (limb, key half).
 template <int LOGN>
-__global__ __launch_bounds__(256) void k_ntt_inv_single_key2_d(DevParams P, const KeyDesc2* d) {
+__global__ __launch_bounds__(SGeo<LOGN>::TH) void k_ntt_inv_single_key2_d(DevParams P, const KeyDesc2* d) {
   __shared__ uint64_t lds[SGeo<LOGN>::LDS];
   const KeyDesc2& e = d[blockIdx.x >> 1];
   const int y = blockIdx.x & 1;
@@ -846,7 +850,7 @@ int ntt_forward(const DevParams& P, LimbMap m, int N1, hipStream_t st) {
     if (!m.src[y]) m.src[y] = m.base[y];
   if (N1 == P.N) {
     dim3 grid(m.nlimbs, m.npolys);
-#define CALL1(L) k_ntt_fwd_single<L><<<grid, 256, 0, st>>>(P, m)
+#define CALL1(L) k_ntt_fwd_single<L><<<grid, SGeo<L>::TH, 0, st>>>(P, m)
     AOA_DISPATCH_LOGN(P.N, CALL1)
 #undef CALL1
     return 0;
@@ -880,7 +884,7 @@ int ntt_inverse(const DevParams& P, LimbMap m, int N1, const uint64_t* key0, con
     if (!m.src[y]) m.src[y] = m.base[y];
   if (N1 == P.N) {
     dim3 grid(m.nlimbs, m.npolys);
-#define CALL1(L) k_ntt_inv_single<L><<<grid, 256, 0, st>>>(P, m, key0, key1)
+#define CALL1(L) k_ntt_inv_single<L><<<grid, SGeo<L>::TH, 0, st>>>(P, m, key0, key1)
     AOA_DISPATCH_LOGN(P.N, CALL1)
 #undef CALL1
     return 0;
@@ -923,7 +927,7 @@ int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
   if (!N1 || ndesc <= 0) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
   if (N1 == P.N) {
-#define CALL1(L) k_ntt_fwd_single_d<L><<<dim3(ndesc), 256, 0, st>>>(P, d)
+#define CALL1(L) k_ntt_fwd_single_d<L><<<dim3(ndesc), SGeo<L>::TH, 0, st>>>(P, d)
     AOA_DISPATCH_LOGN(P.N, CALL1)
 #undef CALL1
     return 0;
@@ -948,7 +952,7 @@ int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
   if (!N1 || ndesc <= 0) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
   if (N1 == P.N) {
-#define CALL1(L) k_ntt_inv_single_d<L><<<dim3(ndesc), 256, 0, st>>>(P, d)
+#define CALL1(L) k_ntt_inv_single_d<L><<<dim3(ndesc), SGeo<L>::TH, 0, st>>>(P, d)
     AOA_DISPATCH_LOGN(P.N, CALL1)
 #undef CALL1
     return 0;
@@ -974,7 +978,7 @@ int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const 
   if (!N1 || n2 <= 0 || ncol <= 0) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
   if (N1 == P.N) {
-#define CALL1(L) k_ntt_inv_single_key2_d<L><<<dim3(2 * n2), 256, 0, st>>>(P, d2)
+#define CALL1(L) k_ntt_inv_single_key2_d<L><<<dim3(2 * n2), SGeo<L>::TH, 0, st>>>(P, d2)
     AOA_DISPATCH_LOGN(P.N, CALL1)
 #undef CALL1
     return 0;

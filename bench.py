@@ -405,6 +405,8 @@ def main():
     ap.add_argument("--rate", type=float, default=300.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--profile", action="store_true",
                     help="headline shape only (plus a 1 GiB copy for counter calibration), for rocprofv3")
+    ap.add_argument("--profile-c1", action="store_true",
+                    help="the N=2^12 end-to-end pipeline only (no CPU oracle), for rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every GPU box here has one GPU and this build has no "
@@ -416,6 +418,10 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     stream = torch.cuda.current_stream()
+    if args.profile_c1:
+        args.cpu_sample = 0
+        print(json.dumps({"c1_pipeline_n12": c1_pipeline(args, stream)}))
+        return
     torch.manual_seed(2)
     P = Params(N, K_Q, K_B, PREC)
     level = K_Q - 1
