@@ -118,16 +118,21 @@ static int ilog2(int n) {
 static unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
 
 // Stages of length <= B inside aligned blocks of B values, in LDS: one
-// workgroup per block (B <= 4096, 64 KiB).  Forward: stages len = 2..B;
-// inverse: len = B..2.  With whole != 0 the block is the whole vector and the
-// bit reversal is folded into the load (forward) or, with the division by n,
-// into the store (inverse).
-constexpr int kFftThreads = 512;
+// workgroup per block (B <= 4096: 64 KiB of values and 8 KiB of the rotation
+// group).  Forward: stages len = 2..B; inverse: len = B..2.  With whole != 0
+// the block is the whole vector and the bit reversal is folded into the load
+// (forward) or, with the division by n, into the store (inverse).  Each
+// thread takes the butterflies b = t, t + 1024 of every stage and requests
+// the next stage's roots before the barrier, so the table latency overlaps
+// it; the arithmetic of each butterfly is unchanged.
+constexpr int kFftThreads = 1024;
 constexpr int kFftBlock = 4096;
+constexpr int kFftPer = kFftBlock / 2 / kFftThreads;  // butterflies per thread and stage
 
 __global__ __launch_bounds__(kFftThreads) void k_enc_fft_block(DevParams P, cplx* v, int B, int logB, int inverse,
                                                                int whole) {
   extern __shared__ cplx buf[];
+  uint32_t* rot = reinterpret_cast<uint32_t*>(buf + B);  // rotation group 5^j mod 2N, j < B/2
   const int t = threadIdx.x;
   cplx* blk = v + (size_t)blockIdx.x * B;
   if (whole && !inverse) {
@@ -135,30 +140,61 @@ __global__ __launch_bounds__(kFftThreads) void k_enc_fft_block(DevParams P, cplx
   } else {
     for (int i = t; i < B; i += kFftThreads) buf[i] = blk[i];
   }
+  for (int i = t; i < B / 2; i += kFftThreads) rot[i] = P.rot_group[i];
   __syncthreads();
+  // root of butterfly b in the stage of length len
+  auto root = [&](int len, int b) {
+    const int lenh = len >> 1, lenq = len << 2;
+    const uint32_t step = (uint32_t)(2 * P.N / lenq);
+    const uint32_t r = rot[b % lenh] % (uint32_t)lenq;
+    return enc_root(P, (inverse ? (uint32_t)lenq - r : r) * step);
+  };
+  cplx w[kFftPer];
+  const int first = inverse ? B : 2;
+#pragma unroll
+  for (int q = 0; q < kFftPer; ++q) {
+    const int b = t + q * kFftThreads;
+    if (b < B / 2) w[q] = root(first, b);
+  }
   if (!inverse) {
     for (int len = 2; len <= B; len <<= 1) {
-      const int lenh = len >> 1, lenq = len << 2;
-      const uint32_t step = (uint32_t)(2 * P.N / lenq);
-      for (int b = t; b < B / 2; b += kFftThreads) {
+      const int lenh = len >> 1;
+#pragma unroll
+      for (int q = 0; q < kFftPer; ++q) {
+        const int b = t + q * kFftThreads;
+        if (b >= B / 2) continue;
         const int i = (b / lenh) * len, j = b % lenh;
-        const uint32_t tw = (P.rot_group[j] % (uint32_t)lenq) * step;
-        const cplx u = buf[i + j], w = cmul(buf[i + j + lenh], enc_root(P, tw));
-        buf[i + j] = cplx{u.re + w.re, u.im + w.im};
-        buf[i + j + lenh] = cplx{u.re - w.re, u.im - w.im};
+        const cplx u = buf[i + j], x = cmul(buf[i + j + lenh], w[q]);
+        buf[i + j] = cplx{u.re + x.re, u.im + x.im};
+        buf[i + j + lenh] = cplx{u.re - x.re, u.im - x.im};
+      }
+      if (len < B) {
+#pragma unroll
+        for (int q = 0; q < kFftPer; ++q) {
+          const int b = t + q * kFftThreads;
+          if (b < B / 2) w[q] = root(len << 1, b);
+        }
       }
       __syncthreads();
     }
   } else {
     for (int len = B; len >= 2; len >>= 1) {
-      const int lenh = len >> 1, lenq = len << 2;
-      const uint32_t step = (uint32_t)(2 * P.N / lenq);
-      for (int b = t; b < B / 2; b += kFftThreads) {
+      const int lenh = len >> 1;
+#pragma unroll
+      for (int q = 0; q < kFftPer; ++q) {
+        const int b = t + q * kFftThreads;
+        if (b >= B / 2) continue;
         const int i = (b / lenh) * len, j = b % lenh;
-        const uint32_t tw = ((uint32_t)lenq - P.rot_group[j] % (uint32_t)lenq) * step;
         const cplx x = buf[i + j], y = buf[i + j + lenh];
         buf[i + j] = cplx{x.re + y.re, x.im + y.im};
-        buf[i + j + lenh] = cmul(cplx{x.re - y.re, x.im - y.im}, enc_root(P, tw));
+        buf[i + j + lenh] = cmul(cplx{x.re - y.re, x.im - y.im}, w[q]);
+      }
+      if (len > 2) {
+#pragma unroll
+        for (int q = 0; q < kFftPer; ++q) {
+          const int b = t + q * kFftThreads;
+          if (b < B / 2) w[q] = root(len >> 1, b);
+        }
       }
       __syncthreads();
     }
@@ -179,7 +215,7 @@ void launch_special_fft(const DevParams& P, double* vals, int n, int inverse, hi
   const int logn = ilog2(n);
   const int B = n < kFftBlock ? n : kFftBlock;
   const int logB = ilog2(B);
-  const size_t lds = (size_t)B * sizeof(cplx);
+  const size_t lds = (size_t)B * sizeof(cplx) + (size_t)(B / 2) * sizeof(uint32_t);
   const unsigned nblk = (unsigned)(n / B);
   const int whole = n == B;
   if (!inverse) {
