@@ -323,7 +323,7 @@ template <int S, bool LZ>
 __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint64_t* src, uint64_t* dst0,
                                                   uint64_t* dst1, const uint64_t* key0, const uint64_t* key1,
                                                   const uint64_t* add0, const uint64_t* add1, int pr, int tile,
-                                                  int g0, uint64_t* lds) {
+                                                  int g0, uint64_t* lds, int ysel) {
   using G = Geo<S>;
   constexpr int C = G::C, TS = G::TS, PS = S + S / 16;
   const DevPrime dp = P.primes[pr];
@@ -334,13 +334,14 @@ __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint
   const size_t off = (size_t)row * S;
 #pragma unroll
   for (int y = 0; y < 2; ++y) {
+    if (ysel >= 0 && y != ysel) continue;
     const uint64_t* kp = (y ? key1 : key0) + off;
     const uint64_t* ap = y ? add1 : add0;
     uint64_t x[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(src[off + sl + TS * k], kp[sl + TS * k], dp);
     if (ap) add_bscaled<LZ>(x, ap + off + sl, TS, P, pr, p);  // set for Q limbs only
-    if (y) __syncthreads();  // the row's LDS block is reused
+    if (y && ysel < 0) __syncthreads();  // the row's LDS block is reused
     inv_row_core<S, LZ>(P, x, (y ? dst1 : dst0) + off, p, T, row, sl, g0, lds + rl * PS);
   }
 }
@@ -548,19 +549,27 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_d(DevParams P, const NttDes
     inv_row_body<S, false>(P, e.src, e.dst, e.key, nullptr, e.prime, blockIdx.x % tiles, g0, lds);
 }
 
+// One workgroup per tile and key half: workgroups w and w + 8 take the two
+// halves of one tile; as workgroups are dispatched round-robin over the 8
+// XCDs, both run on one XCD and the second read of the input tile comes from
+// its L2.  `ntiles` = descriptors x tiles per limb; the grid is rounded up to
+// a multiple of 16.
 template <int S, int MODE>
-__global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const KeyDesc2* d, int g0) {
+__global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const KeyDesc2* d, int g0, int ntiles) {
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
   const int tiles = (P.N / S) / Geo<S>::C;
-  const KeyDesc2& e = d[blockIdx.x / tiles];
+  const int w = blockIdx.x;
+  const int y = (w >> 3) & 1, b = ((w >> 4) << 3) | (w & 7);
+  if (b >= ntiles) return;
+  const KeyDesc2& e = d[b / tiles];
   const uint64_t* src = e.src;
   uint64_t *d0 = e.dst[0], *d1 = e.dst[1];
   const uint64_t *k0 = e.key[0], *k1 = e.key[1], *a0 = e.add[0], *a1 = e.add[1];
   const int pr = e.prime;
   if (use_lazy<MODE>(P, pr))
-    inv_row_key2_body<S, true>(P, src, d0, d1, k0, k1, a0, a1, pr, blockIdx.x % tiles, g0, lds);
+    inv_row_key2_body<S, true>(P, src, d0, d1, k0, k1, a0, a1, pr, b % tiles, g0, lds, y);
   else if (MODE != 1)
-    inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, a0, a1, pr, blockIdx.x % tiles, g0, lds);
+    inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, a0, a1, pr, b % tiles, g0, lds, y);
 }
 
 template <int S, int MODE>
@@ -985,7 +994,16 @@ int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const 
   }
   {
     const int tiles = N1 / (4096 / N2);
-#define CALLR(S) AOA_SPLIT_LAUNCH(k_ntt_inv_row_key2_d, S, tiles, d2, n2, n2strict, g0)
+    auto grid = [&](int n) { return dim3((unsigned)((2 * tiles * n + 15) / 16 * 16)); };
+    const int nlazy = n2strict < 0 ? 0 : n2 - n2strict;
+#define CALLR(S)                                                                                          \
+  if (n2strict < 0) {                                                                                     \
+    k_ntt_inv_row_key2_d<S, 0><<<grid(n2), 256, 0, st>>>(P, d2, g0, tiles * n2);                          \
+  } else {                                                                                                \
+    if (n2strict > 0) k_ntt_inv_row_key2_d<S, 2><<<grid(n2strict), 256, 0, st>>>(P, d2, g0, tiles * n2strict); \
+    if (nlazy > 0)                                                                                        \
+      k_ntt_inv_row_key2_d<S, 1><<<grid(nlazy), 256, 0, st>>>(P, d2 + n2strict, g0, tiles * nlazy);        \
+  }
     AOA_DISPATCH_S(N2, CALLR)
 #undef CALLR
   }
