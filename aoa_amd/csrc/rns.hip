@@ -324,10 +324,6 @@ __device__ __forceinline__ uint64_t moddown_term(uint64_t a, const tw2 w, const 
   return ref_mulmod_shoup(a, w.w, w.q, dq.p, dq.mu, (int)dq.nb, exact ? 0 : dq.quirk_lo, exact ? 0 : dq.quirk_thr);
 }
 
-__device__ __forceinline__ uint64_t lazy_window(const DevPrime& d, int exact) {
-  return exact ? 0 : (d.quirk_thr < d.p ? d.quirk_thr : d.p);
-}
-
 typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
 
 // Sums over j in [j0, j1) of the ModDown products for NL limbs li[] with
@@ -339,8 +335,8 @@ typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
 // limb.  Exact sums equal the reference's term-by-term modular sum whenever
 // no term deviates; SCREEN runs moddown_screen on every product, and a lane
 // whose limb passed it for any j (one term in 10 000 to 300 000 at 48-bit
-// q_i, growing with 2^48 - q_i) recomputes that limb's sum with the
-// reference formula per term.
+// q_i, growing with 2^48 - q_i) screens that limb's terms again and adds
+// ref(a c) - (a c mod q_i) for each one that passes.
 template <int NL, bool SCREEN>
 __device__ __forceinline__ void moddown_conv(const DevParams& P, const uint64_t* aux, int c, const int* li,
                                              int j0, int j1, uint64_t* res) {
@@ -377,20 +373,26 @@ __device__ __forceinline__ void moddown_conv(const DevParams& P, const uint64_t*
 #pragma unroll
   for (int n = 0; n < NL; ++n) {
     const DevPrime dq = ld_prime(P.primes, li[n]);
+    uint64_t h, l;
+    join_28(s0[n], sm[n] - s0[n] - s2[n], s2[n], &h, &l);
+    uint64_t r = reduce128(h, l, dq);
     if (SCREEN && hit[n]) {
-      const uint64_t wthr = lazy_window(dq, 0);
+      // Add ref(a c) - (a c mod q_i) for the terms that pass the screen
+      // again; every other term lies outside the deviation window, where
+      // the two agree.
       const tw2* ci = bq + (size_t)li[n] * K;
-      uint64_t acc = 0;
       for (int j = j0; j < j1; ++j) {
-        const tw2 w = ld_tw2(ci, j);
-        acc += ref_mulmod_shoup_lazy(aux[j * BC_COEF + c], w.w, w.q, dq.p, dq.mu, (int)dq.nb, wthr);
+        const uint64_t a = aux[j * BC_COEF + c];
+        const u32x8 e = tab[(size_t)li[n] * K + j];
+        if (moddown_screen((uint32_t)a, (uint32_t)(a >> 32), e[3], e[4], e[5], bias) <= thr[n]) {
+          const tw2 w = ld_tw2(ci, j);
+          const uint64_t x = mul_shoup(a, w.w, w.q, dq.p);
+          const uint64_t f = ref_barrett(mulhi64(a, w.w), a * w.w, dq.p, dq.mu, (int)dq.nb);
+          r = add_mod(r, sub_mod(f, x, dq.p), dq.p);
+        }
       }
-      res[n] = barrett_reduce(0, acc, dq.p, dq.mu, (int)dq.nb);
-    } else {
-      uint64_t h, l;
-      join_28(s0[n], sm[n] - s0[n] - s2[n], s2[n], &h, &l);
-      res[n] = reduce128(h, l, dq);
     }
+    res[n] = r;
   }
 }
 
@@ -521,7 +523,10 @@ void launch_moddown(const DevParams& P, const ModDownArgs& a, hipStream_t st) {
   k_moddown<<<dim3(P.N / BC_COEF, a.npolys), 256, lds, st>>>(P, a);
 }
 
-__global__ __launch_bounds__(256) void k_moddown_d(DevParams P, const ModDownDesc* dd) {
+// At least six waves per SIMD (the compiler then fits the kernel in 70
+// registers, seven waves): the batched grid is many times the machine, and
+// more waves hide the product loop's table loads.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_moddown_d(DevParams P, const ModDownDesc* dd) {
   extern __shared__ uint64_t aux[];
   const int per = P.N / BC_COEF;
   const ModDownDesc m = dd[blockIdx.x / per];
