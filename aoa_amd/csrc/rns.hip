@@ -513,7 +513,8 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   });
 }
 
-__global__ __launch_bounds__(256) void k_moddown(DevParams P, ModDownArgs A) {
+// Eight waves per SIMD, as k_moddown_d: 2.5 % more plain HomMuls per second.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_moddown(DevParams P, ModDownArgs A) {
   extern __shared__ uint64_t aux[];  // [K][64] then [4][64] partial sums
   const int y = blockIdx.y;
   moddown_body(P, A.inQ[y], A.inB[y], A.add[y], A.out[y], A.level, A.rescale, blockIdx.x, aux);
@@ -524,10 +525,10 @@ void launch_moddown(const DevParams& P, const ModDownArgs& a, hipStream_t st) {
   k_moddown<<<dim3(P.N / BC_COEF, a.npolys), 256, lds, st>>>(P, a);
 }
 
-// At least six waves per SIMD (the compiler then fits the kernel in 70
-// registers, seven waves): the batched grid is many times the machine, and
-// more waves hide the product loop's table loads.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_moddown_d(DevParams P, const ModDownDesc* dd) {
+// Eight waves per SIMD (58 registers): the batched grid is many times the
+// machine, and more waves hide the product loop's table loads; 4 % faster
+// than seven waves (70 registers) and than five (81, the default).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_moddown_d(DevParams P, const ModDownDesc* dd) {
   extern __shared__ uint64_t aux[];
   const int per = P.N / BC_COEF;
   const ModDownDesc m = dd[blockIdx.x / per];
