@@ -316,10 +316,10 @@ void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int ma
 // With A = a*c - qhat*p = delta*p + r (r the exact remainder, delta the
 // Barrett quotient error, 0 <= delta*eps < 2^nb, eps = 2^nb - p), its result
 // is r - eps when eps <= r < delta*eps and r otherwise; delta_max (from the
-// largest product and 2^(2nb) mod p, params.cpp) bounds delta.  moddown_term
-// evaluates the literal formula only when r falls in [eps, delta_max*eps)
-// (the 63-bit q0, and the per-limb recomputation of moddown_conv); primes
-// below 2^57 take the exact sums of moddown_conv.
+// largest product and 2^(2nb) mod p, params.cpp) bounds delta.  For the
+// 63-bit q0, moddown_term evaluates the literal formula only when r falls in
+// [eps, delta_max*eps); primes below 2^57 take the exact sums of
+// moddown_conv, corrected term by term where the screen flags a product.
 // exact != 0 selects the exact remainder for every term (an empty window).
 __device__ __forceinline__ uint64_t moddown_term(uint64_t a, const tw2 w, const DevPrime& dq, int exact) {
   return ref_mulmod_shoup(a, w.w, w.q, dq.p, dq.mu, (int)dq.nb, exact ? 0 : dq.quirk_lo, exact ? 0 : dq.quirk_thr);
