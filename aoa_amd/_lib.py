@@ -46,6 +46,8 @@ SIGNATURES = {
                                                  ctypes.c_int, c_vp, ctypes.c_size_t]),
     "aoa_rotate_left": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int,
                                         ctypes.c_int, c_vp, ctypes.c_size_t]),
+    "aoa_rotate_right": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int,
+                                         ctypes.c_int, c_vp, ctypes.c_size_t]),
 }
 
 
@@ -83,6 +85,7 @@ SIGNATURES.update({
     "aoa_server_stats": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "aoa_server_release": (ctypes.c_int, [c_vp, ctypes.c_int64]),
     "aoa_server_destroy": (None, [c_vp]),
 })
 

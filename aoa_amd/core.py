@@ -187,6 +187,12 @@ class Params:
                                        _ptr(c[1]), _ptr(rtk[0]), _ptr(rtk[1]), rot_slots, level, _ptr(w),
                                        nb))
 
+    def rotate_right(self, out, c, rtk, rot_slots, level, ws, stream=None):
+        w, nb = ws
+        check(self.lib.aoa_rotate_right(self.h, _stream(stream), _ptr(out[0]), _ptr(out[1]), _ptr(c[0]),
+                                        _ptr(c[1]), _ptr(rtk[0]), _ptr(rtk[1]), rot_slots, level, _ptr(w),
+                                        nb))
+
 
 class Batch:
     """aoa_batch handle: stage-fused evaluation of many ciphertexts, replayed
@@ -311,6 +317,9 @@ class Server:
         st, lv, lat = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         check(self.lib.aoa_server_result(self.h, rid, ctypes.byref(st), ctypes.byref(lv), ctypes.byref(lat)))
         return st.value, lv.value, lat.value
+
+    def release(self, rid):
+        check(self.lib.aoa_server_release(self.h, rid))
 
     def stats(self):
         c, o, s = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

@@ -574,3 +574,15 @@ extern "C" int aoa_rotate_left(const aoa_params* h, void* stream, uint64_t* out0
   if (rc) return rc;
   return check_launch("aoa_rotate_left");
 }
+
+// ckks_rotate_right (aoa-ntt/src/ckks/ckks.cpp:953-980): rotate_slots_right is
+// rotate_slots_left by N/2 - rotSlots (aoa-ntt/src/ckks/encoder.cu:356-363);
+// the key switch is the same, with the right rotation key.
+extern "C" int aoa_rotate_right(const aoa_params* h, void* stream, uint64_t* out0, uint64_t* out1,
+                                const uint64_t* c0, const uint64_t* c1, const uint64_t* rtk_b,
+                                const uint64_t* rtk_a, int rot_slots, int level, void* ws, size_t ws_bytes) {
+  if (!h) return fail(AOA_ERR_ARG, "null argument");
+  if (rot_slots < 0 || rot_slots > h->host.N / 2) return fail(AOA_ERR_ARG, "rotation out of range [0, N/2]");
+  return aoa_rotate_left(h, stream, out0, out1, c0, c1, rtk_b, rtk_a, h->host.N / 2 - rot_slots, level, ws,
+                         ws_bytes);
+}

@@ -6,8 +6,10 @@
 // parameter set.  Each step takes the next op of every active request whose
 // workspace fits the step budget, plans one stage-fused batch for all of them
 // (a forward-NTT layer for multiplications whose operand is in the coefficient
-// domain, then the op layer), runs it, and retires finished requests.
-// Requests submitted while a step runs join at the next step boundary.
+// domain, then the op layer) and enqueues it.  The serving thread plans and
+// enqueues step t+1 while step t runs and retires step t's finished requests
+// after that, so host planning overlaps device work.  Requests submitted
+// while a step runs join at the next step boundary.
 // No counterpart in the reference, which evaluates one call at a time on one
 // stream (aoa-ntt/include/newckks/tool/context.h:16-49).
 #include <hip/hip_runtime.h>
@@ -101,30 +103,43 @@ enum { PENDING = 0, ACTIVE = 1, DONE = 2, FAILED = 3 };
 
 struct Request {
   int64_t id = 0;
-  int keyset = 0;
+  const KeySet* ks = nullptr;  // taken under the lock at submit; deque elements never move
   const uint64_t* in[2] = {nullptr, nullptr};
   const uint64_t* m[2] = {nullptr, nullptr};
   uint64_t* out[2] = {nullptr, nullptr};
   std::vector<aoa_op> ops;
-  size_t next = 0;
-  int level = 0;
-  bool ntt = false;
-  char* mem = nullptr;     // pool block: two working ciphertexts
+  size_t next = 0;  // next op to plan
+  int level = 0;    // level after the ops planned so far
+  bool ntt = false; // domain after the ops planned so far
+  char* mem = nullptr;  // pool block: two working ciphertexts
   uint64_t* buf[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   int cur = 0;
   double t_submit = 0, t_done = 0;
   int state = PENDING;
 };
 
+// What one enqueued step will finish, and whether its enqueue failed.
+struct Step {
+  std::vector<Request*> finish;  // requests whose last op (or final copy) is in this step
+  std::vector<Request*> failed;  // requests whose step could not be enqueued
+  int ops = 0;
+  int rc = AOA_OK;
+  bool enqueued = false;
+};
+
 }  // namespace
+
+// Finished requests kept for aoa_server_result before the oldest are dropped.
+constexpr size_t kRetainFinished = (size_t)1 << 16;
 
 struct aoa_server {
   std::mutex mu;
   std::condition_variable cv;
-  std::deque<KeySet> keysets;  // stable references while clients register more
+  std::deque<KeySet> keysets;  // push_back keeps references to existing elements valid
   std::deque<Request*> pending;
   std::vector<Request*> active;
   std::map<int64_t, Request*> all;
+  std::deque<int64_t> finished;  // ids in completion order, for bounded retention
   int64_t next_id = 1;
   int max_active = 64;
   Pool pool;
@@ -132,11 +147,11 @@ struct aoa_server {
   void* step_ws = nullptr;
   size_t step_ws_bytes = 0;
   void* desc_dev = nullptr;
-  char* desc_host = nullptr;
+  char* desc_host[2] = {nullptr, nullptr};  // one pinned staging buffer per step in flight
   size_t desc_cap = 0;
   aoa_batch* plan = nullptr;
   hipStream_t own_stream = nullptr;
-  hipEvent_t ev = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
   std::thread worker;
   bool running = false, stop = false;
   int last_rc = AOA_OK;
@@ -164,14 +179,19 @@ extern "C" int aoa_server_create(size_t pool_bytes, size_t step_ws_bytes, int ma
   s->desc_cap = (size_t)64 << 20;
   if (hipMalloc(&s->pool_mem, pool_bytes) != hipSuccess || hipMalloc(&s->step_ws, step_ws_bytes) != hipSuccess ||
       hipMalloc(&s->desc_dev, s->desc_cap) != hipSuccess ||
-      hipHostMalloc((void**)&s->desc_host, s->desc_cap, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&s->desc_host[0], s->desc_cap, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&s->desc_host[1], s->desc_cap, hipHostMallocDefault) != hipSuccess ||
       hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&s->ev, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&s->ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev[1], hipEventDisableTiming) != hipSuccess) {
     if (s->pool_mem) (void)hipFree(s->pool_mem);
     if (s->step_ws) (void)hipFree(s->step_ws);
     if (s->desc_dev) (void)hipFree(s->desc_dev);
-    if (s->desc_host) (void)hipHostFree(s->desc_host);
+    for (char* h : s->desc_host)
+      if (h) (void)hipHostFree(h);
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
+    for (hipEvent_t e : s->ev)
+      if (e) (void)hipEventDestroy(e);
     delete s;
     return set_error(AOA_ERR_HIP, "server allocation failed");
   }
@@ -233,7 +253,7 @@ extern "C" int aoa_server_submit(aoa_server* s, int keyset, const uint64_t* c0, 
     return set_error(AOA_ERR_WORKSPACE, "request exceeds the HBM budget");
   Request* r = new Request();
   r->id = s->next_id++;
-  r->keyset = keyset;
+  r->ks = &ks;
   r->in[0] = c0;
   r->in[1] = c1;
   r->m[0] = m0;
@@ -252,19 +272,24 @@ extern "C" int aoa_server_submit(aoa_server* s, int keyset, const uint64_t* c0, 
   return AOA_OK;
 }
 
-// One step: admit, run the next op of every admitted request that fits the
-// step workspace, retire finished requests.  Returns the number of requests
-// still pending or active, or a negative status.
-static int server_step(aoa_server* s, hipStream_t st) {
+// Plans one step and enqueues it on `st`: admits pending requests while the
+// pool has room (copying their inputs in), takes the next op of every active
+// request whose workspace fits the step budget, and appends the final copy of
+// every request whose chain ends with this step.  The requests' host state
+// (next op, level, domain, current buffer) advances at planning time: the
+// device runs the steps in stream order, so step t+1 may be planned and
+// enqueued while step t still runs, and a pool block released here is reused
+// only by work enqueued after this step's final copies.  The descriptors are
+// staged in desc_host[slot], which the caller keeps untouched until this
+// step's event has completed.
+static Step enqueue_step(aoa_server* s, hipStream_t st, int slot) {
+  Step out;
   std::vector<Request*> run;
-  std::vector<Request*> finish_now;
   {
     std::lock_guard<std::mutex> g(s->mu);
-    // admission by pool budget (two working ciphertexts per request)
     while (!s->pending.empty() && (int)s->active.size() < s->max_active) {
       Request* r = s->pending.front();
-      const KeySet& ks = s->keysets[r->keyset];
-      const size_t cb = ct_bytes(ks, r->level);
+      const size_t cb = ct_bytes(*r->ks, r->level);
       char* mem = s->pool.alloc(2 * cb + 1024);
       if (!mem) break;
       s->pending.pop_front();
@@ -281,14 +306,13 @@ static int server_step(aoa_server* s, hipStream_t st) {
         (void)hipMemcpyAsync(r->buf[0][y], r->in[y], pb, hipMemcpyDeviceToDevice, st);
       s->active.push_back(r);
     }
-    // requests with no ops left (or none at all) finish without a step
     size_t ws = 0;
     for (Request* r : s->active) {
       if (r->next >= r->ops.size()) {
-        finish_now.push_back(r);
+        out.finish.push_back(r);  // no ops: only the copies
         continue;
       }
-      const size_t need = op_ws_bytes(s->keysets[r->keyset], r->ops[r->next], r->level);
+      const size_t need = op_ws_bytes(*r->ks, r->ops[r->next], r->level);
       if (ws + need > s->step_ws_bytes) continue;  // waits for the next step
       ws += need;
       run.push_back(r);
@@ -298,25 +322,22 @@ static int server_step(aoa_server* s, hipStream_t st) {
   if (!run.empty()) {
     aoa_batch* b = s->plan;
     batch_clear(b);
-    // layer 1: bring multiplication operands into the NTT domain
+    // layer 1: bring multiplication operands into the NTT domain, rotation
+    // inputs into the coefficient domain
     bool any = false;
     for (Request* r : run) {
-      const KeySet& ks = s->keysets[r->keyset];
-      if (r->ops[r->next].kind == AOA_OP_MUL && !r->ntt) {
-        aoa_ntt_item it = {{r->buf[r->cur][0], r->buf[r->cur][1]}, r->level, AOA_FORWARD};
-        batch_add_ntt(b, ks.params, &it, 1);
-        any = true;
-      }
-      if (r->ops[r->next].kind == AOA_OP_ROTATE && r->ntt) {
-        aoa_ntt_item it = {{r->buf[r->cur][0], r->buf[r->cur][1]}, r->level, AOA_INVERSE};
-        batch_add_ntt(b, ks.params, &it, 1);
+      const int kind = r->ops[r->next].kind;
+      if ((kind == AOA_OP_MUL) != r->ntt) {
+        aoa_ntt_item it = {{r->buf[r->cur][0], r->buf[r->cur][1]}, r->level,
+                           kind == AOA_OP_MUL ? AOA_FORWARD : AOA_INVERSE};
+        batch_add_ntt(b, r->ks->params, &it, 1);
         any = true;
       }
     }
     if (any) batch_barrier(b);
     // layer 2: the ops
     for (Request* r : run) {
-      const KeySet& ks = s->keysets[r->keyset];
+      const KeySet& ks = *r->ks;
       const aoa_op& op = r->ops[r->next];
       uint64_t* const* src = r->buf[r->cur];
       uint64_t* const* dst = r->buf[1 - r->cur];
@@ -352,93 +373,132 @@ static int server_step(aoa_server* s, hipStream_t st) {
       if (wsb > s->step_ws_bytes || descb > s->desc_cap) rc = set_error(AOA_ERR_WORKSPACE, "step exceeds its budget");
     }
     if (!rc) {
-      size_t used = batch_fill(b, (uint64_t*)s->step_ws, s->desc_host, (char*)s->desc_dev, s->desc_cap);
+      size_t used = batch_fill(b, (uint64_t*)s->step_ws, s->desc_host[slot], (char*)s->desc_dev, s->desc_cap);
       if (!used) rc = set_error(AOA_ERR_WORKSPACE, "descriptor buffer too small");
-      else if (hipMemcpyAsync(s->desc_dev, s->desc_host, used, hipMemcpyHostToDevice, st) != hipSuccess)
+      else if (hipMemcpyAsync(s->desc_dev, s->desc_host[slot], used, hipMemcpyHostToDevice, st) != hipSuccess)
         rc = set_error(AOA_ERR_HIP, "descriptor upload failed");
     }
     if (!rc) rc = batch_enqueue(b, st);
   }
-  // wait for the step (and the admission copies)
-  if (hipEventRecord(s->ev, st) != hipSuccess || hipEventSynchronize(s->ev) != hipSuccess)
-    rc = rc ? rc : set_error(AOA_ERR_HIP, "step synchronisation failed");
-  std::vector<Request*> done;
+  // advance the host state of the planned requests
+  std::vector<Request*> ended;
+  for (Request* r : run) {
+    if (rc) {
+      out.failed.push_back(r);
+      continue;
+    }
+    const aoa_op& op = r->ops[r->next];
+    r->cur = 1 - r->cur;
+    if (op.kind == AOA_OP_MUL) --r->level;
+    r->ntt = false;
+    ++r->next;
+    ++out.ops;
+    if (r->next >= r->ops.size()) out.finish.push_back(r);
+  }
+  // final copies to the caller's buffers (coefficient domain at the final level)
+  for (Request* r : out.finish) {
+    const size_t pb = (size_t)(r->level + 1) * r->ks->params->host.N * 8;
+    for (int y = 0; y < 2; ++y) (void)hipMemcpyAsync(r->out[y], r->buf[r->cur][y], pb, hipMemcpyDeviceToDevice, st);
+  }
+  out.rc = rc;
+  out.enqueued = !run.empty() || !out.finish.empty() || !out.failed.empty();
+  if (out.enqueued && hipEventRecord(s->ev[slot], st) != hipSuccess && !rc) {
+    out.rc = set_error(AOA_ERR_HIP, "step event record failed");
+  }
   {
     std::lock_guard<std::mutex> g(s->mu);
     s->steps += run.empty() ? 0 : 1;
-    for (Request* r : run) {
-      if (rc) {
-        r->state = FAILED;
-        done.push_back(r);
-        continue;
+    // leave the active set now: later steps never see these requests, and
+    // their pool blocks are reused only by work enqueued after this step
+    for (auto* list : {&out.finish, &out.failed}) {
+      for (Request* r : *list) {
+        s->pool.release(r->mem);
+        r->mem = nullptr;
+        s->active.erase(std::remove(s->active.begin(), s->active.end(), r), s->active.end());
       }
-      const aoa_op& op = r->ops[r->next];
-      r->cur = 1 - r->cur;
-      if (op.kind == AOA_OP_MUL) --r->level;
-      r->ntt = false;
-      ++r->next;
-      ++s->ops_done;
-      if (r->next >= r->ops.size()) done.push_back(r);
     }
-    for (Request* r : finish_now) done.push_back(r);
   }
-  // final copies to the caller's buffers, then retire
-  if (!done.empty()) {
-    for (Request* r : done) {
-      if (r->state == FAILED) continue;
-      const size_t pb = (size_t)(r->level + 1) * s->keysets[r->keyset].params->host.N * 8;
-      for (int y = 0; y < 2; ++y) (void)hipMemcpyAsync(r->out[y], r->buf[r->cur][y], pb, hipMemcpyDeviceToDevice, st);
-    }
-    (void)hipEventRecord(s->ev, st);
-    (void)hipEventSynchronize(s->ev);
-    std::lock_guard<std::mutex> g(s->mu);
-    const double t = now_ms();
-    for (Request* r : done) {
-      if (r->state != FAILED) {
-        r->state = DONE;
-        ++s->completed;
-      }
-      r->t_done = t;
-      s->last_done = t;
-      s->pool.release(r->mem);
-      r->mem = nullptr;
-      s->active.erase(std::remove(s->active.begin(), s->active.end(), r), s->active.end());
+  return out;
+}
+
+// Waits for a step's event and reports its requests.
+static int retire_step(aoa_server* s, Step& step, int slot) {
+  int rc = step.rc;
+  if (step.enqueued && hipEventSynchronize(s->ev[slot]) != hipSuccess)
+    rc = rc ? rc : set_error(AOA_ERR_HIP, "step synchronisation failed");
+  std::lock_guard<std::mutex> g(s->mu);
+  const double t = now_ms();
+  auto finish = [&](Request* r, int state) {
+    r->state = state;
+    r->t_done = t;
+    s->last_done = t;
+    s->finished.push_back(r->id);
+    if (state == DONE) ++s->completed;
+  };
+  for (Request* r : step.failed) finish(r, FAILED);
+  for (Request* r : step.finish) finish(r, rc ? FAILED : DONE);
+  s->ops_done += rc ? 0 : step.ops;
+  // bounded retention: drop the oldest finished records
+  while (s->finished.size() > kRetainFinished) {
+    auto it = s->all.find(s->finished.front());
+    s->finished.pop_front();
+    if (it != s->all.end()) {
+      delete it->second;
+      s->all.erase(it);
     }
   }
   if (rc) {
     s->last_rc = rc;
     s->last_msg = aoa_last_error();
-    return -rc;
   }
-  std::lock_guard<std::mutex> g(s->mu);
-  return (int)(s->pending.size() + s->active.size());
+  step = Step();
+  return rc;
 }
 
 extern "C" int aoa_server_step(aoa_server* s, void* stream) {
   if (!s) return -set_error(AOA_ERR_ARG, "null server");
   if (s->running) return -set_error(AOA_ERR_ARG, "server thread is running");
-  return server_step(s, stream ? (hipStream_t)stream : s->own_stream);
+  Step step = enqueue_step(s, stream ? (hipStream_t)stream : s->own_stream, 0);
+  const int rc = retire_step(s, step, 0);
+  if (rc) return -rc;
+  std::lock_guard<std::mutex> g(s->mu);
+  return (int)(s->pending.size() + s->active.size());
 }
 
+// The serving thread keeps up to two steps in flight: it enqueues step t,
+// then waits for step t-1 and reports its requests while step t runs, so the
+// host's planning and descriptor upload overlap the device's work.
 extern "C" int aoa_server_start(aoa_server* s) {
   if (!s) return set_error(AOA_ERR_ARG, "null server");
   if (s->running) return AOA_OK;
   s->stop = false;
   s->running = true;
   s->worker = std::thread([s]() {
+    Step prev;
+    int slot = 0;
+    bool have_prev = false;
     while (true) {
       {
         std::unique_lock<std::mutex> g(s->mu);
         if (s->stop) break;
-        if (s->pending.empty() && s->active.empty())
+        if (!have_prev && s->pending.empty() && s->active.empty()) {
           s->cv.wait_for(g, std::chrono::milliseconds(1));
-        if (s->stop) break;
-        if (s->pending.empty() && s->active.empty()) continue;
+          continue;
+        }
       }
-      if (server_step(s, s->own_stream) < 0) {
-        // failed requests are marked; keep serving the others
+      // desc_host[slot] was last used by step t-2, whose event has completed
+      Step cur = enqueue_step(s, s->own_stream, slot);
+      if (have_prev) {
+        retire_step(s, prev, 1 - slot);  // failed requests are marked; keep serving the others
+        have_prev = false;
+      }
+      if (cur.enqueued) {
+        prev = std::move(cur);
+        have_prev = true;
+        slot = 1 - slot;
       }
     }
+    if (have_prev) retire_step(s, prev, 1 - slot);
   });
   return AOA_OK;
 }
@@ -460,11 +520,24 @@ extern "C" int aoa_server_result(aoa_server* s, int64_t req_id, int* state, int*
   if (!s) return set_error(AOA_ERR_ARG, "null server");
   std::lock_guard<std::mutex> g(s->mu);
   auto it = s->all.find(req_id);
-  if (it == s->all.end()) return set_error(AOA_ERR_ARG, "unknown request");
+  if (it == s->all.end()) return set_error(AOA_ERR_ARG, "unknown or released request");
   Request* r = it->second;
+  const bool fin = r->state >= DONE;
   if (state) *state = r->state;
+  // the level is final only once the request has finished
   if (level) *level = r->level;
-  if (latency_ms) *latency_ms = r->state >= DONE ? r->t_done - r->t_submit : -1.0;
+  if (latency_ms) *latency_ms = fin ? r->t_done - r->t_submit : -1.0;
+  return AOA_OK;
+}
+
+extern "C" int aoa_server_release(aoa_server* s, int64_t req_id) {
+  if (!s) return set_error(AOA_ERR_ARG, "null server");
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->all.find(req_id);
+  if (it == s->all.end()) return set_error(AOA_ERR_ARG, "unknown or released request");
+  if (it->second->state < DONE) return set_error(AOA_ERR_ARG, "request has not finished");
+  delete it->second;
+  s->all.erase(it);
   return AOA_OK;
 }
 
@@ -488,13 +561,14 @@ extern "C" int aoa_server_stats(aoa_server* s, int64_t* completed, int64_t* ops,
 extern "C" void aoa_server_destroy(aoa_server* s) {
   if (!s) return;
   aoa_server_stop(s);
+  (void)hipStreamSynchronize(s->own_stream);
   for (auto& kv : s->all) delete kv.second;
   if (s->plan) aoa_batch_destroy(s->plan);
   (void)hipFree(s->pool_mem);
   (void)hipFree(s->step_ws);
   (void)hipFree(s->desc_dev);
-  (void)hipHostFree(s->desc_host);
-  (void)hipEventDestroy(s->ev);
+  for (char* h : s->desc_host) (void)hipHostFree(h);
+  for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(s->own_stream);
   delete s;
 }

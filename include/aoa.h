@@ -230,6 +230,14 @@ int aoa_rotate_left(const aoa_params* params, void* stream, uint64_t* out0, uint
                     const uint64_t* c0, const uint64_t* c1, const uint64_t* rtk_b, const uint64_t* rtk_a,
                     int rot_slots, int level, void* workspace, size_t ws_bytes);
 
+/* Rotation by rot_slots slots to the right (0 <= rot_slots <= N/2): the
+ * rotation to the left by N/2 - rot_slots, with the right rotation key for
+ * rot_slots.  Replaces ckks_rotate_right (aoa-ntt/src/ckks/ckks.cpp:953-980,
+ * rotate_slots_right at aoa-ntt/src/ckks/encoder.cu:356-363). */
+int aoa_rotate_right(const aoa_params* params, void* stream, uint64_t* out0, uint64_t* out1,
+                     const uint64_t* c0, const uint64_t* c1, const uint64_t* rtk_b, const uint64_t* rtk_a,
+                     int rot_slots, int level, void* workspace, size_t ws_bytes);
+
 /* ---------------------------------------------------------------- tuning --
  * NTT split N = N1 x (N/N1) of a parameter set (N1 in {64, 128, 256}, N/N1 in
  * the same set).  No counterpart in the reference, whose split is fixed per N
@@ -357,7 +365,8 @@ int aoa_server_register(aoa_server* server, const aoa_params* params, const uint
                         int nrot, const int* rot_steps, const uint64_t* const* rtk_b, const uint64_t* const* rtk_a);
 /* Queues a request; input (c0, c1) at `level` (in_ntt: NTT domain), result
  * copied to (out0, out1) in the coefficient domain at the final level.
- * Inputs and outputs must stay valid until the request is done. */
+ * Inputs and outputs must stay valid until the request is done.  A request
+ * without ops returns its input as given. */
 int aoa_server_submit(aoa_server* server, int keyset, const uint64_t* c0, const uint64_t* c1, int level, int in_ntt,
                       const uint64_t* m0, const uint64_t* m1, const aoa_op* ops, int nops, uint64_t* out0,
                       uint64_t* out1, int64_t* req_id);
@@ -367,7 +376,12 @@ int aoa_server_step(aoa_server* server, void* stream);
 /* Background serving thread (own stream). */
 int aoa_server_start(aoa_server* server);
 int aoa_server_stop(aoa_server* server);
+/* State, final level (once DONE) and latency from submit to completion of a
+ * request.  Finished requests stay queryable until released, or until 65 536
+ * later requests have finished (the oldest records are dropped first). */
 int aoa_server_result(aoa_server* server, int64_t req_id, int* state, int* level, double* latency_ms);
+/* Drops the record of a finished (DONE or FAILED) request. */
+int aoa_server_release(aoa_server* server, int64_t req_id);
 int aoa_server_stats(aoa_server* server, int64_t* completed, int64_t* ops, int64_t* steps, double* span_ms,
                      size_t* pool_used, size_t* key_bytes);
 void aoa_server_destroy(aoa_server* server);

@@ -1120,4 +1120,14 @@ void orc_rotate_left(void* h, uint64_t* o0, uint64_t* o1, const uint64_t* c0, co
   std::memcpy(o1, out1.d.data(), qn * 8);
 }
 
+
+// ckks_rotate_right (ckks.cpp:953-980): rotate_slots_right is rotate_slots_left
+// by N/2 - rotSlots (encoder.cu:356-363), then the same key switch with the
+// right rotation key.
+void orc_rotate_right(void* h, uint64_t* o0, uint64_t* o1, const uint64_t* c0, const uint64_t* c1,
+                      const uint64_t* rtkb, const uint64_t* rtka, int rot_slots, int level) {
+  const Oracle& o = *(Oracle*)h;
+  orc_rotate_left(h, o0, o1, c0, c1, rtkb, rtka, (o.N >> 1) - rot_slots, level);
+}
+
 }  // extern "C"

@@ -50,6 +50,7 @@ def load():
         "orc_automorphism": (None, [_vp, _u64p, _u64p, ctypes.c_int, ctypes.c_int]),
         "orc_hommul": (None, [_vp] + [_u64p] * 6 + [ctypes.c_int, _u64p, _u64p, ctypes.c_int, ctypes.c_int]),
         "orc_rotate_left": (None, [_vp] + [_u64p] * 6 + [ctypes.c_int, ctypes.c_int]),
+        "orc_rotate_right": (None, [_vp] + [_u64p] * 6 + [ctypes.c_int, ctypes.c_int]),
         "orc_encode": (None, [_vp, _u64p, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_uint64,
                               ctypes.c_int]),
         "orc_decode": (None, [_vp, ctypes.POINTER(ctypes.c_double), _u64p, ctypes.c_int, ctypes.c_uint64]),
@@ -216,6 +217,14 @@ class Oracle:
         o1 = np.zeros(qn, dtype=np.uint64)
         self.lib.orc_rotate_left(self.h, _p(o0), _p(o1), _p(c[0]), _p(c[1]), _p(rtk[0]), _p(rtk[1]), rot,
                                  level)
+        return o0, o1
+
+    def rotate_right(self, c, rtk, rot, level):
+        qn = self.k * self.N
+        o0 = np.zeros(qn, dtype=np.uint64)
+        o1 = np.zeros(qn, dtype=np.uint64)
+        self.lib.orc_rotate_right(self.h, _p(o0), _p(o1), _p(c[0]), _p(c[1]), _p(rtk[0]), _p(rtk[1]), rot,
+                                  level)
         return o0, o1
 
     # ---- input helpers ------------------------------------------------------

@@ -8,6 +8,7 @@ limbs 0..l (the reference leaves stale values above the level).
 import numpy as np
 import pytest
 
+from aim import window_aimed_qb
 from orc import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -145,59 +146,18 @@ def test_rescale_modup_moddown_parity(gpu, shape):
         assert np.array_equal(P.to_host(out)[:(level + 1) * N], ref[:(level + 1) * N]), "ModDown level %d" % level
 
 
-def _window_aimed_qb(O, rng, level, frac=4):
-    """A QB polynomial whose special residues put many ModDown products
-    a_j c_ij (a_j = [x_bj (B/b_j)^-1]_bj, c_ij = (B/b_j) mod q_i) at
-    remainders inside the reference's deviation window [eps, delta_max eps)
-    of q_i, at its edges, or below a wider bound (delta_max from b_max q_i
-    alone): for one coefficient in `frac`, every j aims at a random limb
-    i <= level."""
-    N, k, kl = O.N, O.k, O.kl
-    inv_bhat = [int(v) for v in O.table(4)]
-    c = O.table(5).reshape(k, kl)
-    Q, B = O.primes[:k], O.primes[k:]
-    bmax = max(B)
-    bhat = [pow(inv_bhat[j], -1, B[j]) for j in range(kl)]
-    win = []
-    for i in range(level + 1):
-        q = Q[i]
-        nb = q.bit_length()
-        eps = (1 << nb) - q
-        # the window bound delta_max eps, and a wider one
-        x1 = (bmax - 1) * (q - 1) >> (2 * nb - 64)
-        thr = (x1 * ((1 << (2 * nb)) % q) // (q << 64) + 2) * eps
-        wide = ((bmax * q >> (2 * nb)) + 3) * eps
-        cinv = [pow(int(c[i, j]), -1, q) if int(c[i, j]) else 0 for j in range(kl)]
-        win.append((q, eps, thr, wide, cinv))
-    qb = O.random_poly(rng, k + kl).reshape(k + kl, N)
-    for cid in range(0, N, frac):
-        for j in range(kl):
-            q, eps, thr, wide, cinv = win[int(rng.integers(0, level + 1))]
-            if wide >= q or cinv[j] == 0:
-                continue
-            pick = int(rng.integers(0, 6))
-            target = [eps, thr - 1, thr, eps - 1, int(rng.integers(eps, thr)),
-                      int(rng.integers(eps, wide))][pick] % q
-            a = target * cinv[j] % q
-            m = (B[j] - 1 - a) // q
-            a += int(rng.integers(0, m + 1)) * q if m > 0 else 0
-            a %= B[j]
-            qb[k + j, cid] = a * bhat[j] % B[j]
-    return qb.reshape(-1)
-
-
 @pytest.mark.parametrize("shape", [(4096, 4, 5, 52), (16384, 8, 0, 48), (65536, 30, 0, 48)])
 def test_moddown_window_products_parity(gpu, shape):
     """ModDown on inputs whose cross-base products sit in or at the edges of
     the reference's Barrett deviation window: the screened fast path and the
-    per-term literal recomputation give the oracle's residues, rescale fused
-    or not (through HomMul), and the exact mode gives the oracle's exact
-    mode."""
+    per-term literal recomputation give the oracle's residues, and the exact
+    mode gives the oracle's exact mode.  (The fused rescale and the batched
+    kernel see such products through test_gpu_shapes.py's aimed keys.)"""
     P, O = _params(*shape)
     rng = np.random.default_rng(41)
     k, N = O.k, O.N
     for level in sorted({1, k - 1}):
-        qb = _window_aimed_qb(O, rng, level)
+        qb = window_aimed_qb(O, rng, level)
         for exact in (False, True):
             P.set_moddown_exact(exact)
             O.set_moddown_exact(exact)

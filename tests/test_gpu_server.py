@@ -67,16 +67,21 @@ def _requests(rng, sets, n):
     return reqs
 
 
-def test_server_alone_vs_mix(gpu):
+@pytest.mark.parametrize("shapes,nreq", [([(8192, 4), (16384, 8), (32768, 5)], 12),
+                                         ([(65536, 30), (32768, 16), (8192, 4)], 8)])
+def test_server_alone_vs_mix(gpu, shapes, nreq):
+    """Small shapes, and the serving workload's largest ones (N=2^16 with 30
+    limbs, N=2^15 with 16) mixed with a small one."""
     import torch
     from aoa_amd import Server
     rng = np.random.default_rng(41)
-    sets = [_setup(rng, 8192, 4), _setup(rng, 16384, 8), _setup(rng, 32768, 5)]
-    reqs = _requests(rng, sets, 12)
+    print("server shapes %s" % (shapes,), flush=True)
+    sets = [_setup(rng, N, k) for N, k in shapes]
+    reqs = _requests(rng, sets, nreq)
     refs = [_plain_chain(sets[s][0], sets[s][1], sets[s][2], c, m, lv, ops) for s, c, m, lv, ops in reqs]
 
     def run(max_active, stagger, burst=False):
-        srv = Server(pool_bytes=1 << 30, step_ws_bytes=1 << 30, max_active=max_active)
+        srv = Server(pool_bytes=2 << 30, step_ws_bytes=2 << 30, max_active=max_active)
         ks = [srv.register(P, evk, rtk) for P, evk, rtk in sets]
         outs, ids = [], []
         if not burst:
@@ -149,6 +154,13 @@ def test_server_admission_budget(gpu):
     ref, lv = _plain_chain(P, evk, rtk, outs[3][0], outs[3][1], P.k - 1, [("mul",), ("rot", 2)])
     nn = (lv + 1) * P.N
     assert np.array_equal(P.to_host(outs[3][2][0])[:nn], ref[0])
+    # a released record is gone; releasing twice is an error
+    srv.release(ids[0])
+    with pytest.raises(AoaError):
+        srv.result(ids[0])
+    with pytest.raises(AoaError):
+        srv.release(ids[0])
+    assert srv.result(ids[1])[0] == Server.DONE
     # a request larger than the whole pool is refused
     big = Server(pool_bytes=1 << 20, step_ws_bytes=1 << 28, max_active=4)
     kb = big.register(P, evk, rtk)

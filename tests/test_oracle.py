@@ -102,6 +102,70 @@ def test_reference_mod_restatements_agree():
             assert a < p and (a == v % p or (a - v) % p == (-(1 << nb) + p) % p)
 
 
+def _reference_mod_bigint(v, p, mu):
+    """common.cu:257-291 stated on Python integers, word by word: nb-bit digits
+    m0, m1 of v, the quotient estimate q = floor(floor(v / 2^(2nb-64)) MU / 2^64)
+    mod 2^64, the same digits of q p, r = (r0 - r1 p) mod 2^nb with the borrow
+    of the low digit, then one conditional subtraction."""
+    M64 = (1 << 64) - 1
+    nb = p.bit_length()
+    z = (1 << nb) - 1
+    m0 = v & z
+    m1 = (v >> nb) & M64
+    x = v >> (2 * nb - 64)
+    q = ((x & M64) * mu >> 64) + ((x >> 64) * mu)
+    q &= M64
+    qp = q * p
+    q0 = qp & z
+    q1 = (qp >> nb) & M64
+    r0 = (m0 - q0) & M64
+    r1 = (m1 - q1 - (1 if m0 < q0 else 0)) & M64
+    r = (r0 - r1 * p) & z
+    return r - p if r >= p else r
+
+
+def test_reference_mod_bigint_on_window_products():
+    """The oracle's `mod` (orc_mod128) equals a big-integer statement of the
+    reference's Barrett word for word on ModDown cross-base products a c
+    (a < b_j, c = (B/b_j) mod q_i) aimed at the deviation window of q_i, at
+    its edges and at random, at N=2^16 with 30 limbs of 48 bits (K=31) and at
+    the prec-52 layout.  Deviations occur and are exactly r - eps with
+    eps <= r < delta_max eps, r the exact remainder."""
+    from aim import window_bounds
+    rng = random.Random(8)
+    for shape in [(65536, 30, 31, 48), (4096, 4, 5, 52)]:
+        o = Oracle(*shape)
+        k, kl = o.k, o.kl
+        B = o.primes[k:]
+        mu = o.table(9)
+        win = window_bounds(o, k - 1)
+        dev = 0
+        total = 0
+        for i in range(k):
+            q, eps, thr, wide, cinv = win[i]
+            for t in range(1500):
+                j = rng.randrange(kl)
+                c = pow(cinv[j], -1, q)
+                if t % 3 == 2 or wide >= q:
+                    a = rng.randrange(B[j])
+                else:
+                    target = [eps, thr - 1, thr, eps - 1, rng.randrange(eps, thr)][t % 5] % q
+                    a = target * cinv[j] % q
+                    m = (B[j] - 1 - a) // q
+                    a += rng.randrange(m + 1) * q if m > 0 else 0
+                    a %= B[j]
+                v = a * c
+                got = o.mod128(i, v)
+                assert got == _reference_mod_bigint(v, q, int(mu[i])), (shape, i, a, c)
+                r = v % q
+                total += 1
+                if got != r:
+                    dev += 1
+                    assert got == r - eps and eps <= r < thr, (shape, i, a, c)
+        if shape[3] == 48:
+            assert dev > total // 20, (dev, total)  # the aimed products do deviate
+
+
 def test_moddown_fast_term_matches_reference_formula():
     """Outside [eps, delta_max*eps) the reference Barrett equals the exact
     remainder; the product's Shoup fast path relies on it.  The deviation is
@@ -471,3 +535,36 @@ def test_conjugation_is_the_inverse_automorphism():
         xs = [int(v) for v in x[rid * N:(rid + 1) * N]]
         ref = [xs[0]] + [(-xs[N - j]) % p for j in range(1, N)]
         assert [int(v) for v in z[rid * N:(rid + 1) * N]] == ref
+
+
+def test_aimed_key_reaches_moddown_window():
+    """The aimed switching key of the GPU window tests does what they rely on:
+    the special limbs of each key product INTT(X key_y), X = NTT(ModUp(INTT(a1
+    b1))), equal the target's, and the reference's and the exact ModDown then
+    give different HomMul residues."""
+    from aim import aimed_key, window_aimed_qb
+    o = Oracle(4096, 4, 5, 52)
+    rng = np.random.default_rng(5)
+    k, kl, N = o.k, o.kl, o.N
+    level = k - 1
+    c1 = [o.random_poly(rng, k) for _ in range(2)]
+    c2 = [o.random_poly(rng, k) for _ in range(2)]
+    tg = [window_aimed_qb(o, rng, level) for _ in range(2)]
+    keys = aimed_key(o, rng, c1[1], c2[1], level, tg)
+    d2 = np.zeros(k * N, dtype=np.uint64)
+    o.eltwise(2, d2, c1[1], c2[1], k)
+    x = np.zeros((k + kl) * N, dtype=np.uint64)
+    x[:k * N] = d2
+    o.ntt(x, 0, k, 1)
+    o.modup(x, level)
+    o.ntt(x, k, kl, 0)
+    for y in range(2):
+        p = np.zeros_like(x)
+        o.eltwise(2, p, x, keys[y], k + kl)
+        o.ntt(p, k, kl, 1)
+        assert np.array_equal(p[k * N:], tg[y][k * N:])
+    r = o.hommul(c1, c2, keys, level)
+    o.set_moddown_exact(True)
+    e = o.hommul(c1, c2, keys, level)
+    n = level * N
+    assert np.sum(r[0][:n] != e[0][:n]) > 100 and np.sum(r[1][:n] != e[1][:n]) > 100
