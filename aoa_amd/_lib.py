@@ -97,6 +97,10 @@ SIGNATURES.update({
     "aoa_hommul_relin": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                          ctypes.c_int, c_vp, ctypes.c_size_t]),
     "aoa_rho": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "aoa_modup_from": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int]),
+    "aoa_dot": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int]),
+    "aoa_dr2": (ctypes.c_int, [c_vp] * 9),
+    "aoa_rns_primes": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_u64p, ctypes.c_int]),
     "aoa_autotune_ntt": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "aoa_batch_create": (ctypes.c_int, [ctypes.POINTER(c_vp)]),

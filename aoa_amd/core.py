@@ -137,6 +137,22 @@ class Params:
     def moddown(self, out, qb, level, stream=None):
         check(self.lib.aoa_moddown(self.h, _stream(stream), _ptr(out), _ptr(qb), level))
 
+    def modup_from(self, qb, q, level, stream=None):
+        """Special limbs of the QB polynomial qb from Q limbs 0..level of q."""
+        check(self.lib.aoa_modup_from(self.h, _stream(stream), _ptr(qb), _ptr(q), level))
+
+    def dot(self, c, a, b, stream=None):
+        """c = sum_i a[i] b[i] over every QB residue (a, b: lists of QB tensors)."""
+        import torch
+        pa = torch.tensor([t.data_ptr() for t in a], dtype=torch.int64, device="cuda")
+        pb = torch.tensor([t.data_ptr() for t in b], dtype=torch.int64, device="cuda")
+        check(self.lib.aoa_dot(self.h, _stream(stream), _ptr(c), _ptr(pa), _ptr(pb), len(a)))
+        return pa, pb  # keep the pointer arrays alive until the stream has used them
+
+    def dr2(self, c21, c22, c23, a01, a02, a11, a12, stream=None):
+        check(self.lib.aoa_dr2(self.h, _stream(stream), _ptr(c21), _ptr(c22), _ptr(c23), _ptr(a01), _ptr(a02),
+                               _ptr(a11), _ptr(a12)))
+
     def automorphism(self, out, inp, nres, rot_slots, stream=None):
         check(self.lib.aoa_automorphism(self.h, _stream(stream), _ptr(out), _ptr(inp), nres, rot_slots))
 

@@ -351,6 +351,46 @@ extern "C" int aoa_modup(const aoa_params* h, void* stream, uint64_t* d_qb, int 
   return check_launch("aoa_modup");
 }
 
+extern "C" int aoa_modup_from(const aoa_params* h, void* stream, uint64_t* d_qb, const uint64_t* d_q, int level) {
+  if (!h || !d_qb || !d_q) return fail(AOA_ERR_ARG, "null argument");
+  if (level < 0 || level >= h->host.k) return fail(AOA_ERR_ARG, "level out of range");
+  const size_t N = (size_t)h->host.N, k = (size_t)h->host.k;
+  // the special limbs written must not overlap the Q limbs read (unless they
+  // are the same QB polynomial)
+  const uint64_t* w0 = d_qb + k * N;
+  const uint64_t* w1 = w0 + (size_t)h->host.K * N;
+  const uint64_t* r1 = d_q + (size_t)(level + 1) * N;
+  if (d_q != d_qb && r1 > w0 && d_q < w1) return fail(AOA_ERR_ARG, "input overlaps the special limbs written");
+  launch_modup(h->dev, d_q, d_qb + k * N, level, (hipStream_t)stream);
+  return check_launch("aoa_modup_from");
+}
+
+extern "C" int aoa_dot(const aoa_params* h, void* stream, uint64_t* c, const uint64_t* const* a,
+                       const uint64_t* const* b, int k) {
+  if (!h || !c || !a || !b) return fail(AOA_ERR_ARG, "null argument");
+  if (k < 0 || k > 1 << 20) return fail(AOA_ERR_ARG, "term count out of range");
+  launch_dot(h->dev, c, a, b, k, (hipStream_t)stream);
+  return check_launch("aoa_dot");
+}
+
+extern "C" int aoa_dr2(const aoa_params* h, void* stream, uint64_t* c21, uint64_t* c22, uint64_t* c23,
+                       const uint64_t* a01, const uint64_t* a02, const uint64_t* a11, const uint64_t* a12) {
+  if (!h || !c21 || !c22 || !c23 || !a01 || !a02 || !a11 || !a12) return fail(AOA_ERR_ARG, "null argument");
+  launch_dr2(h->dev, c21, c22, c23, a01, a02, a11, a12, (hipStream_t)stream);
+  return check_launch("aoa_dr2");
+}
+
+extern "C" int aoa_rns_primes(int k, int kl, int prec, uint64_t* out, int n) {
+  if (kl <= 0) kl = k + 1;
+  std::vector<uint64_t> pr;
+  const std::string err = select_rns_primes(k, kl, prec, &pr);
+  if (!err.empty()) return fail(AOA_ERR_UNSUPPORTED, err);
+  if (!out) return fail(AOA_ERR_ARG, "null output");
+  if (n < (int)pr.size()) return fail(AOA_ERR_ARG, "output too small");
+  std::memcpy(out, pr.data(), pr.size() * sizeof(uint64_t));
+  return AOA_OK;
+}
+
 extern "C" int aoa_moddown(const aoa_params* h, void* stream, uint64_t* d_q_out, const uint64_t* d_qb_in,
                            int level) {
   if (!h || !d_q_out || !d_qb_in) return fail(AOA_ERR_ARG, "null argument");

@@ -87,6 +87,10 @@ uint64_t galois_inverse(uint64_t galois, int N);
 
 // rho of a QB polynomial: Q limbs times (B mod q_i), special limbs zero.
 void launch_rho(const DevParams& P, uint64_t* out, const uint64_t* in, hipStream_t st);
+void launch_dot(const DevParams& P, uint64_t* c, const uint64_t* const* a, const uint64_t* const* b, int k,
+                hipStream_t st);
+void launch_dr2(const DevParams& P, uint64_t* c21, uint64_t* c22, uint64_t* c23, const uint64_t* a01,
+                const uint64_t* a02, const uint64_t* a11, const uint64_t* a12, hipStream_t st);
 
 // Descriptor-driven (stage-fused) forms used by batches: one launch covers
 // every descriptor of the table (device memory).

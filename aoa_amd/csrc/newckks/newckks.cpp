@@ -81,6 +81,55 @@ void CUDAManager::destroy() {
   is_init = 0;
 }
 
+void CUDAManager::gen_rns_primes(unsigned int k, unsigned int kl) {  // manager.cu:483-564
+  if (is_init && RNSQPrimes.size() == k && RNSBPrimes.size() == kl) return;
+  std::vector<uint64_t> all(k + kl);
+  ok(aoa_rns_primes((int)k, (int)kl, (int)scalingfactor, all.data(), (int)all.size()));
+  RNSQPrimes.assign(all.begin(), all.begin() + k);
+  RNSBPrimes.assign(all.begin() + k, all.end());
+}
+
+void CUDAManager::precompute() {  // manager.cu:566-759: done by aoa_params_create
+  if (!is_init) throw std::runtime_error("CUDAManager is not initialised");
+}
+
+// Product of the primes as little-endian 64-bit words.
+static std::vector<uint64_t> prime_product(const std::vector<uint64_t>& primes) {
+  std::vector<uint64_t> w(1, 1);
+  for (uint64_t p : primes) {
+    unsigned __int128 carry = 0;
+    for (auto& x : w) {
+      const unsigned __int128 t = (unsigned __int128)x * p + carry;
+      x = (uint64_t)t;
+      carry = t >> 64;
+    }
+    if (carry) w.push_back((uint64_t)carry);
+  }
+  return w;
+}
+std::vector<uint64_t> CUDAManager::get_q_product() { return prime_product(RNSQPrimes); }
+std::vector<uint64_t> CUDAManager::get_b_product() { return prime_product(RNSBPrimes); }
+
+void CUDAManager::execute_dot(Context* ctx, uint64_t* c, uint64_t** a, uint64_t** b, const int k) {
+  ok(aoa_dot(params(), ctx->get_stream(), c, a, b, k));
+}
+void CUDAManager::execute_rho(Context* ctx, uint64_t* b, uint64_t* a) {
+  ok(aoa_rho(params(), ctx->get_stream(), b, a));
+}
+void CUDAManager::execute_rescale(Context* ctx, uint64_t* a, uint64_t* b, const int level) {
+  ok(aoa_rescale(params(), ctx->get_stream(), a, b, level));
+}
+void CUDAManager::execute_modup(Context* ctx, uint64_t* a, uint64_t* b, const int level) {
+  ok(aoa_modup_from(params(), ctx->get_stream(), a, b, level));
+}
+void CUDAManager::execute_moddown(Context* ctx, uint64_t* a, uint64_t* b, const int level) {
+  ok(aoa_moddown(params(), ctx->get_stream(), a, b, level));
+}
+void CUDAManager::execute_dr2(Context* ctx, uint64_t* ct21, uint64_t* ct22, uint64_t* ct23, const uint64_t* ct01,
+                              const uint64_t* ct02, const uint64_t* ct11, const uint64_t* ct12) {
+  ok(aoa_dr2(params(), ctx->get_stream(), ct21, ct22, ct23, ct01, ct02, ct11, ct12));
+}
+
 size_t poly_get_size(poly_bases b) { return (size_t)CUDAManager::N * CUDAManager::get_n_residues(b) * sizeof(uint64_t); }
 
 // ------------------------------------------------------------ Context ------
@@ -101,10 +150,36 @@ Context::~Context() {
 
 // ------------------------------------------------------------ poly layer ---
 
+// Polynomials come from the device's stream-ordered memory pool where the
+// device has one (poly_t.cpp:10-31 takes cudaMallocAsync when supported), so
+// allocating and freeing the temporaries of every ckks_* call neither blocks
+// the host nor synchronises the stream.  The pool keeps freed memory for
+// reuse instead of returning it at each synchronisation.
+static bool stream_alloc() {
+  static int supported = -1;
+  if (supported < 0) {
+    int dev = 0, attr = 0;
+    supported = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&attr, hipDeviceAttributeMemoryPoolsSupported, dev) == hipSuccess && attr) {
+      hipMemPool_t pool;
+      if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        supported = 1;
+      }
+    }
+  }
+  return supported == 1;
+}
+
 void poly_init(Context* ctx, poly_t* a, poly_bases b) {
   if (a->state != NONINITIALIZED) return;
   if (!CUDAManager::is_init) throw std::runtime_error("CUDAManager is not initialised");
-  hip_ok(hipMalloc((void**)&a->d_coefs, poly_get_size(b)), "hipMalloc");
+  if (stream_alloc())
+    hip_ok(hipMallocAsync((void**)&a->d_coefs, poly_get_size(b), ctx->get_stream()), "hipMallocAsync");
+  else
+    hip_ok(hipMalloc((void**)&a->d_coefs, poly_get_size(b)), "hipMalloc");
   hip_ok(hipMemsetAsync(a->d_coefs, 0, poly_get_size(b), ctx->get_stream()), "hipMemsetAsync");
   a->state = RNSSTATE;
   a->base = b;
@@ -112,8 +187,12 @@ void poly_init(Context* ctx, poly_t* a, poly_bases b) {
 
 void poly_free(Context* ctx, poly_t* a) {
   if (!a || a->state == NONINITIALIZED) return;
-  (void)hipStreamSynchronize(ctx->get_stream());
-  (void)hipFree(a->d_coefs);
+  if (stream_alloc()) {
+    (void)hipFreeAsync(a->d_coefs, ctx->get_stream());
+  } else {
+    (void)hipStreamSynchronize(ctx->get_stream());
+    (void)hipFree(a->d_coefs);
+  }
   a->d_coefs = nullptr;
   a->state = NONINITIALIZED;
 }
@@ -223,6 +302,24 @@ void poly_mul_add(Context* ctx, poly_t* d, poly_t* a, poly_t* b, poly_t* c) {  /
   COMMONEngine::execute(ctx, c, FORWARD);
   eltwise(ctx, AOA_MULADD, d, a, b, c, base);
   d->state = a->state;
+}
+
+// poly_t.cpp:302-342.  The reference's kernel covers every QB residue
+// whatever the operands' base, so QB-base operands are required here.
+void poly_dr2(Context* ctx, poly_t* ct21, poly_t* ct22, poly_t* ct23, poly_t* ct01, poly_t* ct02, poly_t* ct11,
+              poly_t* ct12) {
+  const poly_bases base = ct01->base;
+  for (poly_t* p : {ct21, ct22, ct23, ct01, ct02, ct11, ct12}) {
+    poly_init(ctx, p, base);
+    if (p->base != QBBase) throw std::runtime_error("poly_dr2 needs QB-base polynomials");
+  }
+  for (poly_t* p : {ct01, ct02, ct11, ct12}) COMMONEngine::execute(ctx, p, FORWARD);
+  CUDAManager::execute_dr2(ctx, ct21->d_coefs, ct22->d_coefs, ct23->d_coefs, ct01->d_coefs, ct02->d_coefs,
+                           ct11->d_coefs, ct12->d_coefs);
+  for (poly_t* p : {ct21, ct22, ct23}) {
+    p->state = TRANSSTATE;
+    p->base = base;
+  }
 }
 
 void poly_negate(Context* ctx, poly_t* a) {

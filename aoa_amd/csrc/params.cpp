@@ -76,6 +76,16 @@ static std::string select_primes(int k, int kl, int prec, std::vector<aoa_prime_
   return "";
 }
 
+std::string select_rns_primes(int k, int kl, int prec, std::vector<uint64_t>* primes) {
+  std::vector<aoa_prime_entry> q, b;
+  std::string err = select_primes(k, kl, prec, &q, &b);
+  if (!err.empty()) return err;
+  primes->clear();
+  for (auto& e : q) primes->push_back(e.p);
+  for (auto& e : b) primes->push_back(e.p);
+  return "";
+}
+
 // The words (w0, w1, w2) of floor(c 2^96 / p) for c < p < 2^63, by long
 // division one 32-bit word at a time.
 void moddown_screen_words(uint64_t c, uint64_t p, uint32_t w[3]) {

@@ -99,6 +99,9 @@ struct HostParams {
 // Builds every table.  Returns an empty string on success, else the reason
 // (unsupported precision, bucket too small, N not supported by a prime).
 std::string build_host_params(int N, int k, int kl, int prec, HostParams* out);
+// The k Q primes then the kl special primes of the reference's layout for
+// `prec` (empty string, or the reason the layout does not exist).
+std::string select_rns_primes(int k, int kl, int prec, std::vector<uint64_t>* primes);
 
 // Modular inverse of a mod p (p prime, a != 0 mod p).
 uint64_t inv_mod(uint64_t a, uint64_t p);

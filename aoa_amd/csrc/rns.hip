@@ -578,6 +578,55 @@ void launch_rho(const DevParams& P, uint64_t* out, const uint64_t* in, hipStream
   k_rho<<<dim3((unsigned)(total / 256)), 256, 0, st>>>(P, out, in);
 }
 
+// ------------------------------------------------------- dot product, dr2 ----
+// c = sum_i a[i] b[i] over every residue of a QB polynomial: the products are
+// summed in 128 bits and reduced once with the reference's Barrett reduction
+// (dot_prod, aoa-ntt/src/cuda/manager.cu:53-67, with add128 and mod of
+// common.cu:195-213, :257-291), so a sum outside Barrett's exact range keeps
+// the reference's result.  a and b are device arrays of k device pointers.
+__global__ __launch_bounds__(256) void k_dot(DevParams P, uint64_t* c, const uint64_t* const* a,
+                                             const uint64_t* const* b, int k) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const DevPrime d = P.primes[(int)(((size_t)blockIdx.x * 256) >> P.logN)];
+  uint64_t hi = 0, lo = 0;
+  for (int i = 0; i < k; ++i) {
+    const uint64_t x = a[i][e], y = b[i][e];
+    const uint64_t l = x * y;
+    lo += l;
+    hi += mulhi64(x, y) + (lo < l ? 1u : 0u);
+  }
+  c[e] = ref_barrett(hi, lo, d.p, d.mu, (int)d.nb);
+}
+
+void launch_dot(const DevParams& P, uint64_t* c, const uint64_t* const* a, const uint64_t* const* b, int k,
+                hipStream_t st) {
+  const size_t total = (size_t)(P.k + P.K) * P.N;
+  k_dot<<<dim3((unsigned)(total / 256)), 256, 0, st>>>(P, c, a, b, k);
+}
+
+// The reference's dr2 kernel (aoa-ntt/src/cuda/manager.cu:370-395) over every
+// residue of a QB polynomial, with its mulmod (Barrett of the full product):
+// c21 = a01 a11, c22 = (a01 a12)(a02 a11) -- a product where the tensor's
+// middle term would be a sum; kept as the reference computes it --,
+// c23 = a02 a12.
+__global__ __launch_bounds__(256) void k_dr2(DevParams P, uint64_t* c21, uint64_t* c22, uint64_t* c23,
+                                             const uint64_t* a01, const uint64_t* a02, const uint64_t* a11,
+                                             const uint64_t* a12) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const DevPrime d = P.primes[(int)(((size_t)blockIdx.x * 256) >> P.logN)];
+  auto mm = [&](uint64_t x, uint64_t y) { return ref_barrett(mulhi64(x, y), x * y, d.p, d.mu, (int)d.nb); };
+  const uint64_t x01 = a01[e], x02 = a02[e], x11 = a11[e], x12 = a12[e];
+  c21[e] = mm(x01, x11);
+  c22[e] = mm(mm(x01, x12), mm(x02, x11));
+  c23[e] = mm(x02, x12);
+}
+
+void launch_dr2(const DevParams& P, uint64_t* c21, uint64_t* c22, uint64_t* c23, const uint64_t* a01,
+                const uint64_t* a02, const uint64_t* a11, const uint64_t* a12, hipStream_t st) {
+  const size_t total = (size_t)(P.k + P.K) * P.N;
+  k_dr2<<<dim3((unsigned)(total / 256)), 256, 0, st>>>(P, c21, c22, c23, a01, a02, a11, a12);
+}
+
 // ---------------------------------------------------------- automorphism ----
 // Forward map (encoder.cu:300-322): X^c -> X^(c g mod 2N), negated on wrap.
 // Gather form: out[j] = in[t] if t = j g^-1 mod 2N < N, else -in[t - N].

@@ -173,6 +173,34 @@ int aoa_rescale(const aoa_params* params, void* stream, uint64_t* c0, uint64_t* 
  * (aoa-ntt/src/cuda/manager.cu:178-249). */
 int aoa_modup(const aoa_params* params, void* stream, uint64_t* d_qb, int level);
 
+/* ModUp from a separate input: the special limbs of the QB polynomial d_qb
+ * from Q limbs 0..level of d_q (coefficient domain; d_q may equal d_qb, or be
+ * any Q polynomial not overlapping those special limbs).  Replaces
+ * CUDAManager::execute_modup(ctx, b, a, level) with b != a
+ * (aoa-ntt/include/newckks/cuda/manager.h:147-151, manager.cu:225-249). */
+int aoa_modup_from(const aoa_params* params, void* stream, uint64_t* d_qb, const uint64_t* d_q, int level);
+
+/* c = sum_{i<k} a[i] b[i] on every residue of QB polynomials (k+kl limbs):
+ * products summed in 128 bits, reduced once with the reference's Barrett
+ * reduction (a sum outside its exact range gives the reference's value).
+ * a and b are DEVICE arrays of k device pointers, as the reference's kernel
+ * reads them.  Replaces CUDAManager::execute_dot
+ * (aoa-ntt/include/newckks/cuda/manager.h:128-133, manager.cu:53-84). */
+int aoa_dot(const aoa_params* params, void* stream, uint64_t* c, const uint64_t* const* a,
+            const uint64_t* const* b, int k);
+
+/* The reference's dr2 on every residue of QB polynomials: c21 = a01 a11,
+ * c22 = (a01 a12)(a02 a11) (a product, as the reference computes it),
+ * c23 = a02 a12, each product reduced with the reference's Barrett.
+ * Replaces CUDAManager::execute_dr2 (manager.h:158-166, manager.cu:370-420). */
+int aoa_dr2(const aoa_params* params, void* stream, uint64_t* c21, uint64_t* c22, uint64_t* c23,
+            const uint64_t* a01, const uint64_t* a02, const uint64_t* a11, const uint64_t* a12);
+
+/* Host only: the k Q primes followed by the kl special primes (kl <= 0: k+1)
+ * of the reference's layout for prec, as CUDAManager::gen_rns_primes selects
+ * them (aoa-ntt/src/cuda/manager.cu:426-530).  n >= k + kl. */
+int aoa_rns_primes(int k, int kl, int prec, uint64_t* out, int n);
+
 /* ModDown: d_q_out limbs 0..level <- ModDown of the QB polynomial d_qb_in
  * (coefficient domain).  Replaces CUDAManager::execute_moddown
  * (aoa-ntt/src/cuda/manager.cu:312-367), including its Barrett reduction of

@@ -10,6 +10,8 @@
 #include "../../aoa.h"
 #include "../defines.h"
 
+class Context;
+
 class CUDAManager {
  public:
   static int is_init;
@@ -31,6 +33,34 @@ class CUDAManager {
       default: return (int)(RNSQPrimes.size() + RNSBPrimes.size());
     }
   }
+  // Selects the k Q primes and kl special primes of the layout for
+  // scalingfactor (manager.cu:483-564) into RNSQPrimes / RNSBPrimes.  init()
+  // calls it; a later call with the initialised sizes leaves them as they are.
+  static void gen_rns_primes(unsigned int k, unsigned int kl);
+  // The RNS constants are computed when the parameter set is created
+  // (aoa_params_create), so this only checks that init() has run.
+  static void precompute();
+  // Products of the Q and special primes.  The reference returns NTL::ZZ,
+  // which this build does not use: here they are little-endian 64-bit words.
+  static std::vector<uint64_t> get_q_product();
+  static std::vector<uint64_t> get_b_product();
+
+  // The reference's device entry points (manager.h:128-166), on raw residue
+  // pointers and the context's stream.  execute_dot: c = sum_i a[i] b[i] over
+  // the QB residues, a and b device arrays of k device pointers.  execute_rho:
+  // b = rho(a).  execute_rescale: a, b rescaled in place from level.
+  // execute_modup: special limbs of the QB polynomial a from Q limbs
+  // 0..level of b (a == b in poly_modup).  execute_moddown: Q limbs 0..level
+  // of a from the QB polynomial b.  execute_dr2: the reference's dr2 kernel
+  // over the QB residues.
+  static void execute_dot(Context* ctx, uint64_t* c, uint64_t** a, uint64_t** b, const int k);
+  static void execute_rho(Context* ctx, uint64_t* b, uint64_t* a);
+  static void execute_rescale(Context* ctx, uint64_t* a, uint64_t* b, const int level);
+  static void execute_modup(Context* ctx, uint64_t* a, uint64_t* b, const int level);
+  static void execute_moddown(Context* ctx, uint64_t* a, uint64_t* b, const int level);
+  static void execute_dr2(Context* ctx, uint64_t* ct21, uint64_t* ct22, uint64_t* ct23, const uint64_t* ct01,
+                          const uint64_t* ct02, const uint64_t* ct11, const uint64_t* ct12);
+
   // The underlying C-ABI handle.
   static aoa_params* params();
   // Extension (no reference counterpart): exact ModDown reduction instead of

@@ -1121,6 +1121,41 @@ void orc_rotate_left(void* h, uint64_t* o0, uint64_t* o1, const uint64_t* c0, co
 }
 
 
+// dot_prod (manager.cu:53-67): v = sum_i a[i][t] b[i][t] with mullazy and
+// add128 (common.cu:157-213, no reduction inside the sum), then mod(v, rid),
+// over every residue of a QB polynomial.  a, b: k host pointers each.
+void orc_dot(void* h, uint64_t* c, const uint64_t* const* a, const uint64_t* const* b, int k) {
+  const Oracle& o = *(Oracle*)h;
+  const size_t n = (size_t)o.R() * o.N;
+  for (size_t t = 0; t < n; ++t) {
+    const int rid = (int)(t / o.N);
+    u128 v;
+    v.hi = v.lo = 0;
+    for (int i = 0; i < k; ++i) {
+      const u128 m = mullazy(a[i][t], b[i][t]);
+      u128 z;
+      z.lo = v.lo + m.lo;
+      z.hi = v.hi + m.hi + (z.lo < v.lo);
+      v = z;
+    }
+    c[t] = ref_mod(o, v, rid);
+  }
+}
+
+// dr2 (manager.cu:370-395) over every residue of a QB polynomial: ct21 =
+// ct01 ct11, ct22 = (ct01 ct12)(ct02 ct11), ct23 = ct02 ct12, each a mulmod.
+void orc_dr2(void* h, uint64_t* c21, uint64_t* c22, uint64_t* c23, const uint64_t* a01, const uint64_t* a02,
+             const uint64_t* a11, const uint64_t* a12) {
+  const Oracle& o = *(Oracle*)h;
+  const size_t n = (size_t)o.R() * o.N;
+  for (size_t t = 0; t < n; ++t) {
+    const int rid = (int)(t / o.N);
+    c21[t] = mulmod(o, a01[t], a11[t], rid);
+    c22[t] = mulmod(o, mulmod(o, a01[t], a12[t], rid), mulmod(o, a02[t], a11[t], rid), rid);
+    c23[t] = mulmod(o, a02[t], a12[t], rid);
+  }
+}
+
 // ckks_rotate_right (ckks.cpp:953-980): rotate_slots_right is rotate_slots_left
 // by N/2 - rotSlots (encoder.cu:356-363), then the same key switch with the
 // right rotation key.

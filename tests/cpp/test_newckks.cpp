@@ -3,6 +3,7 @@
 // random slot values, encrypt -> op -> decrypt, compared with the FP64 result
 // within the reference's bounds (ERRBOUND 0.005, tests.cpp:34; rotations 0.01,
 // tests.cpp:1578, :1635).  Prints one line per case; exit status = failures.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -585,8 +586,142 @@ static void run_adv(int k, int kl, int N, int prec) {
   CUDAManager::destroy();
 }
 
+// CUDAManager's device entry points on raw residue pointers
+// (aoa-ntt/include/newckks/cuda/manager.h:128-166) against the poly_t layer
+// that the reference builds on them, and the prime helpers.
+static void manager_statics() {
+  const int k = 4, kl = 5, N = 4096, prec = 52;
+  CUDAManager::init(k, kl, N, prec);
+  Context* ctx = new Context();
+  const int R = k + kl;
+  std::mt19937_64 gen(11);
+  auto primes = [&]() {
+    std::vector<uint64_t> p(CUDAManager::RNSQPrimes);
+    p.insert(p.end(), CUDAManager::RNSBPrimes.begin(), CUDAManager::RNSBPrimes.end());
+    return p;
+  }();
+  auto random_poly = [&](poly_t* p, poly_bases b) {
+    poly_init(ctx, p, b);
+    const int n = CUDAManager::get_n_residues(b);
+    std::vector<uint64_t> h((size_t)n * N);
+    for (int r = 0; r < n; ++r)
+      for (int i = 0; i < N; ++i) h[(size_t)r * N + i] = gen() % primes[r];
+    poly_copy_to_device(ctx, p, h.data());
+  };
+  auto host = [&](poly_t* p) {
+    uint64_t* h = poly_copy_to_host(ctx, p);
+    std::vector<uint64_t> v(h, h + (size_t)CUDAManager::get_n_residues(p->base) * N);
+    free(h);
+    return v;
+  };
+  auto same = [&](const std::vector<uint64_t>& a, const std::vector<uint64_t>& b, int r0, int r1) {
+    return std::equal(a.begin() + (size_t)r0 * N, a.begin() + (size_t)r1 * N, b.begin() + (size_t)r0 * N);
+  };
+  const int level = k - 2;
+  // execute_modup(ctx, out QB, in Q, level) with separate buffers == poly_modup
+  {
+    poly_t q, ref, raw;
+    random_poly(&q, QBase);
+    poly_modup(ctx, &ref, &q, level);
+    poly_init(ctx, &raw, QBBase);
+    CUDAManager::execute_modup(ctx, raw.d_coefs, q.d_coefs, level);
+    report("ManagerExecuteModUp", same(host(&ref), host(&raw), k, R) ? 0.0 : 1.0, 0.0);
+    for (poly_t* p : {&q, &ref, &raw}) poly_free(ctx, p);
+  }
+  // execute_moddown(ctx, out Q, in QB, level) == poly_moddown
+  {
+    poly_t qb, ref, raw;
+    random_poly(&qb, QBBase);
+    poly_moddown(ctx, &ref, &qb, level);
+    poly_init(ctx, &raw, QBase);
+    CUDAManager::execute_moddown(ctx, raw.d_coefs, qb.d_coefs, level);
+    report("ManagerExecuteModDown", same(host(&ref), host(&raw), 0, level + 1) ? 0.0 : 1.0, 0.0);
+    for (poly_t* p : {&qb, &ref, &raw}) poly_free(ctx, p);
+  }
+  // execute_rescale and execute_rho == poly_rescale, poly_rho
+  {
+    poly_t a, b, a2, b2, x, rho1, rho2;
+    random_poly(&a, QBase);
+    random_poly(&b, QBase);
+    poly_copy(ctx, &a2, &a);
+    poly_copy(ctx, &b2, &b);
+    poly_rescale(ctx, &a, &b, level);
+    CUDAManager::execute_rescale(ctx, a2.d_coefs, b2.d_coefs, level);
+    bool ok = same(host(&a), host(&a2), 0, level) && same(host(&b), host(&b2), 0, level);
+    random_poly(&x, QBBase);
+    poly_rho(ctx, &rho1, &x);
+    poly_init(ctx, &rho2, QBBase);
+    CUDAManager::execute_rho(ctx, rho2.d_coefs, x.d_coefs);
+    ok = ok && same(host(&rho1), host(&rho2), 0, R);
+    report("ManagerExecuteRescaleRho", ok ? 0.0 : 1.0, 0.0);
+    for (poly_t* p : {&a, &b, &a2, &b2, &x, &rho1, &rho2}) poly_free(ctx, p);
+  }
+  // execute_dot with one term == poly_mul; poly_dr2 == its poly_mul composition
+  {
+    poly_t a0, a1, b0, b1, c, m, m2, m3, t21, t22, t23;
+    for (poly_t* p : {&a0, &a1, &b0, &b1}) random_poly(p, QBBase);
+    poly_init(ctx, &c, QBBase);
+    uint64_t* pa[1] = {a0.d_coefs};
+    uint64_t* pb[1] = {b0.d_coefs};
+    uint64_t** da = nullptr;
+    uint64_t** db = nullptr;
+    (void)hipMalloc((void**)&da, sizeof(pa));
+    (void)hipMalloc((void**)&db, sizeof(pb));
+    (void)hipMemcpy(da, pa, sizeof(pa), hipMemcpyHostToDevice);
+    (void)hipMemcpy(db, pb, sizeof(pb), hipMemcpyHostToDevice);
+    CUDAManager::execute_dot(ctx, c.d_coefs, da, db, 1);
+    c.state = RNSSTATE;  // the product of raw residues, read back as stored
+    poly_init(ctx, &m, QBBase);
+    (void)hipStreamSynchronize(ctx->get_stream());
+    std::vector<uint64_t> hc((size_t)R * N), ha(hc.size()), hb(hc.size());
+    (void)hipMemcpy(hc.data(), c.d_coefs, hc.size() * 8, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(ha.data(), a0.d_coefs, ha.size() * 8, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(hb.data(), b0.d_coefs, hb.size() * 8, hipMemcpyDeviceToHost);
+    bool ok = true;
+    for (size_t t = 0; t < hc.size(); ++t)
+      ok = ok && hc[t] == (uint64_t)((unsigned __int128)ha[t] * hb[t] % primes[t / N]);
+    (void)hipFree(da);
+    (void)hipFree(db);
+    poly_dr2(ctx, &t21, &t22, &t23, &a0, &a1, &b0, &b1);
+    poly_mul(ctx, &m, &a0, &b0);
+    ok = ok && same(host(&t21), host(&m), 0, R);
+    poly_mul(ctx, &m2, &a0, &b1);
+    poly_mul(ctx, &m3, &a1, &b0);
+    poly_mul(ctx, &m, &m2, &m3);
+    ok = ok && same(host(&t22), host(&m), 0, R);
+    poly_mul(ctx, &m, &a1, &b1);
+    ok = ok && same(host(&t23), host(&m), 0, R);
+    report("ManagerExecuteDotDr2", ok ? 0.0 : 1.0, 0.0);
+    for (poly_t* p : {&a0, &a1, &b0, &b1, &c, &m, &m2, &m3, &t21, &t22, &t23}) poly_free(ctx, p);
+  }
+  // gen_rns_primes keeps the initialised bases; the products are the primes'
+  {
+    const std::vector<uint64_t> q0 = CUDAManager::RNSQPrimes, b0 = CUDAManager::RNSBPrimes;
+    CUDAManager::gen_rns_primes(k, kl);
+    bool ok = CUDAManager::RNSQPrimes == q0 && CUDAManager::RNSBPrimes == b0;
+    CUDAManager::precompute();
+    for (int which = 0; which < 2; ++which) {
+      const std::vector<uint64_t> w = which ? CUDAManager::get_b_product() : CUDAManager::get_q_product();
+      const std::vector<uint64_t>& ps = which ? b0 : q0;
+      int bits = 0;
+      for (uint64_t p : ps) bits += 64 - __builtin_clzll(p);
+      const int wbits = 64 * ((int)w.size() - 1) + (64 - __builtin_clzll(w.back()));
+      ok = ok && wbits <= bits && wbits > bits - (int)ps.size();
+      for (uint64_t p : ps) {  // every prime divides the product
+        unsigned __int128 r = 0;
+        for (size_t i = w.size(); i-- > 0;) r = ((r << 64) | w[i]) % p;
+        ok = ok && r == 0;
+      }
+    }
+    report("ManagerPrimesAndProducts", ok ? 0.0 : 1.0, 0.0);
+  }
+  delete ctx;
+  CUDAManager::destroy();
+}
+
 int main(int argc, char** argv) {
   const bool quick = argc > 1 && std::string(argv[1]) == "quick";
+  manager_statics();
   mul_n4096_k4();
   run_params(2, 2, 2048, 55, true);
   run_params(3, 3, 4096, 55, true);

@@ -51,6 +51,8 @@ def load():
         "orc_hommul": (None, [_vp] + [_u64p] * 6 + [ctypes.c_int, _u64p, _u64p, ctypes.c_int, ctypes.c_int]),
         "orc_rotate_left": (None, [_vp] + [_u64p] * 6 + [ctypes.c_int, ctypes.c_int]),
         "orc_rotate_right": (None, [_vp] + [_u64p] * 6 + [ctypes.c_int, ctypes.c_int]),
+        "orc_dot": (None, [_vp, _u64p, ctypes.POINTER(_u64p), ctypes.POINTER(_u64p), ctypes.c_int]),
+        "orc_dr2": (None, [_vp] + [_u64p] * 7),
         "orc_encode": (None, [_vp, _u64p, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_uint64,
                               ctypes.c_int]),
         "orc_decode": (None, [_vp, ctypes.POINTER(ctypes.c_double), _u64p, ctypes.c_int, ctypes.c_uint64]),
@@ -218,6 +220,21 @@ class Oracle:
         self.lib.orc_rotate_left(self.h, _p(o0), _p(o1), _p(c[0]), _p(c[1]), _p(rtk[0]), _p(rtk[1]), rot,
                                  level)
         return o0, o1
+
+    def dot(self, a, b):
+        """sum_i a[i] b[i] over every QB residue, the reference's dot_prod."""
+        k = len(a)
+        pa = (_u64p * max(1, k))(*[_p(x) for x in a])
+        pb = (_u64p * max(1, k))(*[_p(x) for x in b])
+        c = np.zeros((self.k + self.kl) * self.N, dtype=np.uint64)
+        self.lib.orc_dot(self.h, _p(c), pa, pb, k)
+        return c
+
+    def dr2(self, a01, a02, a11, a12):
+        n = (self.k + self.kl) * self.N
+        c = [np.zeros(n, dtype=np.uint64) for _ in range(3)]
+        self.lib.orc_dr2(self.h, _p(c[0]), _p(c[1]), _p(c[2]), _p(a01), _p(a02), _p(a11), _p(a12))
+        return c
 
     def rotate_right(self, c, rtk, rot, level):
         qn = self.k * self.N

@@ -274,3 +274,35 @@ def test_hommul_exact_moddown_parity(gpu, shape):
     r0, _ = O.hommul(c1, c2, evk, level)
     if shape[1] >= 30:
         assert not np.array_equal(r0[:n], ref0[:n])
+
+
+@pytest.mark.parametrize("shape", [(2048, 2, 2, 55), (8192, 6, 0, 48), (65536, 30, 0, 48)])
+def test_manager_entry_points_parity(gpu, shape):
+    """CUDAManager's execute_dot (1, 3 and 8 terms; sums of 63-bit products
+    beyond Barrett's range included), execute_dr2 and execute_modup from a
+    separate input, against the oracle."""
+    P, O = _params(*shape)
+    rng = np.random.default_rng(23)
+    k, kl, N = O.k, O.kl, O.N
+    R = k + kl
+    for kk in (1, 3, 8):
+        a = [O.random_poly(rng, R) for _ in range(kk)]
+        b = [O.random_poly(rng, R) for _ in range(kk)]
+        out = P.zeros(R)
+        keep = P.dot(out, [P.to_device(x) for x in a], [P.to_device(x) for x in b])
+        assert np.array_equal(P.to_host(out), O.dot(a, b)), "dot %d terms" % kk
+        del keep
+    x = [O.random_poly(rng, R) for _ in range(4)]
+    c = [P.zeros(R) for _ in range(3)]
+    P.dr2(*c, *[P.to_device(v) for v in x])
+    for got, ref in zip(c, O.dr2(*x)):
+        assert np.array_equal(P.to_host(got), ref), "dr2"
+    for level in sorted({0, k - 1}):
+        q = O.random_poly(rng, k)
+        qb = np.zeros(R * N, dtype=np.uint64)
+        qb[:k * N] = q
+        O.modup(qb, level)
+        dq = P.to_device(q)
+        dqb = P.zeros(R)
+        P.modup_from(dqb, dq, level)
+        assert np.array_equal(P.to_host(dqb)[k * N:], qb[k * N:]), "ModUp from level %d" % level

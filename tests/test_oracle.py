@@ -166,6 +166,47 @@ def test_reference_mod_bigint_on_window_products():
             assert dev > total // 20, (dev, total)  # the aimed products do deviate
 
 
+def test_dot_and_dr2_bigint():
+    """orc_dot (manager.cu:53-67) is the reference's Barrett, stated on Python
+    integers, of the 128-bit sum of the products; below 2^(2nb) that is the
+    sum's remainder or the remainder minus eps (above it the reference's
+    quotient estimate wraps and the result is not congruent, as in the
+    reference); orc_dr2 (manager.cu:370-395) is the three exact products,
+    the middle one (a01 a12 mod p)(a02 a11 mod p) mod p."""
+    rng = np.random.default_rng(21)
+    for shape in [(2048, 2, 2, 55), (2048, 3, 4, 48)]:
+        o = Oracle(*shape)
+        R, N = o.k + o.kl, o.N
+        mu = [int(x) for x in o.table(9)]
+        for kk in (1, 3, 8):
+            a = [o.random_poly(rng, R) for _ in range(kk)]
+            b = [o.random_poly(rng, R) for _ in range(kk)]
+            if kk == 8:  # sums near the top of the range
+                for x in a + b:
+                    for r in range(R):
+                        x[r * N:r * N + 64] = o.primes[r] - 1 - rng.integers(0, 4, 64).astype(np.uint64)
+            c = o.dot(a, b)
+            for t in range(0, R * N, 7):
+                r = t // N
+                p = o.primes[r]
+                v = sum(int(x[t]) * int(y[t]) for x, y in zip(a, b)) % (1 << 128)
+                got = int(c[t])
+                assert got == _reference_mod_bigint(v, p, mu[r]), (shape, kk, t)
+                eps = (1 << p.bit_length()) - p
+                if v < 1 << (2 * p.bit_length()):  # beyond 2^(2nb) the quotient estimate wraps
+                    assert got == v % p or got == (v % p) - eps, (shape, kk, t)
+                if kk == 1:
+                    assert got == v % p
+        x = [o.random_poly(rng, R) for _ in range(4)]
+        c21, c22, c23 = o.dr2(*x)
+        for t in range(0, R * N, 5):
+            p = o.primes[t // N]
+            a01, a02, a11, a12 = (int(v[t]) for v in x)
+            assert int(c21[t]) == a01 * a11 % p
+            assert int(c22[t]) == (a01 * a12 % p) * (a02 * a11 % p) % p
+            assert int(c23[t]) == a02 * a12 % p
+
+
 def test_moddown_fast_term_matches_reference_formula():
     """Outside [eps, delta_max*eps) the reference Barrett equals the exact
     remainder; the product's Shoup fast path relies on it.  The deviation is
