@@ -44,6 +44,9 @@ SIGNATURES = {
                                       ctypes.c_size_t]),
     "aoa_hommul_relin_rescale": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                                  ctypes.c_int, c_vp, ctypes.c_size_t]),
+    "aoa_hommul_relin_rescale_noalloc": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                                         c_vp, ctypes.c_int]),
+    "aoa_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "aoa_rotate_left": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int,
                                         ctypes.c_int, c_vp, ctypes.c_size_t]),
     "aoa_rotate_right": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int,

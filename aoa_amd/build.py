@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libaoa.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["params.cpp", "ntt.hip", "rns.hip", "encoder.hip", "sampler.hip", "capi.cpp", "batch.cpp", "tune.cpp", "server.cpp"]
+SOURCES = ["params.cpp", "ntt.hip", "rns.hip", "encoder.hip", "sampler.hip", "capi.cpp", "batch.cpp", "tune.cpp", "server.cpp", "noalloc.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           "-I" + CSRC, "-I" + os.path.join(ROOT, "include")]

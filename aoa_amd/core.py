@@ -167,6 +167,13 @@ class Params:
             self.h, _stream(stream), _ptr(c3[0]), _ptr(c3[1]), _ptr(c1[0]), _ptr(c1[1]), _ptr(c2[0]),
             _ptr(c2[1]), _ptr(evk[0]), _ptr(evk[1]), level, _ptr(w), nb))
 
+    def hommul_noalloc(self, c3, c1, c2, evk, level, stream=None):
+        """HomMul+relin+rescale with the library's static device workspace
+        (no allocation, no workspace argument)."""
+        check(self.lib.aoa_hommul_relin_rescale_noalloc(
+            self.h, _stream(stream), _ptr(c3[0]), _ptr(c3[1]), _ptr(c1[0]), _ptr(c1[1]), _ptr(c2[0]),
+            _ptr(c2[1]), _ptr(evk[0]), _ptr(evk[1]), level))
+
     def set_moddown_exact(self, exact):
         """False (default): the reference's ModDown reduction, bit-exact residues;
         True: exact reduction."""

@@ -488,9 +488,9 @@ extern "C" int aoa_keyswitch(const aoa_params* h, void* stream, uint64_t* out0, 
   return check_launch("aoa_keyswitch");
 }
 
-static int hommul_core(const aoa_params* h, void* stream, uint64_t* c3_0, uint64_t* c3_1, const uint64_t* a0,
-                       const uint64_t* a1, const uint64_t* b0, const uint64_t* b1, const uint64_t* evk_b,
-                       const uint64_t* evk_a, int level, void* ws, size_t ws_bytes, int rescale) {
+int aoa::hommul_core(const aoa_params* h, void* stream, uint64_t* c3_0, uint64_t* c3_1, const uint64_t* a0,
+                     const uint64_t* a1, const uint64_t* b0, const uint64_t* b1, const uint64_t* evk_b,
+                     const uint64_t* evk_a, int level, void* ws, size_t ws_bytes, int rescale) {
   if (!h || !c3_0 || !c3_1 || !a0 || !a1 || !b0 || !b1 || !evk_b || !evk_a || !ws)
     return fail(AOA_ERR_ARG, "null argument");
   if (!aligned16(c3_0) || !aligned16(c3_1) || !aligned16(a0) || !aligned16(a1) || !aligned16(b0) ||
@@ -574,14 +574,14 @@ extern "C" int aoa_hommul_relin_rescale(const aoa_params* h, void* stream, uint6
                                         const uint64_t* a0, const uint64_t* a1, const uint64_t* b0,
                                         const uint64_t* b1, const uint64_t* evk_b, const uint64_t* evk_a,
                                         int level, void* ws, size_t ws_bytes) {
-  return hommul_core(h, stream, c3_0, c3_1, a0, a1, b0, b1, evk_b, evk_a, level, ws, ws_bytes, 1);
+  return aoa::hommul_core(h, stream, c3_0, c3_1, a0, a1, b0, b1, evk_b, evk_a, level, ws, ws_bytes, 1);
 }
 
 extern "C" int aoa_hommul_relin(const aoa_params* h, void* stream, uint64_t* c3_0, uint64_t* c3_1,
                                 const uint64_t* a0, const uint64_t* a1, const uint64_t* b0, const uint64_t* b1,
                                 const uint64_t* evk_b, const uint64_t* evk_a, int level, void* ws,
                                 size_t ws_bytes) {
-  return hommul_core(h, stream, c3_0, c3_1, a0, a1, b0, b1, evk_b, evk_a, level, ws, ws_bytes, 0);
+  return aoa::hommul_core(h, stream, c3_0, c3_1, a0, a1, b0, b1, evk_b, evk_a, level, ws, ws_bytes, 0);
 }
 
 // rho (aoa-ntt/src/cuda/manager.cu:149-176): Q limbs times B mod q_i, special

@@ -23,6 +23,12 @@ int check_launch(const char* what);
 // 5^rot mod 2N: the Galois element of a left rotation by rot slots.
 uint64_t galois_of(int rot_slots, int N);
 
+// HomMul + relinearisation (+ rescale when rescale != 0) of (a0, a1) and
+// (b0, b1) at `level` into (c3_0, c3_1), with ws_bytes of caller workspace.
+int hommul_core(const aoa_params* h, void* stream, uint64_t* c3_0, uint64_t* c3_1, const uint64_t* a0,
+                const uint64_t* a1, const uint64_t* b0, const uint64_t* b1, const uint64_t* evk_b,
+                const uint64_t* evk_a, int level, void* ws, size_t ws_bytes, int rescale);
+
 // Batch planning with caller-provided memory (the serving loop rebuilds a
 // batch every step from preallocated workspace and descriptor buffers).
 void batch_sizes(const aoa_batch* b, size_t* ws_bytes, size_t* desc_bytes);

@@ -238,6 +238,20 @@ int aoa_hommul_relin_rescale(const aoa_params* params, void* stream, uint64_t* c
                              const uint64_t* c2_1, const uint64_t* evk_b, const uint64_t* evk_a, int level,
                              void* workspace, size_t ws_bytes);
 
+/* aoa_hommul_relin_rescale without a workspace argument, for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  It has no counterpart in the reference.
+ * The workspace is a static device array of aoa_noalloc_scratch_bytes()
+ * (144 MiB) per GPU, held from library load until the process ends; shapes
+ * whose aoa_workspace_bytes exceed it fail with AOA_ERR_WORKSPACE.  The call
+ * itself allocates and frees nothing.  All calls on one GPU share that array:
+ * calls that overlap on one device (other streams, other threads) race. */
+int aoa_hommul_relin_rescale_noalloc(const aoa_params* params, void* stream, uint64_t* c3_0, uint64_t* c3_1,
+                                     const uint64_t* c1_0, const uint64_t* c1_1, const uint64_t* c2_0,
+                                     const uint64_t* c2_1, const uint64_t* evk_b, const uint64_t* evk_a,
+                                     int level);
+size_t aoa_noalloc_scratch_bytes(void);
+
 /* HomMul + relinearisation without the rescale: outputs in the coefficient
  * domain, limbs 0..level (level unchanged).  Replaces ckks_mul_without_rescale
  * (aoa-ntt/src/ckks/ckks.cpp:273-332). */
