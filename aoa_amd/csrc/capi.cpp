@@ -73,7 +73,10 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
       {H.enc_w.data(), H.enc_w.size() * 8, 0},
       {H.rot_group.data(), H.rot_group.size() * 4, 0},
       {H.gauss_cdt.data(), H.gauss_cdt.size() * 8, 0},
-      {H.md_const.data(), H.md_const.size() * sizeof(MdConst), 0},
+      {H.md_terms.data(), H.md_terms.size() * sizeof(MdTerm), 0},
+      {H.md_thr.data(), H.md_thr.size() * 4, 0},
+      {H.md_red.data(), H.md_red.size() * 8, 0},
+      {H.md_fin.data(), H.md_fin.size() * 8, 0},
   };
   const int nsec = sizeof(secs) / sizeof(secs[0]);
   size_t total = 0;
@@ -112,7 +115,10 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
   D.enc_w = (const double*)(b + secs[10].off);
   D.rot_group = (const uint32_t*)(b + secs[11].off);
   D.gauss_cdt = (const uint64_t*)(b + secs[12].off);
-  D.md_const = (const uint32_t*)(b + secs[13].off);
+  D.md_terms = (const uint32_t*)(b + secs[13].off);
+  D.md_thr = (const uint32_t*)(b + secs[14].off);
+  D.md_red = (const uint64_t*)(b + secs[15].off);
+  D.md_fin = (const uint64_t*)(b + secs[16].off);
   *out = h;
   return AOA_OK;
 }

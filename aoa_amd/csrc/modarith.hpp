@@ -102,6 +102,24 @@ AOA_HD uint64_t mul_barrett(uint64_t a, uint64_t b, uint64_t p, uint64_t mu, int
   return barrett_reduce(mulhi64(a, b), a * b, p, mu, nb);
 }
 
+// T = s0 + s1 2^28 + s2 2^56 mod q, left up to four multiples of q too large
+// (in [0, 5q)), for T < 2^(62+nb) with 33 <= nb <= 56 the bit length of q and
+// mu = floor(2^(63+nb) / q) (below 2^64).  The quotient estimate is the high
+// word of T' mu, where T' = floor(s0 / 2^(nb-1)) + floor(s1 / 2^(nb-29)) +
+// s2 2^(57-nb) is at most 2 below T / 2^(nb-1) (two floors), and the low
+// 32x32 partial product is left out (at most 1 more).  So the estimate never
+// exceeds floor(T / q) and is at most 4 below it; T minus its multiple of q
+// is then below 2^64 and computed modulo 2^64.
+AOA_HD uint64_t reduce_sums_lazy(uint64_t s0, uint64_t s1, uint64_t s2, uint64_t q, uint64_t mu, int nb) {
+  const uint64_t t = (s0 >> (nb - 1)) + (s1 >> (nb - 29)) + (s2 << (57 - nb));
+  const uint32_t t0 = (uint32_t)t, t1 = (uint32_t)(t >> 32);
+  const uint32_t m0 = (uint32_t)mu, m1 = (uint32_t)(mu >> 32);
+  const uint64_t x = (uint64_t)t0 * m1;
+  const uint64_t y = (uint64_t)t1 * m0 + (uint32_t)x;
+  const uint64_t qh = (uint64_t)t1 * m1 + (x >> 32) + (y >> 32);
+  return s0 + (s1 << 28) + (s2 << 56) - qh * q;
+}
+
 // Full 128-bit product from four 32x32->64 multiply-adds.
 AOA_HD void mul128(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {
   const uint64_t al = (uint32_t)a, ah = a >> 32, bl = (uint32_t)b, bh = b >> 32;

@@ -215,6 +215,10 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
   }
   P.bhat_mod_q.assign((size_t)k * kl * 2, 0);
   P.md_const.assign((size_t)k * kl, MdConst{});
+  P.md_terms.assign((size_t)kl * k, MdTerm{});
+  P.md_thr.assign((size_t)k, 0);
+  P.md_red.assign((size_t)k * 2, 0);
+  P.md_fin.assign((size_t)k * k * 2, 0);
   P.inv_b_mod_q.assign((size_t)k * 2, 0);
   P.b_mod_q.assign((size_t)k * 2, 0);
   uint64_t bmax = 0;
@@ -276,11 +280,25 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
         m.w2 = w[2];
         m.thr = sthr;
         m.pad = 2;  // moddown_screen's bias 2^33, as its high word
+        P.md_terms[(size_t)j * k + i] = MdTerm{m.cl, m.ch, m.cs, m.w0, m.w1, m.w2};
       }
+      P.md_thr[i] = sthr;
     }
     uint64_t inv = inv_mod(bfull, qi);
     P.inv_b_mod_q[2 * i] = inv;
     P.inv_b_mod_q[2 * i + 1] = shoup_quot(inv, qi);
+    if (d.nb >= 33 && d.nb <= 56) {
+      P.md_red[2 * i] = (uint64_t)((((unsigned __int128)1) << (63 + d.nb)) / qi);
+      P.md_red[2 * i + 1] = 5 * qi;
+    } else {
+      P.md_red[2 * i] = 0;
+      P.md_red[2 * i + 1] = qi;
+    }
+    for (int l = 0; l < k; ++l) {
+      const uint64_t f = mulm(inv, P.inv_ql_mod_qi[((size_t)l * k + i) * 2], qi);
+      P.md_fin[((size_t)l * k + i) * 2] = f;
+      P.md_fin[((size_t)l * k + i) * 2 + 1] = shoup_quot(f, qi);
+    }
     P.b_mod_q[2 * i] = bfull;
     P.b_mod_q[2 * i + 1] = shoup_quot(bfull, qi);
   }

@@ -71,7 +71,19 @@ struct DevParams {
   const double* enc_w;           // [2N][2]: exp(2 pi i t / 2N), the encoder's roots of unity
   const uint32_t* rot_group;     // [N/2]: 5^j mod 2N
   const uint64_t* gauss_cdt;     // [2 kGaussHalf]: round(2^64 P(e <= j - kGaussHalf)), e = round(3.2 z)
-  const uint32_t* md_const;      // [k][K][8]: ModDown product constants (MdConst), Q primes below 2^56
+  const uint32_t* md_terms;      // [K][k][6]: ModDown product constants (MdTerm), Q primes below 2^56
+  const uint32_t* md_thr;        // [k]: ModDown screen threshold of q_i
+  // [k][2]: per Q prime, floor(2^(63+nb) / q_i) (reduce_sums_lazy; 0 for q0)
+  // and the bound of ModDown's lazy sums, 5 q_i (q0: q0, whose sums are exact)
+  const uint64_t* md_red;
+  const uint64_t* md_fin;        // [k][k][2]: (l,i) -> B^-1 q_l^-1 mod q_i, Shoup
+};
+
+// The device layout of the ModDown product constants: MdConst's first six
+// words, ordered [j][i] so that the limbs i, i+4, i+8, i+12 of one wave's
+// pass sit at fixed offsets from one scalar pointer per term j.
+struct MdTerm {
+  uint32_t cl, ch, cs, w0, w1, w2;
 };
 
 // ModDown constants of one (q_i, b_j) pair for c = (B/b_j) mod q_i < 2^56:
@@ -93,7 +105,11 @@ struct HostParams {
   std::vector<double> enc_w;        // 2N complex roots, interleaved (re, im)
   std::vector<uint32_t> rot_group;  // N/2 entries
   std::vector<uint64_t> gauss_cdt;  // 2 kGaussHalf entries
-  std::vector<MdConst> md_const;    // k * K entries
+  std::vector<MdConst> md_const;    // k * K entries, [i][j]
+  std::vector<MdTerm> md_terms;     // K * k entries, [j][i]
+  std::vector<uint32_t> md_thr;     // k entries
+  std::vector<uint64_t> md_red;     // k * 2
+  std::vector<uint64_t> md_fin;     // k * k * 2
 };
 
 // Builds every table.  Returns an empty string on success, else the reason

@@ -325,67 +325,81 @@ __device__ __forceinline__ uint64_t moddown_term(uint64_t a, const tw2 w, const 
   return ref_mulmod_shoup(a, w.w, w.q, dq.p, dq.mu, (int)dq.nb, exact ? 0 : dq.quirk_lo, exact ? 0 : dq.quirk_thr);
 }
 
-typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
-
-// Sums over j in [j0, j1) of the ModDown products for NL limbs li[] with
-// primes below 2^56, reduced mod q_i, into res[].  Each product a c
-// (a < 2^56 from LDS, c = (B/b_j) mod q_i) is accumulated exactly from the
-// 28-bit halves, a c = al cl + (as cs - al cl - ah ch) 2^28 + ah ch 2^56
+// Sums over j in [j0, j1) of the ModDown products for the NL limbs
+// i0, i0 + 4, ... (primes below 2^57), reduced mod q_i up to four multiples
+// of q_i too many (in [0, 5 q_i)), into res[].  Each
+// product a c (a < 2^56 from LDS, c = (B/b_j) mod q_i) is accumulated exactly
+// from the 28-bit halves, a c = al cl + (as cs - al cl - ah ch) 2^28 + ah ch 2^56
 // (as = al + ah, cs = cl + ch): three 32x32->64 multiply-adds into 64-bit
 // sums that cannot overflow for up to 64 terms, one 128-bit reduction per
 // limb.  Exact sums equal the reference's term-by-term modular sum whenever
-// no term deviates; SCREEN runs moddown_screen on every product, and a lane
-// whose limb passed it for any j (one term in 10 000 to 300 000 at 48-bit
-// q_i, growing with 2^48 - q_i) screens that limb's terms again and adds
-// ref(a c) - (a c mod q_i) for each one that passes.
+// no term deviates.  SCREEN runs moddown_screen on every product and records,
+// per lane and limb, whether any product passed it and, per wave, the terms j
+// for which some lane's product did (jm: the scalar OR of the compare masks).
+// A lane whose limb has a passing product (one term in 10 000 to 300 000 at
+// 48-bit q_i, growing with 2^48 - q_i) screens only the terms in jm again and
+// adds ref(a c) - (a c mod q_i) for each one that passes.
 template <int NL, bool SCREEN>
-__device__ __forceinline__ void moddown_conv(const DevParams& P, const uint64_t* aux, int c, const int* li,
+__device__ __forceinline__ void moddown_conv(const DevParams& P, const uint64_t* aux, int c, int i0,
                                              int j0, int j1, uint64_t* res) {
   constexpr uint32_t M28 = (1u << 28) - 1;
-  const int K = P.K;
-  const cptr<u32x8> tab = as_const(reinterpret_cast<const u32x8*>(P.md_const));
+  constexpr int TW = (int)(sizeof(MdTerm) / 4);
+  const int k = P.k;
+  const cptr<uint32_t> terms = as_const(P.md_terms);
   uint64_t s0[NL], s2[NL], sm[NL];
   bool hit[NL];
   uint32_t thr[NL];
-  // the screen's bias comes from the table so that it stays a register
-  // operand of the multiply-add instead of a separate addition
-  const uint64_t bias = (uint64_t)tab[(size_t)li[0] * K][7] << 32;
+  uint64_t jm = 0;  // bit j - j0: the screen passed term j in some lane (wave-uniform)
+  // moddown_screen's bias, opaque to the compiler so that it stays the addend
+  // of the first multiply-add instead of becoming an addition of its own
+  uint64_t bias = 2ull << 32;
+  asm volatile("" : "+v"(bias));
 #pragma unroll
   for (int n = 0; n < NL; ++n) {
     s0[n] = s2[n] = sm[n] = 0;
     hit[n] = false;
-    thr[n] = tab[(size_t)li[n] * K][6];
+    thr[n] = as_const(P.md_thr)[i0 + n * BC_GROUPS];
   }
-#pragma unroll 2
   for (int j = j0; j < j1; ++j) {
     const uint64_t a = aux[j * BC_COEF + c];
     const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
     const uint32_t al = a0 & M28, ah = (uint32_t)(a >> 28), as = al + ah;
+    const cptr<uint32_t> t = terms + ((size_t)j * k + i0) * TW;
+    uint64_t any = 0;
 #pragma unroll
     for (int n = 0; n < NL; ++n) {
-      const u32x8 e = tab[(size_t)li[n] * K + j];
+      const cptr<uint32_t> e = t + n * BC_GROUPS * TW;
       s0[n] += (uint64_t)al * e[0];
       s2[n] += (uint64_t)ah * e[1];
       sm[n] += (uint64_t)as * e[2];
-      if (SCREEN) hit[n] |= moddown_screen(a0, a1, e[3], e[4], e[5], bias) <= thr[n];
+      if (SCREEN) {
+        const bool h = moddown_screen(a0, a1, e[3], e[4], e[5], bias) <= thr[n];
+        hit[n] |= h;
+        any |= __builtin_amdgcn_ballot_w64(h);
+      }
     }
+    if (SCREEN && any) jm |= 1ull << (j - j0);
   }
   const tw2* bq = reinterpret_cast<const tw2*>(P.bhat_mod_q);
+  const int K = P.K;
 #pragma unroll
   for (int n = 0; n < NL; ++n) {
-    const DevPrime dq = ld_prime(P.primes, li[n]);
-    uint64_t h, l;
-    join_28(s0[n], sm[n] - s0[n] - s2[n], s2[n], &h, &l);
-    uint64_t r = reduce128(h, l, dq);
+    const int i = i0 + n * BC_GROUPS;
+    const uint64_t p = P.primes[i].p;
+    const cptr<uint64_t> red = as_const(P.md_red) + 2 * i;
+    uint64_t r = reduce_sums_lazy(s0[n], sm[n] - s0[n] - s2[n], s2[n], p, red[0], (int)P.primes[i].nb);
     if (SCREEN && hit[n]) {
       // Add ref(a c) - (a c mod q_i) for the terms that pass the screen
       // again; every other term lies outside the deviation window, where
       // the two agree.
-      const tw2* ci = bq + (size_t)li[n] * K;
-      for (int j = j0; j < j1; ++j) {
+      const DevPrime dq = ld_prime(P.primes, i);
+      for (uint64_t m = 4 * p; m >= p; m >>= 1) r = r >= m ? r - m : r;
+      const tw2* ci = bq + (size_t)i * K;
+      for (uint64_t m = jm; m; m &= m - 1) {
+        const int j = j0 + __builtin_ctzll(m);
         const uint64_t a = aux[j * BC_COEF + c];
-        const u32x8 e = tab[(size_t)li[n] * K + j];
-        if (moddown_screen((uint32_t)a, (uint32_t)(a >> 32), e[3], e[4], e[5], bias) <= thr[n]) {
+        const cptr<uint32_t> e = terms + ((size_t)j * k + i) * TW;
+        if (moddown_screen((uint32_t)a, (uint32_t)(a >> 32), e[3], e[4], e[5]) <= thr[n]) {
           const tw2 w = ld_tw2(ci, j);
           const uint64_t x = mul_shoup(a, w.w, w.q, dq.p);
           const uint64_t f = ref_barrett(mulhi64(a, w.w), a * w.w, dq.p, dq.mu, (int)dq.nb);
@@ -398,12 +412,12 @@ __device__ __forceinline__ void moddown_conv(const DevParams& P, const uint64_t*
 }
 
 template <int NL>
-__device__ __forceinline__ void moddown_convs(const DevParams& P, const uint64_t* aux, int c, const int* li,
+__device__ __forceinline__ void moddown_convs(const DevParams& P, const uint64_t* aux, int c, int i0,
                                               int j0, int j1, uint64_t* res) {
   if (P.moddown_exact)
-    moddown_conv<NL, false>(P, aux, c, li, j0, j1, res);
+    moddown_conv<NL, false>(P, aux, c, i0, j0, j1, res);
   else
-    moddown_conv<NL, true>(P, aux, c, li, j0, j1, res);
+    moddown_conv<NL, true>(P, aux, c, i0, j0, j1, res);
 }
 
 __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t* inQ, const uint64_t* inB,
@@ -437,10 +451,13 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
     }
     return acc;
   };
+  // (x_i - acc) B^-1 (+ add_i) mod q_i from the sum acc of limb i, which is
+  // below md_red[i][1] (5 q_i, or q0 for the canonical q0 sums).
   auto finish = [&](int i, uint64_t acc) -> uint64_t {
     const uint64_t p = P.primes[i].p;
     const tw2 bi = ibq[i];
-    uint64_t v = mul_shoup(sub_mod(inQ[(size_t)i * N + cid], acc, p), bi.w, bi.q, p);
+    const uint64_t y = inQ[(size_t)i * N + cid] + as_const(P.md_red)[2 * i + 1] - acc;
+    uint64_t v = mul_shoup(y, bi.w, bi.q, p);
     if (add) v = add_mod(v, add[(size_t)i * N + cid], p);
     return v;
   };
@@ -465,15 +482,15 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
       };
       uint64_t r[4];
       if (left >= 4 && narrow(4)) {
-        moddown_convs<4>(P, aux, c, li, 0, K, r);
+        moddown_convs<4>(P, aux, c, i, 0, K, r);
         for (int n = 0; n < 4; ++n) store(li[n], r[n]);
         i += 4 * BC_GROUPS;
       } else if (left >= 2 && narrow(2)) {
-        moddown_convs<2>(P, aux, c, li, 0, K, r);
+        moddown_convs<2>(P, aux, c, i, 0, K, r);
         for (int n = 0; n < 2; ++n) store(li[n], r[n]);
         i += 2 * BC_GROUPS;
       } else {
-        moddown_convs<1>(P, aux, c, li, 0, K, r);
+        moddown_convs<1>(P, aux, c, i, 0, K, r);
         store(i, r[0]);
         i += BC_GROUPS;
       }
@@ -493,23 +510,29 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
     if (P.primes[level].nb > 56) {
       r = conv(level, j0, j1 > j0 ? j1 : j0);
     } else if (j1 > j0) {
-      const int li[1] = {level};
-      moddown_convs<1>(P, aux, c, li, j0, j1, &r);
+      moddown_convs<1>(P, aux, c, level, j0, j1, &r);
     }
     part[g * BC_COEF + c] = r;
   }
   __syncthreads();
   const uint64_t pl = P.primes[level].p;
+  const uint64_t ml = as_const(P.md_red)[2 * level + 1];
   uint64_t acc = 0;
+  // four partial sums below ml each: their sum stays below 2^64 and below ml
+  // once reduced by the multiples of pl that ml allows
 #pragma unroll
-  for (int t = 0; t < BC_GROUPS; ++t) acc = add_mod(acc, part[t * BC_COEF + c], pl);
+  for (int t = 0; t < BC_GROUPS; ++t) acc = add_mod(acc, part[t * BC_COEF + c], ml);
   const uint64_t top = finish(level, acc);
+  // out_i = ((x_i - acc_i) B^-1 + add_i - top) q_l^-1
+  //       = (x_i - acc_i) (B^-1 q_l^-1) + (add_i - top) q_l^-1  (mod q_i); top < q_l < q_i
   const tw2* iql = reinterpret_cast<const tw2*>(P.inv_ql_mod_qi) + (size_t)level * k;
+  const tw2* fin = reinterpret_cast<const tw2*>(P.md_fin) + (size_t)level * k;
   limbs(level, [&](int i, uint64_t acc) {
     const uint64_t p = P.primes[i].p;
-    const uint64_t v = finish(i, acc);
-    const tw2 s = iql[i];
-    out[(size_t)i * N + cid] = mul_shoup(sub_mod(v, top, p), s.w, s.q, p);
+    const uint64_t y = inQ[(size_t)i * N + cid] + as_const(P.md_red)[2 * i + 1] - acc;
+    const uint64_t z = p + (add ? add[(size_t)i * N + cid] : 0) - top;
+    const tw2 f = ld_tw2(fin, i), s = ld_tw2(iql, i);
+    out[(size_t)i * N + cid] = add_mod(mul_shoup(y, f.w, f.q, p), mul_shoup(z, s.w, s.q, p), p);
   });
 }
 

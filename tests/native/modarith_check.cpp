@@ -232,6 +232,104 @@ int64_t chk_moddown_accumulate(int N, int k, int kl, int prec, int64_t trials, u
   return bad;
 }
 
+// reduce_sums_lazy against 128-bit arithmetic for every Q prime below 2^57:
+// the kernel's sums of K random terms; 64 terms at the largest operands
+// (a = 2^56 - 1, c = q - 1, the bound T < 2^(62+nb)); values T = m q + d
+// (d in -2..2) at the quotient steps, split with s0 and s1 at the top of
+// their ranges so that both floors of the estimate drop the most.  The result
+// must be congruent to T and below 5q.  Returns the number of failures.
+int64_t chk_moddown_reduce(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
+  HostParams H;
+  if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  std::mt19937_64 rng(seed);
+  int64_t bad = 0;
+  const uint32_t M28 = (1u << 28) - 1;
+  typedef unsigned __int128 u128;
+  for (int i = 0; i < k; ++i) {
+    const DevPrime& d = H.dprimes[i];
+    if (d.nb > 56) continue;
+    const uint64_t q = d.p, mu = H.md_red[2 * i];
+    if (H.md_red[2 * i + 1] != 5 * q) ++bad;
+    if (mu != (uint64_t)((((u128)1) << (63 + d.nb)) / q)) ++bad;
+    auto check = [&](uint64_t s0, uint64_t s1, uint64_t s2) {
+      const u128 T = (u128)s0 + ((u128)s1 << 28) + ((u128)s2 << 56);
+      const uint64_t r = reduce_sums_lazy(s0, s1, s2, q, mu, (int)d.nb);
+      if (r >= 5 * q || (u128)r % q != T % q) ++bad;
+    };
+    for (int64_t t = 0; t < trials; ++t) {
+      // the kernel's sums
+      uint64_t s0 = 0, s2 = 0, sm = 0;
+      const int nt = t == 0 ? 64 : kl;
+      for (int j = 0; j < nt; ++j) {
+        const uint64_t c = t == 0 ? q - 1 : H.bhat_mod_q[((size_t)i * kl + j) * 2];
+        const uint64_t a = t == 0 ? (((uint64_t)1 << 56) - 1) : rng() % H.primes[k + j];
+        const uint32_t cl = (uint32_t)(c & M28), ch = (uint32_t)(c >> 28), cs = cl + ch;
+        const uint32_t al = (uint32_t)a & M28, ah = (uint32_t)(a >> 28), as = al + ah;
+        s0 += (uint64_t)al * cl;
+        s2 += (uint64_t)ah * ch;
+        sm += (uint64_t)as * cs;
+      }
+      check(s0, sm - s0 - s2, s2);
+      // quotient steps, the low sums at the top of their ranges
+      const u128 tmax = ((u128)1) << (61 + d.nb);
+      const u128 m = ((u128)rng() << 64 | rng()) % (tmax / q);
+      for (int dd = -2; dd <= 2; ++dd) {
+        const u128 T = m * q + (u128)(__int128)dd + 2 * (u128)q;
+        const uint64_t x0 = ((uint64_t)1 << 61) + (uint64_t)(T & M28) - (((uint64_t)1 << 61) & ~(uint64_t)M28);
+        // T = x0 + x1 2^28 + x2 2^56 with x0 near 2^61 and x1 near 2^62
+        const u128 rest = (T - x0) >> 28;  // T - x0 is a multiple of 2^28
+        const uint64_t x1 = (uint64_t)(rest & ((((u128)1) << 28) - 1)) + (((uint64_t)1 << 62) & ~(uint64_t)M28);
+        const u128 rest2 = rest - x1;
+        if (T < x0 || rest < x1 || (rest2 & ((((u128)1) << 28) - 1)) != 0) {
+          check((uint64_t)(T & M28), (uint64_t)((T >> 28) & M28), (uint64_t)(T >> 56));
+          continue;
+        }
+        check(x0, x1, (uint64_t)(rest2 >> 28));
+      }
+    }
+  }
+  return bad;
+}
+
+// ModDown's finish with the lazy sums against the textbook formula, for
+// every Q prime below 2^57 and level l: ((x - T) B^-1 + add - top) q_l^-1
+// mod q_i, computed as (x + 5q - r) (B^-1 q_l^-1) + (q + add - top) q_l^-1 from
+// r = reduce_sums_lazy (the rescale path), and (x + 5q - r) B^-1 + add without
+// the rescale.  Returns the number of mismatches.
+int64_t chk_moddown_finish(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
+  HostParams H;
+  if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  std::mt19937_64 rng(seed);
+  int64_t bad = 0;
+  typedef unsigned __int128 u128;
+  for (int l = 1; l < k; ++l) {
+    for (int i = 0; i < l; ++i) {
+      const DevPrime& d = H.dprimes[i];
+      if (d.nb > 56) continue;
+      const uint64_t q = d.p, ql = H.primes[l];
+      const uint64_t ib = H.inv_b_mod_q[2 * i], ibq = H.inv_b_mod_q[2 * i + 1];
+      const uint64_t iql = H.inv_ql_mod_qi[((size_t)l * k + i) * 2], iqlq = H.inv_ql_mod_qi[((size_t)l * k + i) * 2 + 1];
+      const uint64_t f = H.md_fin[((size_t)l * k + i) * 2], fq = H.md_fin[((size_t)l * k + i) * 2 + 1];
+      for (int64_t t = 0; t < trials; ++t) {
+        const uint64_t x = t == 0 ? q - 1 : rng() % q, add = t == 1 ? q - 1 : rng() % q;
+        const uint64_t top = t == 2 ? ql - 1 : rng() % ql;
+        const uint64_t s0 = rng() >> 3, s1 = rng() >> 2, s2 = (rng() >> 2) & ((((uint64_t)1) << (d.nb + 5)) - 1);
+        const u128 T = (u128)s0 + ((u128)s1 << 28) + ((u128)s2 << 56);
+        const uint64_t r = reduce_sums_lazy(s0, s1, s2, q, H.md_red[2 * i], (int)d.nb);
+        const uint64_t y = x + H.md_red[2 * i + 1] - r;
+        const uint64_t tm = (uint64_t)(T % q);
+        const uint64_t v = (uint64_t)(((u128)((x + q - tm) % q) * ib) % q);
+        const uint64_t want_r = (uint64_t)(((u128)((v + add + q - top % q) % q) * iql) % q);
+        const uint64_t got_r = add_mod(mul_shoup(y, f, fq, q), mul_shoup(q + add - top, iql, iqlq, q), q);
+        const uint64_t want = (v + add) % q;
+        const uint64_t got = add_mod(mul_shoup(y, ib, ibq, q), add, q);
+        if (got_r != want_r || got != want) ++bad;
+      }
+    }
+  }
+  return bad;
+}
+
 // The product's host tables (same numbering as aoa_params_table), built
 // without a GPU.  Returns the number of words (out may be null).
 int64_t chk_table(int N, int k, int kl, int prec, int which, uint64_t* out) {

@@ -39,6 +39,10 @@ def load():
     lib.chk_moddown_screen_words.argtypes = [ctypes.c_int64, ctypes.c_uint64]
     lib.chk_moddown_accumulate.restype = ctypes.c_int64
     lib.chk_moddown_accumulate.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int64, ctypes.c_uint64]
+    lib.chk_moddown_reduce.restype = ctypes.c_int64
+    lib.chk_moddown_reduce.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int64, ctypes.c_uint64]
+    lib.chk_moddown_finish.restype = ctypes.c_int64
+    lib.chk_moddown_finish.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int64, ctypes.c_uint64]
     lib.chk_table.restype = ctypes.c_int64
     lib.chk_table.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(ctypes.c_uint64)]
     _lib = lib

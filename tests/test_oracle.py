@@ -277,6 +277,25 @@ def test_moddown_three_product_accumulation():
         assert chk.chk_moddown_accumulate(*shape, 300, 3) == 0, shape
 
 
+def test_moddown_lazy_reduction():
+    """The three sums of a limb reduced at once, up to four multiples of q too
+    many: congruent and below 5q at the kernel's sums, at the largest sums the
+    bound allows and at every quotient step with both floors of the quotient
+    estimate dropping the most."""
+    chk = native_chk.load()
+    for shape in [(65536, 30, 31, 48), (4096, 4, 5, 52), (16384, 5, 5, 55), (4096, 10, 11, 45),
+                  (65536, 30, 64, 48)]:
+        assert chk.chk_moddown_reduce(*shape, 3000, 17) == 0, shape
+
+
+def test_moddown_lazy_finish():
+    """ModDown's finish from the lazy sum, with and without the fused rescale,
+    against ((x - T) B^-1 + add - top) q_l^-1 mod q_i in 128-bit arithmetic."""
+    chk = native_chk.load()
+    for shape in [(65536, 30, 31, 48), (4096, 4, 5, 52), (16384, 5, 5, 55), (4096, 10, 11, 45)]:
+        assert chk.chk_moddown_finish(*shape, 200, 19) == 0, shape
+
+
 def test_product_exact_mults():
     chk = native_chk.load()
     for shape in [(65536, 30, 31, 48), (4096, 4, 5, 52), (4096, 5, 5, 55)]:
