@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../arithmetic/poly_t.h"
+#include "../tool/json.h"
 
 struct ciphertext {
   poly_t c[2];
@@ -21,8 +22,11 @@ void cipher_init(Context* ctx, cipher_t* c, int slots = 1);
 void cipher_clear(Context* ctx, cipher_t* ct);
 void cipher_free(Context* ctx, cipher_t* c);
 void cipher_copy(Context* ctx, cipher_t* b, cipher_t* a);
-// The reference's JSON layout (cipher_t.cpp:42-82) as a string:
+// The reference's document (cipher_t.cpp:42-82):
 // {"ct": {"0": "<poly_export>", "1": "..."}, "props": {"level", "scale", "slots"}}.
+json cipher_export(Context* ctx, cipher_t* ct);
+cipher_t* cipher_import(Context* ctx, const json& k);
+// The same as compact text (json::dump / json::parse).
 std::string cipher_export_json(Context* ctx, cipher_t* ct);
 cipher_t* cipher_import_json(Context* ctx, const std::string& doc);
 // Binary form (no counterpart in the reference): a 40-byte header (magic,

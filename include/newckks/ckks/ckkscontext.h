@@ -12,6 +12,7 @@
 
 #include <map>
 #include <string>
+#include <vector>
 
 // Names of the precomputed Maclaurin series (ckkscontext.h:22-27).
 static const std::string LN1MINUS = "Log1Minus";  // log(1-x)
@@ -38,6 +39,15 @@ class CKKSContext : public Context {
   // scale: in/out; -1 selects 2^scalingfactor.
   void encode(poly_t* a, int64_t* scale, std::complex<double>* val, int slots, int empty_slots = 0);
   void decode(std::complex<double>* val, poly_t* a, int64_t scale, int slots);
+
+  // {"sk": {"s"}, "pk": {"a", "b"}, "evk": {"a", "b"}}, each a poly_export
+  // string of the QB key (ckkscontext.h:141-158, ckkscontext.cpp:41-100).
+  json export_keys();
+  void load_keys(const json& k);
+  void clear_keys();
+  // Binary form of the same five keys (no counterpart in the reference).
+  std::vector<uint8_t> export_keys_binary();
+  void load_keys_binary(const uint8_t* data, size_t size);
 };
 
 #endif

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -214,6 +215,44 @@ static void run_params(int k, int kl, int N, int prec, bool heavy, bool exact = 
     for (cipher_t* x : {a, j, b}) {
       cipher_free(ctx, x);
       delete x;
+    }
+  }
+  // Keys through export_keys / load_keys (ckkscontext.cpp:50-100) and the
+  // binary form: a second context with the loaded keys encrypts, multiplies
+  // and decrypts within the bound
+  if (heavy) {
+    ctx->keys.sk = sk;
+    const std::string doc = ctx->export_keys().dump();
+    const std::vector<uint8_t> kb = ctx->export_keys_binary();
+    for (int form = 0; form < 2; ++form) {
+      CKKSContext* c2 = new CKKSContext();
+      if (form == 0)
+        c2->load_keys(json::parse(doc));
+      else
+        c2->load_keys_binary(kb.data(), kb.size());
+      bool same = poly_are_equal(c2, &c2->keys.evk->b, &ctx->keys.evk->b) &&
+                  poly_are_equal(c2, &c2->keys.sk->s, &sk->s) && poly_are_equal(c2, &c2->keys.pk->a, &ctx->keys.pk->a);
+      for (int i = 0; i < slots; ++i) {
+        v1[i] = cd(dsq(gen), dsq(gen));
+        v2[i] = cd(dsq(gen), dsq(gen));
+        exp[i] = v1[i] * v2[i];
+      }
+      cipher_t* a = ckks_encrypt(c2, v1.data(), slots);
+      cipher_t* b = ckks_encrypt(c2, v2.data(), slots);
+      cipher_t* m = ckks_mul(c2, a, b);
+      cd* out = ckks_decrypt(c2, m, c2->keys.sk);
+      report(std::string(form ? "KeysBinary_" : "KeysJson_") + tag, same ? maxerr(exp.data(), out, slots) : 1.0,
+             ERRBOUND);
+      free(out);
+      for (cipher_t* x : {a, b, m}) {
+        cipher_free(c2, x);
+        delete x;
+      }
+      SecretKey* s2 = c2->keys.sk;
+      ckkskeychain_free(c2, &c2->keys);
+      ckks_free_sk(c2, s2);
+      delete s2;
+      delete c2;
     }
   }
   // level 0 cannot be rescaled (ckks.cpp:224-227)
@@ -719,7 +758,75 @@ static void manager_statics() {
   CUDAManager::destroy();
 }
 
+static std::string read_file(const char* path) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+  std::string s;
+  char buf[1 << 16];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) s.append(buf, n);
+  std::fclose(f);
+  return s;
+}
+
+static void write_file(const std::string& path, const void* data, size_t n) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f || std::fwrite(data, 1, n, f) != n) throw std::runtime_error("cannot write " + path);
+  std::fclose(f);
+}
+
+static void write_poly(FILE* f, Context* ctx, poly_t* p) {
+  uint64_t* h = poly_copy_to_host(ctx, p);
+  std::fwrite(h, 1, poly_get_size(p->base), f);
+  free(h);
+}
+
+// Reads a ciphertext document and a key document in the reference's layouts
+// (N=2048, k=2, kl=1, prec 55), writes their residues (coefficient domain,
+// uint64) and properties as imported, their exports as compact text, and
+// checks the binary forms round-trip.  Returns 0 on success.
+static int serial_mode(const char* ct_path, const char* keys_path, const std::string& out) {
+  CUDAManager::init(2, 1, 2048, 55);
+  CKKSContext* ctx = new CKKSContext();
+  cipher_t* ct = cipher_import(ctx, json::parse(read_file(ct_path)));
+  FILE* f = std::fopen((out + "/ct.bin").c_str(), "wb");
+  write_poly(f, ctx, &ct->c[0]);
+  write_poly(f, ctx, &ct->c[1]);
+  std::fclose(f);
+  const std::string props = std::to_string(ct->level) + " " + std::to_string(ct->scale) + " " + std::to_string(ct->slots);
+  write_file(out + "/ct_props.txt", props.data(), props.size());
+  const std::string ce = cipher_export(ctx, ct).dump();
+  write_file(out + "/ct_export.json", ce.data(), ce.size());
+  ctx->load_keys(json::parse(read_file(keys_path)));
+  f = std::fopen((out + "/keys.bin").c_str(), "wb");
+  for (poly_t* p : {&ctx->keys.sk->s, &ctx->keys.pk->a, &ctx->keys.pk->b, &ctx->keys.evk->a, &ctx->keys.evk->b})
+    write_poly(f, ctx, p);
+  std::fclose(f);
+  const std::string ke = ctx->export_keys().dump();
+  write_file(out + "/keys_export.json", ke.data(), ke.size());
+  // binary forms
+  std::vector<uint8_t> cb = cipher_export_binary(ctx, ct);
+  cipher_t* c2 = cipher_import_binary(ctx, cb.data(), cb.size());
+  bool ok = poly_are_equal(ctx, &c2->c[0], &ct->c[0]) && poly_are_equal(ctx, &c2->c[1], &ct->c[1]) &&
+            c2->level == ct->level && c2->scale == ct->scale && c2->slots == ct->slots;
+  std::vector<uint8_t> kb = ctx->export_keys_binary();
+  CKKSContext* k2 = new CKKSContext();
+  k2->load_keys_binary(kb.data(), kb.size());
+  ok = ok && poly_are_equal(k2, &k2->keys.sk->s, &ctx->keys.sk->s) && poly_are_equal(k2, &k2->keys.evk->a, &ctx->keys.evk->a) &&
+       poly_are_equal(k2, &k2->keys.evk->b, &ctx->keys.evk->b) && poly_are_equal(k2, &k2->keys.pk->b, &ctx->keys.pk->b);
+  // a truncated or foreign document is refused
+  bool refused = false;
+  try {
+    cipher_import(ctx, json::parse("{\"ct\":{\"0\":\"1,2,\",\"1\":\"3,\"},\"props\":{\"level\":1,\"scale\":2,\"slots\":4}}"));
+  } catch (const std::runtime_error&) {
+    refused = true;
+  }
+  std::printf("%s SerialBinaryRoundTrip\n%s SerialShortDocumentRefused\n", ok ? "PASS" : "FAIL", refused ? "PASS" : "FAIL");
+  return ok && refused ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
+  if (argc == 5 && std::string(argv[1]) == "serial") return serial_mode(argv[2], argv[3], argv[4]);
   const bool quick = argc > 1 && std::string(argv[1]) == "quick";
   manager_statics();
   mul_n4096_k4();

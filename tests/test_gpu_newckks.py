@@ -1,7 +1,9 @@
 """The newckks-compatible C++ API (libnewckks.so over the C-ABI) on the GPU:
 encode/encrypt/add/mul/square/rescale/rotate/sumslots/decrypt against FP64
 within the reference's error bounds at N=4096, prec 52, 2048 slots
-at scale 2^52 and 2^40, and the level-0 rescale error path."""
+at scale 2^52 and 2^40, the level-0 rescale error path, and keys exported and
+loaded into a second context (text and binary) that then multiplies within
+the bound."""
 import os
 import subprocess
 
@@ -22,6 +24,9 @@ def test_newckks_api(gpu):
     assert lines, "no test output"
     assert any("MulN4096K4_scale2^52" in ln for ln in lines)
     assert any("Mul_N65536_k30_kl0_prec48_exactmoddown" in ln for ln in lines)
+    # keys through export_keys / load_keys and the binary key form
+    assert any(ln.startswith("PASS KeysJson_") for ln in lines)
+    assert any(ln.startswith("PASS KeysBinary_") for ln in lines)
     failed = [ln for ln in lines if ln.startswith("FAIL")]
     assert not failed, "\n".join(failed)
     assert r.returncode == 0
