@@ -118,6 +118,23 @@ SIGNATURES.update({
                                              ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
 })
 
+class BatchOptions(ctypes.Structure):
+    _fields_ = [("ntt_split", ctypes.c_int), ("prime_major", ctypes.c_int), ("xcd_chunks", ctypes.c_int),
+                ("strict_split", ctypes.c_int)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("opt", BatchOptions), ("ms", ctypes.c_double), ("checksum", ctypes.c_uint64),
+                ("exact", ctypes.c_int)]
+
+
+SIGNATURES.update({
+    "aoa_batch_default_options": (None, [ctypes.POINTER(BatchOptions)]),
+    "aoa_batch_set_options": (ctypes.c_int, [c_vp, ctypes.POINTER(BatchOptions)]),
+    "aoa_batch_get_options": (ctypes.c_int, [c_vp, ctypes.POINTER(BatchOptions)]),
+    "aoa_batch_autotune": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int, ctypes.POINTER(TuneResult), ctypes.c_int]),
+})
+
 _lib = None
 
 

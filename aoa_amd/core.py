@@ -7,7 +7,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import MulItem, Op, RotItem, check, load
+from ._lib import BatchOptions, MulItem, Op, RotItem, TuneResult, check, load
 
 FORWARD, INVERSE = 0, 1
 ADD, SUB, MUL, NEGATE, MULADD = 0, 1, 2, 3, 5
@@ -261,6 +261,36 @@ class Batch:
 
     def launch(self, stream=None, graph=True):
         check(self.lib.aoa_batch_launch(self.h, _stream(stream), 1 if graph else 0))
+
+    OPTION_FIELDS = ("ntt_split", "prime_major", "xcd_chunks", "strict_split")
+
+    def options(self):
+        """The batch's launch options as a dict."""
+        o = BatchOptions()
+        check(self.lib.aoa_batch_get_options(self.h, ctypes.byref(o)))
+        return {f: getattr(o, f) for f in self.OPTION_FIELDS}
+
+    def set_options(self, **kw):
+        """Sets launch options (missing ones keep their current values)."""
+        cur = self.options()
+        cur.update(kw)
+        o = BatchOptions(*[int(cur[f]) for f in self.OPTION_FIELDS])
+        check(self.lib.aoa_batch_set_options(self.h, ctypes.byref(o)))
+
+    def autotune(self, stream=None, reps=5, max_results=64):
+        """Times every launch-option candidate on this batch and keeps the
+        fastest exact one; returns the candidates as dicts (options, ms,
+        checksum, exact)."""
+        res = (TuneResult * max_results)()
+        n = self.lib.aoa_batch_autotune(self.h, _stream(stream), reps, res, max_results)
+        if n < 0:
+            check(-n)
+        out = []
+        for r in res[:min(n, max_results)]:
+            d = {f: getattr(r.opt, f) for f in self.OPTION_FIELDS}
+            d.update(ms=r.ms, checksum=int(r.checksum), exact=bool(r.exact))
+            out.append(d)
+        return out
 
     STAGES = ("pre_fwd", "pre_inv", "tensor", "automorphism", "intt", "modup", "ntt", "key_intt", "moddown")
 

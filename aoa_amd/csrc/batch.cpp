@@ -68,6 +68,7 @@ size_t rot_ws_limbs(int L, int K) { return 5 * (size_t)L + 3 * (size_t)K; }
 
 struct aoa_batch {
   std::vector<BatchLayer> layers;
+  aoa_batch_options opt{0, 1, 0, 0};
   bool built = false;
   void* dmem = nullptr;  // descriptors
   void* ws = nullptr;    // workspace
@@ -278,8 +279,11 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
       // to each other, so their workgroups (spread over the XCDs by block
       // index) find that prime's twiddles, and in the key stage its key
       // limbs, in the L2 cache.  Entries of a stage are independent.
-      auto by_prime = [](auto& v) {
-        std::stable_sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.prime < b.prime; });
+      // Item order (prime_major = 0) keeps the order of insertion: the limbs of
+      // one item after the other.
+      const bool pm = b->opt.prime_major != 0;
+      auto by_prime = [pm](auto& v) {
+        if (pm) std::stable_sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.prime < b.prime; });
       };
       by_prime(prefwd);
       by_prime(preinv);
@@ -302,11 +306,17 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
           if (pr[v[t].prime] >= (1ull << 61)) return -1;
         return n;
       };
-      // Small stages (a serving step of a few requests) keep one launch: the
-      // split pays only where a launch fills the GPU.
+      // Small stages (a serving step of a few requests) keep one launch by
+      // default: the split pays only where a launch fills the GPU.  The split
+      // needs the q0 limbs first, i.e. prime-major order.
       constexpr size_t kSplitMin = 256;
-      g.ns_key2 = key2.size() >= kSplitMin ? strict_prefix(key2) : -1;
-      g.ns_trow = trow.size() >= kSplitMin ? strict_prefix(trow) : -1;
+      auto split_at = [&](const auto& v) -> int {
+        if (b->opt.strict_split < 0 || !pm) return -1;
+        if (b->opt.strict_split == 0 && v.size() < kSplitMin) return -1;
+        return strict_prefix(v);
+      };
+      g.ns_key2 = split_at(key2);
+      g.ns_trow = split_at(trow);
       g.n_trow = (int)trow.size();
       g.n_tcol = (int)tcol.size();
       g.d_trow = (TensorRowDesc*)(trow.empty() ? nullptr : put(trow.data(), trow.size() * sizeof(TensorRowDesc)));
@@ -362,6 +372,15 @@ struct StageTimer {
 // Issues every stage launch of the batch on stream st (stage order
 // AOA_STAGE_*); with a timer, each (layer, stage) launch run is bracketed by
 // an event pair.
+// The parameters a group's launches use: its handle's, with the batch's NTT
+// split (when the group's ring admits it) and XCD mapping.
+static DevParams launch_params(const aoa_batch* b, const BatchGroup& g) {
+  DevParams P = g.params->dev;
+  if (b->opt.ntt_split && ntt_valid_split(P.N, b->opt.ntt_split)) P.ntt_n1 = b->opt.ntt_split;
+  P.xcd_chunks = b->opt.xcd_chunks ? 1 : 0;
+  return P;
+}
+
 int batch_enqueue(aoa_batch* b, hipStream_t st, StageTimer* tm) {
   int rc = AOA_OK;
   auto run = [&](int s, auto&& has, auto&& go) {
@@ -376,34 +395,34 @@ int batch_enqueue(aoa_batch* b, hipStream_t st, StageTimer* tm) {
   for (auto& L : b->layers) {
     b->layers_cur = &L;
     run(AOA_STAGE_PRE_FWD, [](BatchGroup& g) { return g.n_pre_fwd > 0; },
-        [&](BatchGroup& g) { return ntt_forward_desc(g.params->dev, g.d_pre_fwd, g.n_pre_fwd, 0, st); });
+        [&](BatchGroup& g) { return ntt_forward_desc(launch_params(b, g), g.d_pre_fwd, g.n_pre_fwd, 0, st); });
     run(AOA_STAGE_PRE_INV, [](BatchGroup& g) { return g.n_pre_inv > 0; },
-        [&](BatchGroup& g) { return ntt_inverse_desc(g.params->dev, g.d_pre_inv, g.n_pre_inv, 0, st); });
+        [&](BatchGroup& g) { return ntt_inverse_desc(launch_params(b, g), g.d_pre_inv, g.n_pre_inv, 0, st); });
     run(AOA_STAGE_TENSOR, [](BatchGroup& g) { return g.n_tensor > 0; }, [&](BatchGroup& g) {
-      launch_tensor_desc(g.params->dev, g.d_tensor, g.n_tensor, st);
+      launch_tensor_desc(launch_params(b, g), g.d_tensor, g.n_tensor, st);
       return 0;
     });
     run(AOA_STAGE_AUTOMORPHISM, [](BatchGroup& g) { return g.n_auto > 0; }, [&](BatchGroup& g) {
-      launch_automorphism_desc(g.params->dev, g.d_auto, g.n_auto, st);
+      launch_automorphism_desc(launch_params(b, g), g.d_auto, g.n_auto, st);
       return 0;
     });
     run(AOA_STAGE_INTT, [](BatchGroup& g) { return g.n_intt3 > 0 || g.n_trow > 0; }, [&](BatchGroup& g) {
       if (g.n_trow > 0 &&
-          ntt_inverse_tensor_desc(g.params->dev, g.d_trow, g.n_trow, g.d_tcol, g.n_tcol, 0, st, g.ns_trow))
+          ntt_inverse_tensor_desc(launch_params(b, g), g.d_trow, g.n_trow, g.d_tcol, g.n_tcol, 0, st, g.ns_trow))
         return -1;
-      return g.n_intt3 > 0 ? ntt_inverse_desc(g.params->dev, g.d_intt3, g.n_intt3, 0, st) : 0;
+      return g.n_intt3 > 0 ? ntt_inverse_desc(launch_params(b, g), g.d_intt3, g.n_intt3, 0, st) : 0;
     });
     run(AOA_STAGE_MODUP, [](BatchGroup& g) { return g.n_modup > 0; }, [&](BatchGroup& g) {
-      launch_modup_desc(g.params->dev, g.d_modup, g.n_modup, g.max_level, st);
+      launch_modup_desc(launch_params(b, g), g.d_modup, g.n_modup, g.max_level, st);
       return 0;
     });
     run(AOA_STAGE_NTT, [](BatchGroup& g) { return g.n_fwd > 0; },
-        [&](BatchGroup& g) { return ntt_forward_desc(g.params->dev, g.d_fwd, g.n_fwd, 0, st); });
+        [&](BatchGroup& g) { return ntt_forward_desc(launch_params(b, g), g.d_fwd, g.n_fwd, 0, st); });
     run(AOA_STAGE_KEY_INTT, [](BatchGroup& g) { return g.n_key2 > 0; }, [&](BatchGroup& g) {
-      return ntt_inverse_key2_desc(g.params->dev, g.d_key2, g.n_key2, g.d_keycol, g.n_keycol, 0, st, g.ns_key2);
+      return ntt_inverse_key2_desc(launch_params(b, g), g.d_key2, g.n_key2, g.d_keycol, g.n_keycol, 0, st, g.ns_key2);
     });
     run(AOA_STAGE_MODDOWN, [](BatchGroup& g) { return g.n_moddown > 0; }, [&](BatchGroup& g) {
-      launch_moddown_desc(g.params->dev, g.d_moddown, g.n_moddown, st);
+      launch_moddown_desc(launch_params(b, g), g.d_moddown, g.n_moddown, st);
       return 0;
     });
   }
@@ -535,6 +554,47 @@ extern "C" int aoa_batch_build(aoa_batch* b) {
 
 extern "C" size_t aoa_batch_workspace_bytes(const aoa_batch* b) { return b ? b->ws_bytes : 0; }
 
+extern "C" void aoa_batch_default_options(aoa_batch_options* o) {
+  if (o) *o = aoa_batch_options{0, 1, 0, 0};
+}
+
+extern "C" int aoa_batch_get_options(const aoa_batch* b, aoa_batch_options* o) {
+  if (!b || !o) return set_error(AOA_ERR_ARG, "null argument");
+  *o = b->opt;
+  return AOA_OK;
+}
+
+static void drop_graph(aoa_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+extern "C" int aoa_batch_set_options(aoa_batch* b, const aoa_batch_options* o) {
+  if (!b || !o) return set_error(AOA_ERR_ARG, "null argument");
+  if (o->strict_split < -1 || o->strict_split > 1) return set_error(AOA_ERR_ARG, "strict_split must be -1, 0 or 1");
+  if (o->ntt_split) {
+    bool any = false;
+    for (auto& L : b->layers)
+      for (auto& g : L.groups) any = any || ntt_valid_split(g.params->host.N, o->ntt_split);
+    if (!any) return set_error(AOA_ERR_ARG, "the NTT split fits no ring dimension of the batch");
+  }
+  b->opt = *o;
+  b->opt.prime_major = o->prime_major ? 1 : 0;
+  b->opt.xcd_chunks = o->xcd_chunks ? 1 : 0;
+  if (!b->built) return AOA_OK;
+  // same descriptors in another order: refill the tables in place
+  if (hipDeviceSynchronize() != hipSuccess) return set_error(AOA_ERR_HIP, "hipDeviceSynchronize failed");
+  drop_graph(b);
+  std::vector<char> host(b->desc_bytes);
+  size_t used = batch_fill(b, (uint64_t*)b->ws, host.data(), (char*)b->dmem, b->desc_bytes);
+  if (b->desc_bytes && used != b->desc_bytes) return set_error(AOA_ERR_ARG, "descriptor refill failed");
+  if (used && hipMemcpy(b->dmem, host.data(), used, hipMemcpyHostToDevice) != hipSuccess)
+    return set_error(AOA_ERR_HIP, "hipMemcpy of descriptors failed");
+  return AOA_OK;
+}
+
 extern "C" int aoa_batch_launch(aoa_batch* b, void* stream, int use_graph) {
   if (!b) return set_error(AOA_ERR_ARG, "null batch");
   int rc = aoa_batch_build(b);
@@ -584,4 +644,120 @@ extern "C" int aoa_batch_stage_times(aoa_batch* b, void* stream, double* ms, int
     launches[tm.stage[i]] += 1;
   }
   return AOA_OK;
+}
+
+// ------------------------------------------------------------ autotuner ----
+// The reference fixes its NTT factorisation per ring dimension in a table
+// (aoa-ntt/include/newckks/cuda/htrans/ntt.h:13-44).  Here every launch option
+// of a batch is timed on the batch itself and the fastest is kept.
+
+// Words of every output of the batch: HomMul outputs after the rescale (level
+// limbs), rotation outputs (level + 1 limbs), transformed polynomials.
+static void output_ranges(const aoa_batch* b, std::vector<std::pair<const uint64_t*, size_t>>* out) {
+  for (auto& L : b->layers)
+    for (auto& g : L.groups) {
+      const size_t N = g.params->host.N;
+      for (auto& it : g.muls)
+        for (int y = 0; y < 2; ++y) out->push_back({it.out[y], (size_t)it.level * N});
+      for (auto& it : g.rots)
+        for (int y = 0; y < 2; ++y) out->push_back({it.out[y], (size_t)(it.level + 1) * N});
+      for (auto& it : g.ntts)
+        for (int y = 0; y < 2; ++y) out->push_back({it.poly[y], (size_t)(it.level + 1) * N});
+    }
+}
+
+extern "C" int aoa_batch_autotune(aoa_batch* b, void* stream, int reps, aoa_tune_result* results, int max) {
+  if (!b) return -set_error(AOA_ERR_ARG, "null batch");
+  if (reps <= 0) reps = 5;
+  int rc = aoa_batch_build(b);
+  if (rc) return -rc;
+  hipStream_t st = (hipStream_t)stream;
+  // NTT splits valid for every ring of the batch (0: each handle's own)
+  std::vector<int> splits;
+  for (int n1 : {64, 128, 256, 2048, 4096}) {
+    bool ok = true, any = false;
+    for (auto& L : b->layers)
+      for (auto& g : L.groups) {
+        ok = ok && ntt_valid_split(g.params->host.N, n1);
+        any = true;
+      }
+    if (ok && any) splits.push_back(n1);
+  }
+  if (splits.size() < 2) splits.assign(1, 0);
+  std::vector<aoa_batch_options> cand;
+  for (int n1 : splits)
+    for (int xcd = 0; xcd < 2; ++xcd) {
+      cand.push_back({n1, 0, xcd, -1});  // item order: no strict split
+      for (int ss = -1; ss <= 1; ++ss) cand.push_back({n1, 1, xcd, ss});
+    }
+  std::vector<std::pair<const uint64_t*, size_t>> outs;
+  output_ranges(b, &outs);
+  uint64_t* acc = nullptr;
+  if (hipMalloc(&acc, sizeof(uint64_t)) != hipSuccess) return -set_error(AOA_ERR_HIP, "hipMalloc failed");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const aoa_batch_options saved = b->opt;
+  auto checksum = [&](uint64_t* v) -> int {
+    if (hipMemsetAsync(acc, 0, sizeof(uint64_t), st) != hipSuccess) return set_error(AOA_ERR_HIP, "hipMemsetAsync failed");
+    uint64_t off = 0;
+    for (auto& r : outs) {
+      launch_checksum(r.first, r.second, off, acc, st);
+      off += r.second;
+    }
+    if (hipMemcpyAsync(v, acc, sizeof(uint64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return set_error(AOA_ERR_HIP, "checksum copy failed");
+    return check_launch("aoa_batch_autotune");
+  };
+  int best = -1, count = 0;
+  double best_ms = 1e30;
+  uint64_t base_sum = 0;
+  bool reproducible = true;
+  for (size_t c = 0; c < cand.size() && !rc; ++c) {
+    rc = aoa_batch_set_options(b, &cand[c]);
+    if (rc) break;
+    rc = aoa_batch_launch(b, st, 1);  // warm-up (captures the graph)
+    if (rc) break;
+    if (hipStreamSynchronize(st) != hipSuccess) {
+      rc = set_error(AOA_ERR_HIP, "hipStreamSynchronize failed");
+      break;
+    }
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps && !rc; ++r) rc = aoa_batch_launch(b, st, 1);
+    (void)hipEventRecord(e1, st);
+    if (rc || hipEventSynchronize(e1) != hipSuccess) {
+      if (!rc) rc = set_error(AOA_ERR_HIP, "hipEventSynchronize failed");
+      break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    uint64_t sum = 0;
+    if ((rc = checksum(&sum))) break;
+    if (c == 0) {
+      // outputs that are a function of unchanged inputs repeat exactly
+      base_sum = sum;
+      if ((rc = aoa_batch_launch(b, st, 1))) break;
+      uint64_t again = 0;
+      if ((rc = checksum(&again))) break;
+      reproducible = again == base_sum;
+    }
+    const bool exact = reproducible && sum == base_sum;
+    const double t = ms / reps;
+    if (count < max && results) results[count] = aoa_tune_result{cand[c], t, sum, exact ? 1 : 0};
+    ++count;
+    if ((exact || !reproducible) && t < best_ms) {
+      best_ms = t;
+      best = (int)c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(acc);
+  if (rc) {
+    (void)aoa_batch_set_options(b, &saved);
+    return -rc;
+  }
+  rc = aoa_batch_set_options(b, best >= 0 ? &cand[best] : &saved);
+  return rc ? -rc : count;
 }

@@ -79,6 +79,8 @@ void launch_moddown(const DevParams& P, const ModDownArgs& a, hipStream_t st);
 
 // Rescale c0 and c1 in place (coefficient domain) from `level`.
 void launch_rescale(const DevParams& P, uint64_t* c0, uint64_t* c1, int level, hipStream_t st);
+// acc += sum_i x[i] (2 (i + off) + 1) mod 2^64 over n words (acc on the device).
+void launch_checksum(const uint64_t* x, size_t n, uint64_t off, uint64_t* acc, hipStream_t st);
 
 // Galois automorphism X -> X^g (g odd) in the coefficient domain, gather form.
 void launch_automorphism(const DevParams& P, uint64_t* out, const uint64_t* in, int nlimbs,

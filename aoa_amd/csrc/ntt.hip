@@ -516,37 +516,54 @@ __device__ __forceinline__ bool use_lazy(const DevParams& P, int pr) {
   return MODE == 1 ? true : (MODE == 2 ? false : lazy_prime(P, pr));
 }
 
+// Work item of this workgroup.  Workgroups are dispatched round-robin over
+// the 8 XCDs (workgroup w runs on XCD w % 8).  With P.xcd_chunks the n work
+// items are dealt out in 8 contiguous chunks, one per XCD (the first n % 8
+// chunks one item longer), so that neighbouring items -- in a prime-major
+// descriptor list, the tiles of one prime -- share an XCD and its L2.  The
+// map is a bijection of [0, n): XCD x receives exactly the ids w < n with
+// w % 8 == x, as many as its chunk holds.
+__device__ __forceinline__ int work_id(const DevParams& P) {
+  const unsigned w = blockIdx.x;
+  if (!P.xcd_chunks) return (int)w;
+  const unsigned n = gridDim.x, q = n >> 3, r = n & 7, x = w & 7;
+  return (int)(x * q + (x < r ? x : r) + (w >> 3));
+}
+
 template <int S, int MODE>
 __global__ __launch_bounds__(256) void k_ntt_fwd_col_d(DevParams P, const NttDesc* d, int N2) {
   __shared__ uint64_t lds[Geo<S>::COL_LDS];
+  const int wid = work_id(P);
   const int tiles = N2 / Geo<S>::C;
-  const NttDesc e = d[blockIdx.x / tiles];
+  const NttDesc e = d[wid / tiles];
   if (use_lazy<MODE>(P, e.prime))
-    fwd_col_body<S, true>(P, e.src, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
+    fwd_col_body<S, true>(P, e.src, e.dst, e.prime, wid % tiles, N2, lds);
   else if (MODE != 1)
-    fwd_col_body<S, false>(P, e.src, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
+    fwd_col_body<S, false>(P, e.src, e.dst, e.prime, wid % tiles, N2, lds);
 }
 
 template <int S, int MODE>
 __global__ __launch_bounds__(256) void k_ntt_fwd_row_d(DevParams P, const NttDesc* d, int g0) {
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
+  const int wid = work_id(P);
   const int tiles = (P.N / S) / Geo<S>::C;
-  const NttDesc e = d[blockIdx.x / tiles];
+  const NttDesc e = d[wid / tiles];
   if (use_lazy<MODE>(P, e.prime))
-    fwd_row_body<S, true>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, g0, lds);
+    fwd_row_body<S, true>(P, e.dst, e.dst, e.prime, wid % tiles, g0, lds);
   else if (MODE != 1)
-    fwd_row_body<S, false>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, g0, lds);
+    fwd_row_body<S, false>(P, e.dst, e.dst, e.prime, wid % tiles, g0, lds);
 }
 
 template <int S, int MODE>
 __global__ __launch_bounds__(256) void k_ntt_inv_row_d(DevParams P, const NttDesc* d, int g0) {
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
+  const int wid = work_id(P);
   const int tiles = (P.N / S) / Geo<S>::C;
-  const NttDesc e = d[blockIdx.x / tiles];
+  const NttDesc e = d[wid / tiles];
   if (use_lazy<MODE>(P, e.prime))
-    inv_row_body<S, true>(P, e.src, e.dst, e.key, nullptr, e.prime, blockIdx.x % tiles, g0, lds);
+    inv_row_body<S, true>(P, e.src, e.dst, e.key, nullptr, e.prime, wid % tiles, g0, lds);
   else if (MODE != 1)
-    inv_row_body<S, false>(P, e.src, e.dst, e.key, nullptr, e.prime, blockIdx.x % tiles, g0, lds);
+    inv_row_body<S, false>(P, e.src, e.dst, e.key, nullptr, e.prime, wid % tiles, g0, lds);
 }
 
 // One workgroup per tile and key half: workgroups w and w + 8 take the two
@@ -575,26 +592,28 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const K
 template <int S, int MODE>
 __global__ __launch_bounds__(256) void k_ntt_inv_row_tensor_d(DevParams P, const TensorRowDesc* d, int g0) {
   __shared__ uint64_t lds[Geo<S>::ROW_LDS];
+  const int wid = work_id(P);
   const int tiles = (P.N / S) / Geo<S>::C;
-  const TensorRowDesc& e = d[blockIdx.x / tiles];
+  const TensorRowDesc& e = d[wid / tiles];
   const uint64_t *a0 = e.a0, *a1 = e.a1, *b0 = e.b0, *b1 = e.b1;
   uint64_t *xd = e.x, *o0 = e.dst[0], *o1 = e.dst[1], *o2 = e.dst[2];
   const int pr = e.prime;
   if (use_lazy<MODE>(P, pr))
-    inv_row_tensor_body<S, true>(P, a0, a1, b0, b1, xd, o0, o1, o2, pr, blockIdx.x % tiles, g0, lds);
+    inv_row_tensor_body<S, true>(P, a0, a1, b0, b1, xd, o0, o1, o2, pr, wid % tiles, g0, lds);
   else if (MODE != 1)
-    inv_row_tensor_body<S, false>(P, a0, a1, b0, b1, xd, o0, o1, o2, pr, blockIdx.x % tiles, g0, lds);
+    inv_row_tensor_body<S, false>(P, a0, a1, b0, b1, xd, o0, o1, o2, pr, wid % tiles, g0, lds);
 }
 
 template <int S, int MODE>
 __global__ __launch_bounds__(256) void k_ntt_inv_col_d(DevParams P, const NttDesc* d, int N2) {
   __shared__ uint64_t lds[Geo<S>::COL_LDS];
+  const int wid = work_id(P);
   const int tiles = N2 / Geo<S>::C;
-  const NttDesc e = d[blockIdx.x / tiles];
+  const NttDesc e = d[wid / tiles];
   if (use_lazy<MODE>(P, e.prime))
-    inv_col_body<S, true>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
+    inv_col_body<S, true>(P, e.dst, e.dst, e.prime, wid % tiles, N2, lds);
   else if (MODE != 1)
-    inv_col_body<S, false>(P, e.dst, e.dst, e.prime, blockIdx.x % tiles, N2, lds);
+    inv_col_body<S, false>(P, e.dst, e.dst, e.prime, wid % tiles, N2, lds);
 }
 
 // ------------------------------------------- single-workgroup transform --

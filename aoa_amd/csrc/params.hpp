@@ -58,6 +58,9 @@ struct DevParams {
   // 0: ModDown reproduces the reference's Barrett reduction of its
   // cross-base products (bit-exact residues); 1: exact reduction.
   int moddown_exact;
+  // 1: descriptor-driven NTT passes deal their work items out to the 8 XCDs
+  // in contiguous chunks (work_id); 0: workgroup id order
+  int xcd_chunks;
   const DevPrime* primes;        // [k+K]: Q primes then B primes
   const uint64_t* tw_fwd;        // [k+K][N][2]: psi^brev(i) and Shoup quotient
   const uint64_t* tw_inv;        // [k+K][N][2]: psi^-brev(i) and Shoup quotient

@@ -5,6 +5,8 @@
 // aoa-ntt/src/cuda/htrans/ntt.cu:479-667, aoa-ntt/src/ckks/encoder.cu:300-322);
 // every product is exact except the one ModDown cross-base product, which uses
 // the reference's Barrett formula so that residues stay bit-identical.
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -561,6 +563,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, hipStream_t st) {
   const size_t lds = (size_t)(P.K + BC_GROUPS) * BC_COEF * sizeof(uint64_t);
   k_moddown_d<<<dim3((unsigned)(ndesc * (P.N / BC_COEF))), 256, lds, st>>>(P, d);
+}
+
+// -------------------------------------------------------------- checksum ----
+// acc += sum_i x[i] (2 (i + off) + 1) mod 2^64 over n words: a position-
+// weighted sum that changes when any word changes or words move.  One atomic
+// add per wave.
+__global__ __launch_bounds__(256) void k_checksum(const uint64_t* x, uint64_t n, uint64_t off, uint64_t* acc) {
+  uint64_t s = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) s += x[i] * (2 * (i + off) + 1);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)acc, (unsigned long long)s);
+}
+
+void launch_checksum(const uint64_t* x, size_t n, uint64_t off, uint64_t* acc, hipStream_t st) {
+  if (!n) return;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+  k_checksum<<<dim3((unsigned)blocks), 256, 0, st>>>(x, (uint64_t)n, off, acc);
 }
 
 // --------------------------------------------------------------- rescale ----

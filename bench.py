@@ -120,6 +120,24 @@ def plain_fn(P, evk, items, ws_cache):
     return run
 
 
+def batch_options(bt, key, args, stream):
+    """Launch options of a batch: tuned now (--tune, recorded in the table),
+    else the autotuner table's entry for the workload, else the defaults.
+    Returns (options, where they came from)."""
+    import torch
+    from aoa_amd import tuning
+    if args.tune:
+        props = torch.cuda.get_device_properties(0)
+        dev = "%s (%s)" % (props.name, getattr(props, "gcnArchName", ""))
+        _, cands = tuning.tune(bt, key, stream=stream, reps=5, device=dev)
+        src = "autotuned in this run (%d candidates)" % len(cands)
+    elif tuning.apply(bt, key) is not None:
+        src = "autotuner table (aoa_amd/tuned/batch_options.json)"
+    else:
+        src = "defaults (no table entry)"
+    return bt.options(), src
+
+
 def stage_split(bt, stream, reps):
     """Per-stage milliseconds per batch launch and launches per batch, from
     `reps` eager runs timed with HIP events around each stage launch."""
@@ -136,7 +154,7 @@ def mixed_n15(args, rng, stream):
     """256 ciphertext pairs at N=2^15, k=20 (prec 48), K=21, levels uniform in
     1..19 (2..20 active limbs): plain calls vs one batched graph replay."""
     import torch
-    from aoa_amd import Batch, Params
+    from aoa_amd import Batch, Params, tuning
     P3 = Params(32768, 20, 0, PREC)
     evk3 = rand_polys(P3, 2, P3.k + P3.kl)
     lv3 = [int(x) for x in rng.integers(1, 20, size=256)]
@@ -146,6 +164,7 @@ def mixed_n15(args, rng, stream):
     b3 = Batch()
     b3.add_mul(P3, evk3, it3)
     b3.build()
+    opts, src = batch_options(b3, tuning.key(32768, P3.k + P3.kl, 256, "mul"), args, stream)
     b3_ms, _ = time_steps(lambda: b3.launch(graph=True), 5, 2, stream)
     mean_bytes = float(np.mean([op_bytes(32768, lv + 1, P3.kl) for lv in lv3]))
     del b3, it3
@@ -153,7 +172,8 @@ def mixed_n15(args, rng, stream):
     return {"workload": "256 HomMul+relin+rescale, N=2^15, k=20, K=21, levels uniform in 1..19",
             "plain_ops_s": round(256 * 1000.0 / p3_ms, 2), "batched_ops_s": round(256 * 1000.0 / b3_ms, 2),
             "batched_over_plain": round(p3_ms / b3_ms, 3),
-            "batched_hbm_frac": round(mean_bytes * 256 / (b3_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+            "batched_hbm_frac": round(mean_bytes * 256 / (b3_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "batch_options": opts, "batch_options_source": src}
 
 
 def sumslots_n16(args, stream):
@@ -162,7 +182,7 @@ def sumslots_n16(args, stream):
     (prec 48), K=25, top level: 15 layers of 64 fused rotate+add items in one
     graph, against plain aoa_rotate_left + add calls on 4 ciphertexts."""
     import torch
-    from aoa_amd import ADD, Batch, Params
+    from aoa_amd import ADD, Batch, Params, tuning
     P4 = Params(65536, 24, 0, PREC)
     lv = P4.k - 1
     L = lv + 1
@@ -181,6 +201,10 @@ def sumslots_n16(args, stream):
         bt.add_rotate(P4, items)
         bt.barrier()
     bt.build()
+    # rotate-and-sum accumulates in place: its outputs change from replay to
+    # replay, so the tuner times the candidates without comparing outputs
+    # (the options' exactness is covered by the reproducible batches)
+    opts, src = batch_options(bt, tuning.key(65536, P4.k + P4.kl, nct, "rotsum15"), args, stream)
     b_ms, _ = time_steps(lambda: bt.launch(graph=True), 3, 1, stream)
     del bt
     torch.cuda.empty_cache()
@@ -206,6 +230,7 @@ def sumslots_n16(args, stream):
          "plain_rotations_s": round(nrot_p * 1000.0 / p_ms, 1),
          "batched_over_plain": round((nrot_b / b_ms) / (nrot_p / p_ms), 3),
          "batch_ms": round(b_ms, 3),
+         "batch_options": opts, "batch_options_source": src,
          "batched_hbm_frac": round(rot_bytes(65536, L, P4.kl) * nrot_b / (b_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
     del cts, acc, rk, rtk, tmp, ws
     torch.cuda.empty_cache()
@@ -403,6 +428,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
     ap.add_argument("--requests", type=int, default=600, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=300.0, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the batch autotuner on the batched workloads first and record the chosen "
+                         "options in aoa_amd/tuned/batch_options.json")
     ap.add_argument("--profile", action="store_true",
                     help="headline shape only (plus a 1 GiB copy for counter calibration), for rocprofv3")
     ap.add_argument("--profile-c1", action="store_true",
@@ -414,7 +442,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from aoa_amd import Batch, Params
+    from aoa_amd import Batch, Params, tuning
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     stream = torch.cuda.current_stream()
@@ -442,6 +470,7 @@ def main():
     bt = Batch()
     bt.add_mul(P, evk, items)
     bt.build()
+    opts, opts_src = batch_options(bt, tuning.key(N, K_Q + K_B, B, "mul"), args, stream)
     batch_ms, wall = time_steps(lambda: bt.launch(graph=True), args.steps, args.warmup, stream)
     batch_ops = B * 1000.0 / batch_ms
     blat = latencies(lambda: bt.launch(graph=True), min(20, args.steps), stream)
@@ -470,7 +499,8 @@ def main():
         "data": "synthetic: uniform random residues (ciphertexts and evaluation key, N=2^16, k=30, K=31), on device",
         "config": {"workload": "batched HomMul+relin+rescale, %d ciphertext pairs per step, N=2^16, k=30 "
                                "(prec 48), K=31, level 29" % B,
-                   "N": N, "k": K_Q, "K": K_B, "prec": PREC, "level": level, "batch": B},
+                   "N": N, "k": K_Q, "K": K_B, "prec": PREC, "level": level, "batch": B,
+                   "batch_options": opts, "batch_options_source": opts_src},
         "plain_ops_s": round(plain_ops, 2),
         "batched_over_plain": round(batch_ops / plain_ops, 3),
         "p50_ms": round(float(np.percentile(lat, 50)), 4),

@@ -360,6 +360,41 @@ size_t aoa_batch_workspace_bytes(const aoa_batch* batch);
 int aoa_batch_launch(aoa_batch* batch, void* stream, int use_graph);
 void aoa_batch_destroy(aoa_batch* batch);
 
+/* Launch options of a batch (no counterpart in the reference, whose NTT
+ * factorisation is a fixed table, aoa-ntt/include/newckks/cuda/htrans/ntt.h:13-44).
+ * Every choice gives the same residues; aoa_batch_autotune times them. */
+typedef struct {
+  int ntt_split;    /* N1 of the two-pass NTT for the groups whose ring admits it (0: the handle's split) */
+  int prime_major;  /* 1: the descriptors of a stage sorted by prime (default); 0: item after item */
+  int xcd_chunks;   /* 1: the NTT passes deal their tiles out to the 8 XCDs in contiguous chunks */
+  int strict_split; /* limbs of the 63-bit q0 in a launch of their own: -1 never, 0 from 256
+                       descriptors per stage up (default), 1 always */
+} aoa_batch_options;
+void aoa_batch_default_options(aoa_batch_options* opt);
+/* Sets the options; a built batch refills its descriptor tables and
+ * captures a new graph at its next graph launch. */
+int aoa_batch_set_options(aoa_batch* batch, const aoa_batch_options* opt);
+int aoa_batch_get_options(const aoa_batch* batch, aoa_batch_options* opt);
+
+/* One candidate of aoa_batch_autotune: its options, milliseconds per graph
+ * replay, and a position-weighted checksum of every output residue. */
+typedef struct {
+  aoa_batch_options opt;
+  double ms;
+  uint64_t checksum;
+  int exact; /* 1: the checksum equals the first candidate's (0 when the batch's outputs
+                are not reproducible between replays, e.g. in-place rotate-and-sum) */
+} aoa_tune_result;
+
+/* Autotuner: runs every candidate (each NTT split valid for every ring of
+ * the batch, x prime-major or item order, x XCD chunks off / on, x the three
+ * strict-split rules), each as `reps` graph replays after one warm-up
+ * replay, keeps the fastest candidate whose outputs equal the first
+ * candidate's and sets it on the batch.  Writes up to max results; returns
+ * the number of candidates, or -status.  Allocates a few bytes of device
+ * memory for the checksums; synchronises `stream`. */
+int aoa_batch_autotune(aoa_batch* batch, void* stream, int reps, aoa_tune_result* results, int max);
+
 /* Pipeline stages of a batch, in launch order inside each layer.  NTT stages
  * are two kernels (column and row pass); the others one kernel. */
 enum {
