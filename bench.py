@@ -36,6 +36,13 @@ HBM_PEAK_GBS = 8000.0
 # 1 GiB device copy read 1 GiB and write 1 GiB); the run is summarised in
 # profiles/moddown_pmc_traffic_n16_k30.md.
 MODDOWN_TRAFFIC_BYTES = {32: 3025937376}
+# VALU wave-instructions per k_moddown_d launch at 32 pairs (SQ_INSTS_VALU summed
+# over the counter instances of one dispatch, rocprofv3 --pmc of
+# `bench.py --profile`; profiles/hommul_batched_n16_k30_valu.md), and the
+# chip's peak issue rate: one wave-instruction per CU per cycle, 256 CUs at
+# 2.4 GHz.
+MODDOWN_VALU_INSTS = {32: 1.028e9}
+VALU_PEAK_INSTS_S = 256 * 2.4e9
 
 
 def op_bytes(N, L, K):
@@ -118,6 +125,16 @@ def plain_fn(P, evk, items, ws_cache):
                 ws_cache[lv] = P.workspace(lv)
             P.hommul(out, c1, c2, evk, lv, ws_cache[lv])
     return run
+
+
+def valu_issue(insts, launch_ms):
+    """The kernel's VALU issue rate against the chip's: wave-instructions per
+    launch (from the counter profile) over the live launch time."""
+    if not insts:
+        return None
+    rate = insts / (launch_ms * 1e-3)
+    return {"insts_per_launch": insts, "achieved_insts_s": round(rate / 1e9, 1), "peak_insts_s": VALU_PEAK_INSTS_S / 1e9,
+            "unit": "G wave-instr/s", "frac": round(rate / VALU_PEAK_INSTS_S, 4)}
 
 
 def batch_options(bt, key, args, stream):
@@ -237,71 +254,133 @@ def sumslots_n16(args, stream):
     return r
 
 
+PROC_SHAPES = [(8192, 4), (16384, 8), (32768, 16), (65536, 30)]
+
+
+def procedure_table(args, stream, niter=50):
+    """Plain (one call at a time) latency of each procedure of the path, in
+    the style of the reference's benchmark (aoa-ntt/tests/benchmark.cpp:
+    1159-1175, 1324-1344: one procedure, NITERATIONS back-to-back calls on one
+    stream, mean latency per call), at the serving workload's shapes
+    N = 2^13..2^16 with k = 4, 8, 16, 30 (prec 48), kl = k + 1, top level.
+    Returns {procedure: {N: microseconds}}."""
+    import torch
+    from aoa_amd import Params
+    out = {}
+
+    def put(name, N, ms):
+        out.setdefault(name, {})[str(N)] = round(ms * 1000.0, 2)
+
+    for N_, k_ in PROC_SHAPES:
+        P_ = Params(N_, k_, 0, PREC)
+        R = P_.k + P_.kl
+        lv = P_.k - 1
+        x = rand_polys(P_, 2, P_.k)
+        qb = rand_polys(P_, 1, R)[0]
+        keys = rand_polys(P_, 2, R)
+        c3 = [P_.zeros(P_.k), P_.zeros(P_.k)]
+        ws = P_.workspace(lv)
+        ms, _ = time_steps(lambda: P_.ntt(x[0], 0, P_.k, 0), niter, 3, stream)
+        put("NTT", N_, ms)
+        ms, _ = time_steps(lambda: P_.ntt(x[0], 0, P_.k, 1), niter, 3, stream)
+        put("INTT", N_, ms)
+        ms, _ = time_steps(lambda: P_.modup(qb, lv), niter, 3, stream)
+        put("ModUp", N_, ms)
+        ms, _ = time_steps(lambda: P_.moddown(c3[0], qb, lv), niter, 3, stream)
+        put("ModDown", N_, ms)
+        ms, _ = time_steps(lambda: P_.rescale(x[0], x[1], lv), niter, 3, stream)
+        put("Rescale", N_, ms)
+        ms, _ = time_steps(lambda: P_.rotate_left(c3, x, keys, 1, lv, ws), niter, 3, stream)
+        put("Rotate", N_, ms)
+        ms, _ = time_steps(lambda: P_.hommul(c3, x, x, keys, lv, ws), niter, 3, stream)
+        put("Mul", N_, ms)
+        del x, qb, keys, c3, ws, P_
+        torch.cuda.empty_cache()
+    return out
+
+
 def serving_bench(args, rng):
     """Poisson arrivals; ring dimension uniform in {2^13..2^16} with
     {4, 8, 16, 30} limbs; each request is 1-5 HomMul+relin+rescale (at most
     its level) and 0-4 rotations by power-of-two steps, in random order.
-    Keys stay resident; admission by an HBM pool budget."""
+    Keys stay resident; admission by an HBM pool budget.  One run at
+    --rate, then a sweep of arrival rates up to past saturation: offered and
+    served ops/s, p50 and p99 latency per rate."""
     import torch
     from aoa_amd import Params, Server
     shapes = [(8192, 4), (16384, 8), (32768, 16), (65536, 30)]
     steps = [1, 2, 4, 8, 16]
-    srv = Server(pool_bytes=24 << 30, step_ws_bytes=24 << 30, max_active=256)
     sets = []
     for N_, k_ in shapes:
         P_ = Params(N_, k_, 0, PREC)
         evk_ = rand_polys(P_, 2, P_.k + P_.kl)
         rk = rand_polys(P_, 2 * len(steps), P_.k + P_.kl)
         rtk_ = {s: (rk[2 * i], rk[2 * i + 1]) for i, s in enumerate(steps)}
-        ks = srv.register(P_, evk_, rtk_)
         ins = rand_polys(P_, 8, P_.k)  # 4 input ciphertexts and 4 operands per shape
-        sets.append((P_, ks, ins, evk_, rtk_))
-    reqs = []
-    for _ in range(args.requests):
-        s = int(rng.integers(0, len(shapes)))
-        P_ = sets[s][0]
-        nmul = int(rng.integers(1, 6))
-        nmul = min(nmul, P_.k - 1)
-        nrot = int(rng.integers(0, 5))
-        ops = [("mul",)] * nmul + [("rot", int(rng.choice(steps)))] * nrot
-        order = rng.permutation(len(ops))
-        reqs.append((s, [ops[i] for i in order]))
-    gaps = rng.exponential(1.0 / args.rate, size=args.requests)
-    outs = [[sets[s][0].zeros(sets[s][0].k), sets[s][0].zeros(sets[s][0].k)] for s, _ in reqs]
-    torch.cuda.synchronize()
-    srv.start()
-    ids = []
-    t0 = time.perf_counter()
-    t_next = t0
-    for (s, ops), out, gap in zip(reqs, outs, gaps):
-        t_next += gap
-        d = t_next - time.perf_counter()
-        if d > 0:
-            time.sleep(d)
-        P_, ks, ins, _, _ = sets[s]
-        j = int(rng.integers(0, 4))
-        ids.append(srv.submit(ks, (ins[2 * j], ins[2 * j + 1]), P_.k - 1, True, (ins[2 * j + 1], ins[2 * j]), ops,
-                              out))
-    deadline = time.perf_counter() + 300
-    while time.perf_counter() < deadline:
-        if all(srv.result(i)[0] >= Server.DONE for i in ids):
-            break
-        time.sleep(0.005)
-    srv.stop()
-    res = [srv.result(i) for i in ids]
-    st = srv.stats()
-    srv.close()
-    lat = np.array([r[2] for r in res if r[0] == Server.DONE])
-    nops = sum(len(o) for _, o in reqs)
-    return {"requests": args.requests, "completed": int(st["completed"]), "failed": int(sum(r[0] == Server.FAILED
-                                                                                         for r in res)),
-            "arrival_rate_per_s": args.rate, "ops": int(st["ops"]), "steps": int(st["steps"]),
-            "ops_per_s": round(st["ops"] / (st["span_ms"] / 1000.0), 1) if st["span_ms"] > 0 else None,
-            "requests_per_s": round(st["completed"] / (st["span_ms"] / 1000.0), 1) if st["span_ms"] > 0 else None,
-            "p50_ms": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
-            "p99_ms": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
-            "mean_ops_per_request": round(nops / args.requests, 2),
-            "resident_key_bytes": int(st["key_bytes"])}
+        sets.append((P_, ins, evk_, rtk_))
+
+    def run(rate, nreq):
+        srv = Server(pool_bytes=24 << 30, step_ws_bytes=24 << 30, max_active=256)
+        kss = [srv.register(P_, evk_, rtk_) for P_, _, evk_, rtk_ in sets]
+        reqs = []
+        for _ in range(nreq):
+            s = int(rng.integers(0, len(shapes)))
+            P_ = sets[s][0]
+            nmul = min(int(rng.integers(1, 6)), P_.k - 1)
+            nrot = int(rng.integers(0, 5))
+            ops = [("mul",)] * nmul + [("rot", int(rng.choice(steps)))] * nrot
+            order = rng.permutation(len(ops))
+            reqs.append((s, [ops[i] for i in order]))
+        gaps = rng.exponential(1.0 / rate, size=nreq)
+        outs = [[sets[s][0].zeros(sets[s][0].k), sets[s][0].zeros(sets[s][0].k)] for s, _ in reqs]
+        torch.cuda.synchronize()
+        srv.start()
+        ids = []
+        t_next = time.perf_counter()
+        for (s, ops), out, gap in zip(reqs, outs, gaps):
+            t_next += gap
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            P_, ins, _, _ = sets[s]
+            j = int(rng.integers(0, 4))
+            ids.append(srv.submit(kss[s], (ins[2 * j], ins[2 * j + 1]), P_.k - 1, True, (ins[2 * j + 1], ins[2 * j]),
+                                  ops, out))
+        deadline = time.perf_counter() + 300
+        while time.perf_counter() < deadline:
+            if all(srv.result(i)[0] >= Server.DONE for i in ids):
+                break
+            time.sleep(0.002)
+        srv.stop()
+        res = [srv.result(i) for i in ids]
+        st = srv.stats()
+        srv.close()
+        lat = np.array([r[2] for r in res if r[0] == Server.DONE])
+        nops = sum(len(o) for _, o in reqs)
+        span = st["span_ms"] / 1000.0
+        return {"requests": nreq, "completed": int(st["completed"]),
+                "failed": int(sum(r[0] == Server.FAILED for r in res)),
+                "arrival_rate_per_s": rate, "offered_ops_per_s": round(rate * nops / nreq, 1),
+                "ops": int(st["ops"]), "steps": int(st["steps"]),
+                "ops_per_s": round(st["ops"] / span, 1) if span > 0 else None,
+                "requests_per_s": round(st["completed"] / span, 1) if span > 0 else None,
+                "p50_ms": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
+                "p99_ms": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
+                "mean_ops_per_request": round(nops / nreq, 2),
+                "resident_key_bytes": int(st["key_bytes"])}
+
+    out = run(args.rate, args.requests)
+    sweep = []
+    for rate in (300, 600, 1200, 2400, 4800, 9600):
+        r = run(rate, 400)
+        sweep.append({k: r[k] for k in ("arrival_rate_per_s", "offered_ops_per_s", "ops_per_s", "p50_ms", "p99_ms",
+                                         "completed", "failed")})
+    out["sweep"] = sweep
+    # saturation: served throughput stops following the offered load
+    out["capacity_ops_per_s"] = max(r["ops_per_s"] or 0 for r in sweep)
+    knee = [r["arrival_rate_per_s"] for r in sweep if (r["ops_per_s"] or 0) < 0.9 * r["offered_ops_per_s"]]
+    out["saturation_rate_per_s"] = knee[0] if knee else None
+    return out
 
 
 def c1_pipeline(args, stream):
@@ -412,6 +491,25 @@ def cpu_baseline(args, P, plain_items, evk, level):
         t1 = run(1, args.cpu_sample_1)
         res["single_thread_ops_s"] = round(1.0 / t1, 4)
         res["single_thread_sample"] = "%d runs, median" % args.cpu_sample_1
+    # forward and inverse NTT on 30 and 61 limbs (the HomMul's Q and QB
+    # polynomials), median of 5 after a warm-up, all threads and one
+    ntt = {}
+    rng = np.random.default_rng(9)
+    for R in (K_Q, K_Q + K_B):
+        x = np.concatenate([rng.integers(0, O.primes[r], size=N, dtype=np.uint64) for r in range(R)])
+        for nth in (threads, 1):
+            Oracle.threads(nth)
+            for d, name in ((0, "fwd"), (1, "inv")):
+                y = x.copy()
+                O.ntt(y, 0, R, d)
+                ts = []
+                for _ in range(5):
+                    y = x.copy()
+                    t = time.perf_counter()
+                    O.ntt(y, 0, R, d)
+                    ts.append(time.perf_counter() - t)
+                ntt["%s_%d_limbs_ms_%dt" % (name, R, nth)] = round(1000.0 * float(np.median(ts)), 3)
+    res["ntt_ms"] = ntt
     Oracle.threads(threads)
     return res
 
@@ -423,7 +521,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=32, help="ciphertext pairs per batched step")
     ap.add_argument("--cpu-sample", type=int, default=10, help="HomMuls timed on the CPU oracle, all threads (0: skip)")
-    ap.add_argument("--cpu-sample-1", type=int, default=2, help="HomMuls timed on the CPU oracle, one thread")
+    ap.add_argument("--cpu-sample-1", type=int, default=5, help="HomMuls timed on the CPU oracle, one thread")
     ap.add_argument("--no-extra", action="store_true", help="skip the N=2^15 and rotate-and-sum workloads")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
     ap.add_argument("--requests", type=int, default=600, help="serving: number of requests")
@@ -513,7 +611,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(md_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(md_gbs / HBM_PEAK_GBS, 4), "traffic": MODDOWN_TRAFFIC_BYTES.get(B),
                      "kernel": "aoa::k_moddown_d (ModDown + rescale, %d items x 2 polys)" % B,
-                     "launch_ms": round(md_launch_ms, 4), "algorithmic_bytes": md_bytes},
+                     "launch_ms": round(md_launch_ms, 4), "algorithmic_bytes": md_bytes,
+                     "limiter": "valu",
+                     "valu": valu_issue(MODDOWN_VALU_INSTS.get(B), md_launch_ms)},
     }
 
     if args.profile:
@@ -553,6 +653,9 @@ def main():
     del x
 
     if not args.no_extra:
+        res["procedures_us"] = {"note": "plain per-call latency, mean of 50 calls, top level, N=2^13..2^16 with "
+                                        "k=4, 8, 16, 30 (prec 48), kl=k+1",
+                                **procedure_table(args, stream)}
         res["mixed_levels_n15"] = mixed_n15(args, rng, stream)
         res["sumslots_n16"] = sumslots_n16(args, stream)
         res["c1_pipeline_n12"] = c1_pipeline(args, stream)
