@@ -26,6 +26,7 @@ uint64_t chk_mul_barrett(uint64_t a, uint64_t b, uint64_t p, uint64_t mu, int nb
 int64_t chk_exact_mults(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  kl = H.K;  // kl = 0 selects k + 1
   std::mt19937_64 rng(seed);
   int64_t bad = 0;
   for (size_t r = 0; r < H.primes.size(); ++r) {
@@ -67,6 +68,7 @@ int64_t chk_exact_mults(int N, int k, int kl, int prec, int64_t trials, uint64_t
 int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t seed, int64_t* out) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  kl = H.K;  // kl = 0 selects k + 1
   std::mt19937_64 rng(seed);
   int64_t bad = 0, quirks = 0, slow = 0, screened = 0;
   for (int i = 0; i < k; ++i) {
@@ -131,6 +133,7 @@ int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t s
 int64_t chk_barrett_delta(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  kl = H.K;  // kl = 0 selects k + 1
   std::mt19937_64 rng(seed);
   uint64_t bmax = 0;
   for (int j = 0; j < kl; ++j) bmax = H.primes[k + j] > bmax ? H.primes[k + j] : bmax;
@@ -204,6 +207,7 @@ int64_t chk_moddown_screen_words(int64_t trials, uint64_t seed) {
 int64_t chk_moddown_accumulate(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  kl = H.K;  // kl = 0 selects k + 1
   std::mt19937_64 rng(seed);
   int64_t bad = 0;
   const uint32_t M28 = (1u << 28) - 1;
@@ -241,6 +245,7 @@ int64_t chk_moddown_accumulate(int N, int k, int kl, int prec, int64_t trials, u
 int64_t chk_moddown_reduce(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  kl = H.K;  // kl = 0 selects k + 1
   std::mt19937_64 rng(seed);
   int64_t bad = 0;
   const uint32_t M28 = (1u << 28) - 1;
@@ -299,6 +304,7 @@ int64_t chk_moddown_reduce(int N, int k, int kl, int prec, int64_t trials, uint6
 int64_t chk_moddown_finish(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  kl = H.K;  // kl = 0 selects k + 1
   std::mt19937_64 rng(seed);
   int64_t bad = 0;
   typedef unsigned __int128 u128;
@@ -335,6 +341,7 @@ int64_t chk_moddown_finish(int N, int k, int kl, int prec, int64_t trials, uint6
 int64_t chk_table(int N, int k, int kl, int prec, int which, uint64_t* out) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  kl = H.K;  // kl = 0 selects k + 1
   std::vector<uint64_t> v;
   auto take = [&](const std::vector<uint64_t>& src, int stride) {
     for (size_t i = 0; i < src.size(); i += stride) v.push_back(src[i]);
