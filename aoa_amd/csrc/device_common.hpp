@@ -30,6 +30,9 @@ __device__ __forceinline__ tw2 ld_tw2(const tw2* base, size_t i) {
   return tw2{w[0], w[1]};
 }
 __device__ __forceinline__ uint64_t ld_u64(const uint64_t* base, size_t i) { return as_const(base)[i]; }
+// One field of a DevPrime at a wave-uniform index, as a scalar load.
+__device__ __forceinline__ uint64_t ld_prime_p(const DevPrime* base, int i) { return as_const(&base[i].p)[0]; }
+__device__ __forceinline__ int ld_prime_nb(const DevPrime* base, int i) { return (int)as_const(&base[i].nb)[0]; }
 __device__ __forceinline__ DevPrime ld_prime(const DevPrime* base, int i) {
   constexpr int W = (int)(sizeof(DevPrime) / 8);
   const cptr<uint64_t> w = as_const(reinterpret_cast<const uint64_t*>(base + i));

@@ -71,8 +71,23 @@ static std::string select_primes(int k, int kl, int prec, std::vector<aoa_prime_
   for (int j = 0; j < kl; ++j) b->push_back(bb[boff + j]);
   // ModUp splits its operands (< b_j) at bit 28 and ModDown keeps lazy sums
   // of K terms below 2^57 each: both need special primes below 2^56.
-  for (auto& e : *b)
+  uint64_t bmax = 0;
+  for (auto& e : *b) {
     if (e.p >> 56) return "special primes must be below 2^56";
+    bmax = e.p > bmax ? e.p : bmax;
+  }
+  // ModDown's lazy sums of limb i hold K products a c (a < b_j, c < q_i), one
+  // folded term below 2^57 q_i and at most K screen corrections below 2 q_i:
+  // reduce_sums_lazy needs their total below 2^(62+nb), which
+  // K b_max + 2^57 + 2K <= 2^62 ensures.
+  typedef unsigned __int128 u128;
+  if ((u128)kl * bmax + ((u128)1 << 57) + 2 * (u128)kl > ((u128)1 << 62))
+    return "special primes too large for ModDown's lazy sums";
+  // ModDown's rescale forms q_i + add_i - top with top < q_l for i < l, which
+  // needs q_l < q_i: the Q primes decrease (q0 first, then the bucket in
+  // decreasing order).
+  for (size_t i = 1; i < q->size(); ++i)
+    if ((*q)[i].p >= (*q)[i - 1].p) return "Q primes must decrease";
   return "";
 }
 
@@ -219,6 +234,7 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
   P.md_thr.assign((size_t)k, 0);
   P.md_red.assign((size_t)k * 2, 0);
   P.md_fin.assign((size_t)k * k * 2, 0);
+  P.md_negb.assign((size_t)k * 4, 0);
   P.inv_b_mod_q.assign((size_t)k * 2, 0);
   P.b_mod_q.assign((size_t)k * 2, 0);
   uint64_t bmax = 0;
@@ -298,6 +314,12 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
       const uint64_t f = mulm(inv, P.inv_ql_mod_qi[((size_t)l * k + i) * 2], qi);
       P.md_fin[((size_t)l * k + i) * 2] = f;
       P.md_fin[((size_t)l * k + i) * 2 + 1] = shoup_quot(f, qi);
+    }
+    if (d.nb <= 56) {
+      const uint64_t nb_ = (qi - bfull) % qi;
+      P.md_negb[4 * i] = (uint32_t)(nb_ & ((1u << 28) - 1));
+      P.md_negb[4 * i + 1] = (uint32_t)(nb_ >> 28);
+      P.md_negb[4 * i + 2] = P.md_negb[4 * i] + P.md_negb[4 * i + 1];
     }
     P.b_mod_q[2 * i] = bfull;
     P.b_mod_q[2 * i + 1] = shoup_quot(bfull, qi);

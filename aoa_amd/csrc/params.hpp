@@ -80,6 +80,9 @@ struct DevParams {
   // and the bound of ModDown's lazy sums, 5 q_i (q0: q0, whose sums are exact)
   const uint64_t* md_red;
   const uint64_t* md_fin;        // [k][k][2]: (l,i) -> B^-1 q_l^-1 mod q_i, Shoup
+  // [k][4]: 28-bit halves (low, high, their sum, 0) of (-B) mod q_i, Q primes
+  // below 2^57: ModDown adds z (-B) to its lazy sums as one more term
+  const uint32_t* md_negb;
 };
 
 // The device layout of the ModDown product constants: MdConst's first six
@@ -113,6 +116,7 @@ struct HostParams {
   std::vector<uint32_t> md_thr;     // k entries
   std::vector<uint64_t> md_red;     // k * 2
   std::vector<uint64_t> md_fin;     // k * k * 2
+  std::vector<uint32_t> md_negb;    // k * 4
 };
 
 // Builds every table.  Returns an empty string on success, else the reason

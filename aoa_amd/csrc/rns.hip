@@ -335,23 +335,27 @@ __device__ __forceinline__ uint64_t moddown_term(uint64_t a, const tw2 w, const 
 // (as = al + ah, cs = cl + ch): three 32x32->64 multiply-adds into 64-bit
 // sums that cannot overflow for up to 64 terms, one 128-bit reduction per
 // limb.  Exact sums equal the reference's term-by-term modular sum whenever
-// no term deviates.  SCREEN runs moddown_screen on every product and records,
-// per lane and limb, whether any product passed it and, per wave, the terms j
-// for which some lane's product did (jm: the scalar OR of the compare masks).
-// A lane whose limb has a passing product (one term in 10 000 to 300 000 at
-// 48-bit q_i, growing with 2^48 - q_i) screens only the terms in jm again and
-// adds ref(a c) - (a c mod q_i) for each one that passes.
-template <int NL, bool SCREEN>
+// no term deviates.  SCREEN runs moddown_screen on every product; when a
+// product of some lane passes it (one term in 10 000 to 300 000 at 48-bit
+// q_i, growing with 2^48 - q_i) the wave takes a branch in which the lanes
+// whose product passed add ref(a c) - (a c mod q_i) + q_i (in (0, 2 q_i)) to
+// their low sum (and to sm, so the middle sum keeps its value): every other
+// term lies outside the deviation window, where the two agree.  The pass/fail
+// masks come straight from the compares and are not carried across
+// iterations.  fold(i, &z) may add one more term z (-B mod q_i), z < 2^57
+// (the caller's addend and the rescale's dropped limb), so that the caller
+// finishes the limb with one Shoup product.  The host checks that K products,
+// the folded term and K corrections stay below 2^(62+nb) (build_host_params).
+template <int NL, bool SCREEN, class Fold>
 __device__ __forceinline__ void moddown_conv(const DevParams& P, const uint64_t* aux, int c, int i0,
-                                             int j0, int j1, uint64_t* res) {
+                                             int j0, int j1, uint64_t* res, Fold fold) {
   constexpr uint32_t M28 = (1u << 28) - 1;
   constexpr int TW = (int)(sizeof(MdTerm) / 4);
-  const int k = P.k;
+  const int k = P.k, K = P.K;
   const cptr<uint32_t> terms = as_const(P.md_terms);
+  const tw2* bq = reinterpret_cast<const tw2*>(P.bhat_mod_q);
   uint64_t s0[NL], s2[NL], sm[NL];
-  bool hit[NL];
   uint32_t thr[NL];
-  uint64_t jm = 0;  // bit j - j0: the screen passed term j in some lane (wave-uniform)
   // moddown_screen's bias, opaque to the compiler so that it stays the addend
   // of the first multiply-add instead of becoming an addition of its own
   uint64_t bias = 2ull << 32;
@@ -359,7 +363,6 @@ __device__ __forceinline__ void moddown_conv(const DevParams& P, const uint64_t*
 #pragma unroll
   for (int n = 0; n < NL; ++n) {
     s0[n] = s2[n] = sm[n] = 0;
-    hit[n] = false;
     thr[n] = as_const(P.md_thr)[i0 + n * BC_GROUPS];
   }
   for (int j = j0; j < j1; ++j) {
@@ -367,6 +370,7 @@ __device__ __forceinline__ void moddown_conv(const DevParams& P, const uint64_t*
     const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
     const uint32_t al = a0 & M28, ah = (uint32_t)(a >> 28), as = al + ah;
     const cptr<uint32_t> t = terms + ((size_t)j * k + i0) * TW;
+    bool h[NL];
     uint64_t any = 0;
 #pragma unroll
     for (int n = 0; n < NL; ++n) {
@@ -375,52 +379,61 @@ __device__ __forceinline__ void moddown_conv(const DevParams& P, const uint64_t*
       s2[n] += (uint64_t)ah * e[1];
       sm[n] += (uint64_t)as * e[2];
       if (SCREEN) {
-        const bool h = moddown_screen(a0, a1, e[3], e[4], e[5], bias) <= thr[n];
-        hit[n] |= h;
-        any |= __builtin_amdgcn_ballot_w64(h);
+        h[n] = moddown_screen(a0, a1, e[3], e[4], e[5], bias) <= thr[n];
+        any |= __builtin_amdgcn_ballot_w64(h[n]);
       }
     }
-    if (SCREEN && any) jm |= 1ull << (j - j0);
-  }
-  const tw2* bq = reinterpret_cast<const tw2*>(P.bhat_mod_q);
-  const int K = P.K;
+    if (SCREEN && any) {
+      // j made opaque here so that the compiler does not keep running
+      // pointers into bq across the loop for this rarely taken branch
+      int jo = j;
+      asm volatile("" : "+s"(jo));
 #pragma unroll
-  for (int n = 0; n < NL; ++n) {
-    const int i = i0 + n * BC_GROUPS;
-    const uint64_t p = P.primes[i].p;
-    const cptr<uint64_t> red = as_const(P.md_red) + 2 * i;
-    uint64_t r = reduce_sums_lazy(s0[n], sm[n] - s0[n] - s2[n], s2[n], p, red[0], (int)P.primes[i].nb);
-    if (SCREEN && hit[n]) {
-      // Add ref(a c) - (a c mod q_i) for the terms that pass the screen
-      // again; every other term lies outside the deviation window, where
-      // the two agree.
-      const DevPrime dq = ld_prime(P.primes, i);
-      for (uint64_t m = 4 * p; m >= p; m >>= 1) r = r >= m ? r - m : r;
-      const tw2* ci = bq + (size_t)i * K;
-      for (uint64_t m = jm; m; m &= m - 1) {
-        const int j = j0 + __builtin_ctzll(m);
-        const uint64_t a = aux[j * BC_COEF + c];
-        const cptr<uint32_t> e = terms + ((size_t)j * k + i) * TW;
-        if (moddown_screen((uint32_t)a, (uint32_t)(a >> 32), e[3], e[4], e[5]) <= thr[n]) {
-          const tw2 w = ld_tw2(ci, j);
+      for (int n = 0; n < NL; ++n) {
+        if (h[n]) {
+          const int i = i0 + n * BC_GROUPS;
+          const DevPrime dq = ld_prime(P.primes, i);
+          const tw2 w = ld_tw2(bq + (size_t)i * K, jo);
           const uint64_t x = mul_shoup(a, w.w, w.q, dq.p);
           const uint64_t f = ref_barrett(mulhi64(a, w.w), a * w.w, dq.p, dq.mu, (int)dq.nb);
-          r = add_mod(r, sub_mod(f, x, dq.p), dq.p);
+          const uint64_t d = f + dq.p - x;
+          s0[n] += d;
+          sm[n] += d;  // leaves the middle sum sm - s0 - s2 as it was
         }
       }
     }
-    res[n] = r;
+  }
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    const int i = i0 + n * BC_GROUPS;
+    uint64_t z;
+    if (fold(i, &z)) {
+      // one more term z (-B mod q_i), z < 2^57: halves below 2^29
+      const cptr<uint32_t> e = as_const(P.md_negb) + 4 * i;
+      const uint32_t zl = (uint32_t)z & M28, zh = (uint32_t)(z >> 28), zs = zl + zh;
+      s0[n] += (uint64_t)zl * e[0];
+      s2[n] += (uint64_t)zh * e[1];
+      sm[n] += (uint64_t)zs * e[2];
+    }
+    const cptr<uint64_t> red = as_const(P.md_red) + 2 * i;
+    res[n] = reduce_sums_lazy(s0[n], sm[n] - s0[n] - s2[n], s2[n], ld_prime_p(P.primes, i), red[0],
+                              ld_prime_nb(P.primes, i));
   }
 }
 
-template <int NL>
+template <int NL, class Fold>
 __device__ __forceinline__ void moddown_convs(const DevParams& P, const uint64_t* aux, int c, int i0,
-                                              int j0, int j1, uint64_t* res) {
+                                              int j0, int j1, uint64_t* res, Fold fold) {
   if (P.moddown_exact)
-    moddown_conv<NL, false>(P, aux, c, i0, j0, j1, res);
+    moddown_conv<NL, false>(P, aux, c, i0, j0, j1, res, fold);
   else
-    moddown_conv<NL, true>(P, aux, c, i0, j0, j1, res);
+    moddown_conv<NL, true>(P, aux, c, i0, j0, j1, res, fold);
 }
+
+// No folded term.
+struct NoFold {
+  __device__ bool operator()(int, uint64_t*) const { return false; }
+};
 
 __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t* inQ, const uint64_t* inB,
                                              const uint64_t* add, uint64_t* out, int level, int rescale,
@@ -456,50 +469,67 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   // (x_i - acc) B^-1 (+ add_i) mod q_i from the sum acc of limb i, which is
   // below md_red[i][1] (5 q_i, or q0 for the canonical q0 sums).
   auto finish = [&](int i, uint64_t acc) -> uint64_t {
-    const uint64_t p = P.primes[i].p;
-    const tw2 bi = ibq[i];
+    const uint64_t p = ld_prime_p(P.primes, i);
+    const tw2 bi = ld_tw2(ibq, i);
     const uint64_t y = inQ[(size_t)i * N + cid] + as_const(P.md_red)[2 * i + 1] - acc;
     uint64_t v = mul_shoup(y, bi.w, bi.q, p);
     if (add) v = add_mod(v, add[(size_t)i * N + cid], p);
     return v;
   };
-  // This wave's limbs g, g+4, ... below iend, in passes of four, two or one
-  // (limbs of primes wider than 56 bits, i.e. q0, one at a time with
-  // conv).  store() writes limb i from its sum.
-  auto limbs = [&](int iend, auto store) {
+  // x_i + md_red[i][1] - acc: (x_i - acc) mod q_i as a value below 2^64
+  auto lift = [&](int i, uint64_t acc) -> uint64_t {
+    return inQ[(size_t)i * N + cid] + as_const(P.md_red)[2 * i + 1] - acc;
+  };
+  // This wave's limbs g, g+4, ... below iend, in passes of four, two or one.
+  // A limb of a prime wider than 56 bits (q0) is summed term by term with
+  // conv and written by wide(i, acc); the others take the lazy sums of
+  // moddown_convs, with the term z (-B) when fold(i, &z) says so, and are
+  // written by narrow(i, acc).
+  auto limbs = [&](int iend, auto wide, auto fold, auto narrow) {
     int i = g;
     while (i < iend) {
       const int left = (iend - i + BC_GROUPS - 1) / BC_GROUPS;
-      if (P.primes[i].nb > 56) {
-        store(i, conv(i, 0, K));
+      if (ld_prime_nb(P.primes, i) > 56) {
+        wide(i, conv(i, 0, K));
         i += BC_GROUPS;
         continue;
       }
       int li[4] = {i, i + BC_GROUPS, i + 2 * BC_GROUPS, i + 3 * BC_GROUPS};
       // the first n limbs of li all have primes below 2^57
-      auto narrow = [&](int n) {
+      auto is_narrow = [&](int n) {
         bool ok = true;
-        for (int t = 1; t < n; ++t) ok &= P.primes[li[t]].nb <= 56;
+        for (int t = 1; t < n; ++t) ok &= ld_prime_nb(P.primes, li[t]) <= 56;
         return ok;
       };
       uint64_t r[4];
-      if (left >= 4 && narrow(4)) {
-        moddown_convs<4>(P, aux, c, i, 0, K, r);
-        for (int n = 0; n < 4; ++n) store(li[n], r[n]);
+      if (left >= 4 && is_narrow(4)) {
+        moddown_convs<4>(P, aux, c, i, 0, K, r, fold);
+        for (int n = 0; n < 4; ++n) narrow(li[n], r[n]);
         i += 4 * BC_GROUPS;
-      } else if (left >= 2 && narrow(2)) {
-        moddown_convs<2>(P, aux, c, i, 0, K, r);
-        for (int n = 0; n < 2; ++n) store(li[n], r[n]);
+      } else if (left >= 2 && is_narrow(2)) {
+        moddown_convs<2>(P, aux, c, i, 0, K, r, fold);
+        for (int n = 0; n < 2; ++n) narrow(li[n], r[n]);
         i += 2 * BC_GROUPS;
       } else {
-        moddown_convs<1>(P, aux, c, i, 0, K, r);
-        store(i, r[0]);
+        moddown_convs<1>(P, aux, c, i, 0, K, r, fold);
+        narrow(i, r[0]);
         i += BC_GROUPS;
       }
     }
   };
   if (!rescale) {
-    limbs(level + 1, [&](int i, uint64_t acc) { out[(size_t)i * N + cid] = finish(i, acc); });
+    // add_i joins the sums as z = add_i: (x_i - acc + add_i B) B^-1 = (x_i - acc) B^-1 + add_i
+    limbs(
+        level + 1, [&](int i, uint64_t acc) { out[(size_t)i * N + cid] = finish(i, acc); },
+        [&](int i, uint64_t* z) {
+          if (!add) return false;
+          *z = add[(size_t)i * N + cid];
+          return true;
+        },
+        [&](int i, uint64_t acc) {
+          const tw2 bi = ld_tw2(ibq, i);
+          out[(size_t)i * N + cid] = mul_shoup(lift(i, acc), bi.w, bi.q, ld_prime_p(P.primes, i));
+        });
     return;
   }
   // Rescale needs limb `level` everywhere: its sum over j is split across the
@@ -509,15 +539,14 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   const int j0 = g * jq, j1 = min(K, j0 + jq);
   {
     uint64_t r = 0;
-    if (P.primes[level].nb > 56) {
+    if (ld_prime_nb(P.primes, level) > 56) {
       r = conv(level, j0, j1 > j0 ? j1 : j0);
     } else if (j1 > j0) {
-      moddown_convs<1>(P, aux, c, level, j0, j1, &r);
+      moddown_convs<1>(P, aux, c, level, j0, j1, &r, NoFold());
     }
     part[g * BC_COEF + c] = r;
   }
   __syncthreads();
-  const uint64_t pl = P.primes[level].p;
   const uint64_t ml = as_const(P.md_red)[2 * level + 1];
   uint64_t acc = 0;
   // four partial sums below ml each: their sum stays below 2^64 and below ml
@@ -526,16 +555,28 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   for (int t = 0; t < BC_GROUPS; ++t) acc = add_mod(acc, part[t * BC_COEF + c], ml);
   const uint64_t top = finish(level, acc);
   // out_i = ((x_i - acc_i) B^-1 + add_i - top) q_l^-1
-  //       = (x_i - acc_i) (B^-1 q_l^-1) + (add_i - top) q_l^-1  (mod q_i); top < q_l < q_i
+  //       = (x_i - acc_i + z_i B) (B^-1 q_l^-1),  z_i = q_i + add_i - top  (mod q_i)
+  // with top < q_l < q_i (the Q primes decrease): 0 < z_i < 2 q_i.  The
+  // narrow limbs add the term z_i (-B) to their sums and take one Shoup
+  // product; q0 (summed term by term) multiplies both parts.
   const tw2* iql = reinterpret_cast<const tw2*>(P.inv_ql_mod_qi) + (size_t)level * k;
   const tw2* fin = reinterpret_cast<const tw2*>(P.md_fin) + (size_t)level * k;
-  limbs(level, [&](int i, uint64_t acc) {
-    const uint64_t p = P.primes[i].p;
-    const uint64_t y = inQ[(size_t)i * N + cid] + as_const(P.md_red)[2 * i + 1] - acc;
-    const uint64_t z = p + (add ? add[(size_t)i * N + cid] : 0) - top;
-    const tw2 f = ld_tw2(fin, i), s = ld_tw2(iql, i);
-    out[(size_t)i * N + cid] = add_mod(mul_shoup(y, f.w, f.q, p), mul_shoup(z, s.w, s.q, p), p);
-  });
+  auto zval = [&](int i) { return ld_prime_p(P.primes, i) + (add ? add[(size_t)i * N + cid] : 0) - top; };
+  limbs(
+      level,
+      [&](int i, uint64_t acc) {
+        const uint64_t p = ld_prime_p(P.primes, i);
+        const tw2 f = ld_tw2(fin, i), sq = ld_tw2(iql, i);
+        out[(size_t)i * N + cid] = add_mod(mul_shoup(lift(i, acc), f.w, f.q, p), mul_shoup(zval(i), sq.w, sq.q, p), p);
+      },
+      [&](int i, uint64_t* z) {
+        *z = zval(i);
+        return true;
+      },
+      [&](int i, uint64_t acc) {
+        const tw2 f = ld_tw2(fin, i);
+        out[(size_t)i * N + cid] = mul_shoup(lift(i, acc), f.w, f.q, ld_prime_p(P.primes, i));
+      });
 }
 
 // Eight waves per SIMD, as k_moddown_d: 2.5 % more plain HomMuls per second.

@@ -300,7 +300,9 @@ int64_t chk_moddown_reduce(int N, int k, int kl, int prec, int64_t trials, uint6
 // every Q prime below 2^57 and level l: ((x - T) B^-1 + add - top) q_l^-1
 // mod q_i, computed as (x + 5q - r) (B^-1 q_l^-1) + (q + add - top) q_l^-1 from
 // r = reduce_sums_lazy (the rescale path), and (x + 5q - r) B^-1 + add without
-// the rescale.  Returns the number of mismatches.
+// the rescale; then as the kernel's narrow limbs do, with z = q + add - top
+// (or z = add) added to the sums as the term z (-B mod q) and one Shoup
+// product by B^-1 q_l^-1 (or B^-1).  Returns the number of mismatches.
 int64_t chk_moddown_finish(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
@@ -330,6 +332,20 @@ int64_t chk_moddown_finish(int N, int k, int kl, int prec, int64_t trials, uint6
         const uint64_t want = (v + add) % q;
         const uint64_t got = add_mod(mul_shoup(y, ib, ibq, q), add, q);
         if (got_r != want_r || got != want) ++bad;
+        // the kernel's narrow limbs: z = q + add - top (rescale) or z = add
+        // joins the sums as the term z (-B mod q), then one Shoup product
+        const uint32_t* e = &H.md_negb[4 * i];
+        const uint64_t negb = (uint64_t)e[0] + ((uint64_t)e[1] << 28);
+        if (e[2] != e[0] + e[1] || e[0] >> 28 || negb != (q - H.b_mod_q[2 * i]) % q) ++bad;
+        auto folded = [&](uint64_t z, uint64_t w, uint64_t wq) {
+          const uint32_t M28 = (1u << 28) - 1;
+          const uint32_t zl = (uint32_t)z & M28, zh = (uint32_t)(z >> 28);
+          const uint64_t f0 = s0 + (uint64_t)zl * e[0], f2 = s2 + (uint64_t)zh * e[1];
+          const uint64_t f1 = s1 + (uint64_t)zl * e[1] + (uint64_t)zh * e[0];
+          const uint64_t rf = reduce_sums_lazy(f0, f1, f2, q, H.md_red[2 * i], (int)d.nb);
+          return mul_shoup(x + H.md_red[2 * i + 1] - rf, w, wq, q);
+        };
+        if (folded(q + add - top, f, fq) != want_r || folded(add, ib, ibq) != want) ++bad;
       }
     }
   }
