@@ -78,6 +78,7 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
       {H.md_red.data(), H.md_red.size() * 8, 0},
       {H.md_fin.data(), H.md_fin.size() * 8, 0},
       {H.md_negb.data(), H.md_negb.size() * 4, 0},
+      {H.mu_terms.data(), H.mu_terms.size() * 4, 0},
   };
   const int nsec = sizeof(secs) / sizeof(secs[0]);
   size_t total = 0;
@@ -121,6 +122,7 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
   D.md_red = (const uint64_t*)(b + secs[15].off);
   D.md_fin = (const uint64_t*)(b + secs[16].off);
   D.md_negb = (const uint32_t*)(b + secs[17].off);
+  D.mu_terms = (const uint32_t*)(b + secs[18].off);
   *out = h;
   return AOA_OK;
 }

@@ -218,6 +218,22 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
       P.inv_ql_mod_qi[((size_t)l * k + i) * 2 + 1] = shoup_quot(v, Qp[i]);
     }
   }
+  // ModUp's constants (Q_l/q_i) mod b_j split at bit 28, in the order the
+  // kernel reads them: one block of kp = K rounded up to 16 entries per (l, i),
+  // special limb j = g + 4t (g = j % 4) at position (t / 4) 16 + 4 g + t % 4,
+  // so that the limbs of one wave's pass are consecutive entries.
+  const int kp = (kl + 15) & ~15;
+  P.mu_terms.assign((size_t)k * k * kp * 4, 0);
+  for (int l = 0; l < k; ++l)
+    for (int i = 0; i <= l; ++i)
+      for (int j = 0; j < kl; ++j) {
+        const uint64_t w = P.qhat_mod_b[((size_t)l * kl + j) * k + i];
+        const int t = j / 4, pos = (t / 4) * 16 + 4 * (j % 4) + t % 4;
+        uint32_t* e = &P.mu_terms[(((size_t)l * k + i) * kp + pos) * 4];
+        e[0] = (uint32_t)(w & ((1u << 28) - 1));
+        e[1] = (uint32_t)(w >> 28);
+        e[2] = e[0] + e[1];
+      }
   P.inv_bhat.assign((size_t)kl * 2, 0);
   for (int j = 0; j < kl; ++j) {
     uint64_t bj = Bp[j];

@@ -83,6 +83,9 @@ struct DevParams {
   // [k][4]: 28-bit halves (low, high, their sum, 0) of (-B) mod q_i, Q primes
   // below 2^57: ModDown adds z (-B) to its lazy sums as one more term
   const uint32_t* md_negb;
+  // [k][k][kp][4], kp = K rounded up to 16: (l, i, position of j) -> 28-bit
+  // halves of (Q_l/q_i) mod b_j and their sum; positions as in build_host_params
+  const uint32_t* mu_terms;
 };
 
 // The device layout of the ModDown product constants: MdConst's first six
@@ -117,6 +120,7 @@ struct HostParams {
   std::vector<uint64_t> md_red;     // k * 2
   std::vector<uint64_t> md_fin;     // k * k * 2
   std::vector<uint32_t> md_negb;    // k * 4
+  std::vector<uint32_t> mu_terms;   // k * k * kp * 4
 };
 
 // Builds every table.  Returns an empty string on success, else the reason

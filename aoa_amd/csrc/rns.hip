@@ -202,46 +202,61 @@ constexpr int BC_GROUPS = 4;
 // q_i > b_j (always for q0).  With v, w < 2^56 split at bit 28 and
 // vs = vl + vh, ws = wl + wh: v w = vl wl + (vs ws - vl wl - vh wh) 2^28 +
 // vh wh 2^56, three 32x32->64 multiply-adds per term into 64-bit sums that
-// cannot overflow for up to 64 terms, one 128-bit reduction per limb.
+// cannot overflow for up to 64 terms, one 128-bit reduction per limb.  The
+// halves of w come split from the mu_terms table, the NL limbs' entries
+// side by side, so that a term takes one scalar load and no scalar ALU work.
 template <int NL>
 __device__ __forceinline__ void modup_pass(const DevParams& P, const uint64_t* tl, int c, const int* jl, int level,
                                            uint64_t* outB, size_t cid) {
-  constexpr uint64_t M28 = (1ull << 28) - 1;
+  constexpr uint32_t M28 = (1u << 28) - 1;
   const int L = level + 1;
+  // The pass's limbs jl[n] = jl[0] + 4n are consecutive entries of mu_terms:
+  // (level, i, pos0 + n) sits at e0 + i kp 4 + 4n.
+  const int kp = (P.K + 15) & ~15;
+  const int t0 = jl[0] / BC_GROUPS;
+  const int pos0 = (t0 / 4) * 16 + 4 * (jl[0] % BC_GROUPS) + t0 % 4;
+  const cptr<uint32_t> e0 = as_const(P.mu_terms) + ((size_t)level * P.k * kp + pos0) * 4;
   DevPrime d[NL];
-  const uint64_t* cw[NL];
   uint32_t red = 0;
   uint64_t s0[NL], s2[NL], sm[NL];
 #pragma unroll
   for (int n = 0; n < NL; ++n) {
     d[n] = ld_prime(P.primes, P.k + jl[n]);
-    cw[n] = P.qhat_mod_b + ((size_t)level * P.K + jl[n]) * P.k;
     red |= d[n].modup_reduce;
     s0[n] = s2[n] = sm[n] = 0;
   }
-  auto acc = [&](int n, uint64_t v, uint64_t w) {
-    const uint32_t vl = (uint32_t)(v & M28), vh = (uint32_t)(v >> 28);
-    const uint32_t wl = (uint32_t)(w & M28), wh = (uint32_t)(w >> 28);
-    s0[n] += (uint64_t)vl * wl;
-    s2[n] += (uint64_t)vh * wh;
-    sm[n] += (uint64_t)(vl + vh) * (wl + wh);
+  auto acc = [&](int n, uint64_t v, cptr<uint32_t> e) {
+    const uint32_t vl = (uint32_t)v & M28, vh = (uint32_t)(v >> 28);
+    s0[n] += (uint64_t)vl * e[0];
+    s2[n] += (uint64_t)vh * e[1];
+    sm[n] += (uint64_t)(vl + vh) * e[2];
   };
   auto reduced = [&](int i) {
     const uint64_t v = tl[i * BC_COEF + c];
-    const uint64_t qi = ld_prime(P.primes, i).p;
+    const uint64_t qi = ld_prime_p(P.primes, i);
+    const cptr<uint32_t> e = e0 + (size_t)i * kp * 4;
 #pragma unroll
     for (int n = 0; n < NL; ++n)
-      acc(n, qi > d[n].p ? barrett_reduce(0, v, d[n].p, d[n].mu, (int)d[n].nb) : v, ld_u64(cw[n], i));
+      acc(n, qi > d[n].p ? barrett_reduce(0, v, d[n].p, d[n].mu, (int)d[n].nb) : v, e + 4 * n);
   };
   reduced(0);  // q0 is wider than every special prime
   if (red) {
     for (int i = 1; i < L; ++i) reduced(i);
   } else {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 #pragma unroll 2
     for (int i = 1; i < L; ++i) {
       const uint64_t v = tl[i * BC_COEF + c];
+      const uint32_t vl = (uint32_t)v & M28, vh = (uint32_t)(v >> 28), vs = vl + vh;
+      // whole 16-byte entries, so that the NL entries load as one block
+      const cptr<u32x4> e = (cptr<u32x4>)(e0 + (size_t)i * kp * 4);
 #pragma unroll
-      for (int n = 0; n < NL; ++n) acc(n, v, ld_u64(cw[n], i));
+      for (int n = 0; n < NL; ++n) {
+        const u32x4 w = e[n];
+        s0[n] += (uint64_t)vl * w.x;
+        s2[n] += (uint64_t)vh * w.y;
+        sm[n] += (uint64_t)vs * w.z;
+      }
     }
   }
 #pragma unroll
@@ -480,7 +495,7 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   auto lift = [&](int i, uint64_t acc) -> uint64_t {
     return inQ[(size_t)i * N + cid] + as_const(P.md_red)[2 * i + 1] - acc;
   };
-  // This wave's limbs g, g+4, ... below iend, in passes of four, two or one.
+  // This wave's limbs g, g+4, ... below iend, in passes of four, three, two or one.
   // A limb of a prime wider than 56 bits (q0) is summed term by term with
   // conv and written by wide(i, acc); the others take the lazy sums of
   // moddown_convs, with the term z (-B) when fold(i, &z) says so, and are
@@ -506,6 +521,10 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
         moddown_convs<4>(P, aux, c, i, 0, K, r, fold);
         for (int n = 0; n < 4; ++n) narrow(li[n], r[n]);
         i += 4 * BC_GROUPS;
+      } else if (left == 3 && is_narrow(3)) {
+        moddown_convs<3>(P, aux, c, i, 0, K, r, fold);
+        for (int n = 0; n < 3; ++n) narrow(li[n], r[n]);
+        i += 3 * BC_GROUPS;
       } else if (left >= 2 && is_narrow(2)) {
         moddown_convs<2>(P, aux, c, i, 0, K, r, fold);
         for (int n = 0; n < 2; ++n) narrow(li[n], r[n]);
