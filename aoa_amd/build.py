@@ -78,6 +78,21 @@ def build_newckks(verbose=False):
     os.replace(CPP_TEST + ".tmp", CPP_TEST)
     if verbose:
         print("built", NEWCKKS_LIB, CPP_TEST)
+    build_issue_probe(verbose)
+
+
+ISSUE_PROBE = os.path.join(ROOT, "tests", "native", "_build", "issue_rate")
+
+
+def build_issue_probe(verbose=False):
+    """tests/native/_build/issue_rate: the issue-rate probe behind the VALU
+    reference rates quoted with the roofline (run by a GPU test)."""
+    os.makedirs(os.path.dirname(ISSUE_PROBE), exist_ok=True)
+    _run([HIPCC, "-O3", "-std=c++17"] + HIP_FLAGS[:1] + HIP_FLAGS[3:] +
+         [os.path.join(ROOT, "tests", "native", "issue_rate.hip"), "-o", ISSUE_PROBE + ".tmp"])
+    os.replace(ISSUE_PROBE + ".tmp", ISSUE_PROBE)
+    if verbose:
+        print("built", ISSUE_PROBE)
 
 
 if __name__ == "__main__":

@@ -30,6 +30,16 @@ __device__ __forceinline__ tw2 ld_tw2(const tw2* base, size_t i) {
   return tw2{w[0], w[1]};
 }
 __device__ __forceinline__ uint64_t ld_u64(const uint64_t* base, size_t i) { return as_const(base)[i]; }
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on
+// XCD w % 8).  This map deals the n work items out in 8 contiguous chunks,
+// one per XCD (the first n % 8 chunks one item longer): XCD x receives
+// exactly the ids w < n with w % 8 == x, as many as its chunk holds, so the
+// map is a bijection of [0, n).
+__device__ __forceinline__ int xcd_chunk_id(unsigned w, unsigned n) {
+  const unsigned q = n >> 3, r = n & 7, x = w & 7;
+  return (int)(x * q + (x < r ? x : r) + (w >> 3));
+}
+
 // One field of a DevPrime at a wave-uniform index, as a scalar load.
 __device__ __forceinline__ uint64_t ld_prime_p(const DevPrime* base, int i) { return as_const(&base[i].p)[0]; }
 __device__ __forceinline__ int ld_prime_nb(const DevPrime* base, int i) { return (int)as_const(&base[i].nb)[0]; }

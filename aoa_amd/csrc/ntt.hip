@@ -516,18 +516,11 @@ __device__ __forceinline__ bool use_lazy(const DevParams& P, int pr) {
   return MODE == 1 ? true : (MODE == 2 ? false : lazy_prime(P, pr));
 }
 
-// Work item of this workgroup.  Workgroups are dispatched round-robin over
-// the 8 XCDs (workgroup w runs on XCD w % 8).  With P.xcd_chunks the n work
-// items are dealt out in 8 contiguous chunks, one per XCD (the first n % 8
-// chunks one item longer), so that neighbouring items -- in a prime-major
-// descriptor list, the tiles of one prime -- share an XCD and its L2.  The
-// map is a bijection of [0, n): XCD x receives exactly the ids w < n with
-// w % 8 == x, as many as its chunk holds.
+// Work item of this workgroup: blockIdx.x, or with P.xcd_chunks its XCD
+// chunk id (xcd_chunk_id), so that neighbouring items -- in a prime-major
+// descriptor list, the tiles of one prime -- share an XCD and its L2.
 __device__ __forceinline__ int work_id(const DevParams& P) {
-  const unsigned w = blockIdx.x;
-  if (!P.xcd_chunks) return (int)w;
-  const unsigned n = gridDim.x, q = n >> 3, r = n & 7, x = w & 7;
-  return (int)(x * q + (x < r ? x : r) + (w >> 3));
+  return P.xcd_chunks ? xcd_chunk_id(blockIdx.x, gridDim.x) : (int)blockIdx.x;
 }
 
 template <int S, int MODE>

@@ -734,11 +734,17 @@ void launch_dr2(const DevParams& P, uint64_t* c21, uint64_t* c22, uint64_t* c23,
 // ---------------------------------------------------------- automorphism ----
 // Forward map (encoder.cu:300-322): X^c -> X^(c g mod 2N), negated on wrap.
 // Gather form: out[j] = in[t] if t = j g^-1 mod 2N < N, else -in[t - N].
+// The gather reads one limb at a stride of g^-1: every 8-byte read falls in
+// another cache line, so the source limb (N 8 bytes, 512 KiB at N=2^16) has
+// to stay in L2 while its workgroups run.  Workgroups take their block of the
+// output through xcd_chunk_id, so the blocks of one limb run on one XCD (one
+// L2) one after another instead of on all eight.
 __global__ __launch_bounds__(256) void k_automorphism(DevParams P, uint64_t* out, const uint64_t* in,
                                                       uint64_t ginv) {
   const size_t N = (size_t)P.N;
-  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-  const int li = (int)(((size_t)blockIdx.x * 256) >> P.logN);
+  const size_t blk = (size_t)xcd_chunk_id(blockIdx.x, gridDim.x);
+  const size_t e = blk * 256 + threadIdx.x;
+  const int li = (int)((blk * 256) >> P.logN);
   const uint64_t j = e & (N - 1);
   const uint64_t p = P.primes[li].p;
   const uint64_t t = (j * ginv) & (2 * N - 1);
@@ -766,11 +772,13 @@ void launch_automorphism(const DevParams& P, uint64_t* out, const uint64_t* in, 
 }
 
 // Descriptor form, one entry per limb: out = sigma(in) (+ addsrc when set).
+// Blocks mapped to XCDs as in k_automorphism.
 __global__ __launch_bounds__(256) void k_automorphism_d(DevParams P, const AutoDesc* dd) {
   const int per = P.N / 256;
-  const AutoDesc a = dd[blockIdx.x / per];
+  const int blk = xcd_chunk_id(blockIdx.x, gridDim.x);
+  const AutoDesc a = dd[blk / per];
   const uint64_t N = (uint64_t)P.N;
-  const uint64_t j = (uint64_t)(blockIdx.x % per) * 256 + threadIdx.x;
+  const uint64_t j = (uint64_t)(blk % per) * 256 + threadIdx.x;
   const uint64_t p = P.primes[a.prime].p;
   const uint64_t t = (j * a.ginv) & (2 * N - 1);
   uint64_t v = t < N ? a.in[t] : neg_mod(a.in[t - N], p);

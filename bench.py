@@ -35,13 +35,16 @@ HBM_PEAK_GBS = 8000.0
 # `bench.py --profile` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0; the factors make a
 # 1 GiB device copy read 1 GiB and write 1 GiB); the run is summarised in
 # profiles/moddown_pmc_traffic_n16_k30.md.
-MODDOWN_TRAFFIC_BYTES = {32: 3025937376}
+MODDOWN_TRAFFIC_BYTES = {32: 3025807979}
 # VALU wave-instructions per k_moddown_d launch at 32 pairs (SQ_INSTS_VALU summed
 # over the counter instances of one dispatch, rocprofv3 --pmc of
-# `bench.py --profile`; profiles/hommul_batched_n16_k30_valu.md), and the
-# chip's peak issue rate: one wave-instruction per CU per cycle, 256 CUs at
-# 2.4 GHz.
-MODDOWN_VALU_INSTS = {32: 1.028e9}
+# `bench.py --profile`; profiles/hommul_batched_n16_k30_valu.md), and a
+# reference issue rate: MI355X's vector FP64 FMA peak (78.6 TFLOP/s = 2 flops x
+# 64 lanes x 256 CUs x 2.4 GHz) counted in wave instructions, one per CU per
+# cycle.  64-bit integer multiply-adds issue at about this rate; 32-bit adds
+# and logic issue faster (about 1.6 times), so the fraction is a proxy that a
+# mix of both can exceed.
+MODDOWN_VALU_INSTS = {32: 9.812e8}
 VALU_PEAK_INSTS_S = 256 * 2.4e9
 
 
@@ -514,6 +517,44 @@ def cpu_baseline(args, P, plain_items, evk, level):
     return res
 
 
+def calibration_copy():
+    """1 GiB device-to-device copy (three times): the kernel whose FETCH_SIZE
+    and WRITE_SIZE scale the counter readings to bytes."""
+    import torch
+    a = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return 1 << 30
+
+
+def profile_workload(args, stream):
+    """One workload for a counter or trace profile: `plain` runs 20 plain
+    HomMul+relin+rescale calls at the headline shape (after 3 untimed ones),
+    `c3` the mixed-level N=2^15 batch against its plain calls, `c4` the
+    rotate-and-sum batch against its plain rotations; then the calibration
+    copy."""
+    import torch
+    from aoa_amd import Params
+    torch.manual_seed(2)
+    rng = np.random.default_rng(3)
+    if args.profile_workload == "plain":
+        P = Params(N, K_Q, K_B, PREC)
+        evk = rand_polys(P, 2, K_Q + K_B)
+        fn = plain_fn(P, evk, make_items(P, [K_Q - 1]), {})
+        ms, _ = time_steps(fn, 20, 3, stream)
+        res = {"plain_n16": {"workload": "plain HomMul+relin+rescale, N=2^16, k=30, K=31, level 29",
+                             "ms_per_op": round(ms, 4), "ops_s": round(1000.0 / ms, 2)}}
+    elif args.profile_workload == "c3":
+        res = {"mixed_levels_n15": mixed_n15(args, rng, stream)}
+    else:
+        res = {"sumslots_n16": sumslots_n16(args, stream)}
+    res["calibration_copy_bytes"] = calibration_copy()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -533,6 +574,10 @@ def main():
                     help="headline shape only (plus a 1 GiB copy for counter calibration), for rocprofv3")
     ap.add_argument("--profile-c1", action="store_true",
                     help="the N=2^12 end-to-end pipeline only (no CPU oracle), for rocprofv3")
+    ap.add_argument("--profile-workload", choices=["plain", "c3", "c4"],
+                    help="one other workload only, for rocprofv3 (plus the 1 GiB calibration copy): plain "
+                         "HomMul+relin+rescale calls at the headline shape, the mixed-level N=2^15 batch, or "
+                         "the N=2^16 rotate-and-sum batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every GPU box here has one GPU and this build has no "
@@ -547,6 +592,9 @@ def main():
     if args.profile_c1:
         args.cpu_sample = 0
         print(json.dumps({"c1_pipeline_n12": c1_pipeline(args, stream)}))
+        return
+    if args.profile_workload:
+        print(json.dumps(profile_workload(args, stream)))
         return
     torch.manual_seed(2)
     P = Params(N, K_Q, K_B, PREC)
@@ -617,14 +665,7 @@ def main():
     }
 
     if args.profile:
-        # 1 GiB device-to-device copy: the counter calibration kernel
-        a = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
-        b = torch.empty_like(a)
-        a.fill_(1)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        res["calibration_copy_bytes"] = 1 << 30
+        res["calibration_copy_bytes"] = calibration_copy()
         print(json.dumps(res))
         return
 
