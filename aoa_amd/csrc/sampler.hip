@@ -18,7 +18,11 @@
 //  * Gaussian: e = round(3.2 z), z ~ N(0, 1), as sample_DG (sampler.cu:72-117),
 //    by inversion of a 64-entry table of P(e <= m) scaled to 2^64 (integer
 //    compares only), one value per coefficient shared by all limbs;
-//  * ZO: 0 with probability 1/2, +1 and -1 with 1/4 each, per coefficient;
+//  * ZO: exactly N/2 coefficients +-1, the others 0, as sample_Z0
+//    (sampler.cu:119-150, which shuffles N/2 values +-1 over the N
+//    positions): coefficient i is nonzero when a keyed permutation of
+//    [0, N) -- six Feistel rounds on the index bits, round function Philox --
+//    maps it below N/2, with the sign of its own draw (zo_nonzero);
 //  * narrow: uniform on {-1, 0, 1};
 //  * HW: exactly 64 coefficients +-1 at distinct positions (sample_hw,
 //    sampler.cu:157-192), chosen by one wave in attempt order.
@@ -56,6 +60,29 @@ __device__ __forceinline__ uint64_t hi64(u32x4 c) { return ((uint64_t)c.w << 32)
 
 __device__ __forceinline__ uint64_t small_residue(int64_t v, uint64_t q) { return v >= 0 ? (uint64_t)v : q + v; }
 
+// Whether coefficient i of a ZO draw is nonzero: the index's logN bits split
+// into a high part H of ceil(logN/2) bits and a low part Lo of floor(logN/2)
+// bits; six rounds alternately replace Lo by Lo ^ F_r(H) and H by
+// H ^ F_r(Lo) (the last round changes H, on which the choice depends), F_r
+// the first word of the Philox draw (kind ZO, round tag 0xFFFF00 + r) at the
+// other part, cut to the part's width.  Each round is invertible, so
+// i -> (H, Lo) is a permutation of [0, N) and exactly the N/2 indices whose
+// image has H's top bit clear are nonzero.  (The sign is bit 0
+// of the second word of i's own draw, tag 0xFFFFFF.)
+__device__ __forceinline__ bool zo_nonzero(int logN, uint64_t seed, uint64_t nonce, uint32_t i) {
+  const int hb = (logN + 1) / 2, lb = logN / 2;
+  uint32_t h = i >> lb, lo = i & ((1u << lb) - 1u);
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    const uint32_t tag = ((uint32_t)SAMPLE_ZO << 24) | (0xFFFF00u + (uint32_t)r);
+    if (r & 1)
+      h ^= draw(seed, nonce, tag, lo).x & ((1u << hb) - 1u);
+    else
+      lo ^= draw(seed, nonce, tag, h).x & ((1u << lb) - 1u);
+  }
+  return (h >> (hb - 1)) == 0;
+}
+
 __global__ __launch_bounds__(256) void k_sample_uniform(DevParams P, uint64_t* out, int res0, int nres, uint64_t seed,
                                                         uint64_t nonce) {
   const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -85,8 +112,7 @@ __device__ __forceinline__ void sample_small_at(const DevParams& P, uint64_t* ou
     for (int j = 0; j < 2 * kGaussHalf; ++j) cnt += u >= cdt[j] ? 1 : 0;
     v = cnt - kGaussHalf;
   } else if (kind == SAMPLE_ZO) {
-    const uint32_t b = c.x & 3u;
-    v = b < 2 ? 0 : (b == 2 ? 1 : -1);
+    v = zo_nonzero(P.logN, seed, nonce, i) ? ((c.y & 1u) ? -1 : 1) : 0;
   } else {  // narrow
     v = (int64_t)__umul64hi(lo64(c), 3) - 1;
   }

@@ -27,6 +27,7 @@
 #include <omp.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -803,8 +804,21 @@ static void sample(const Oracle& o, uint64_t* out, int res0, int nres, int kind,
       for (int j = 0; j < 64; ++j) cnt += u >= cdt[j];
       v = cnt - 32;
     } else if (kind == S_ZO) {
-      const uint32_t b = c.v[0] & 3u;
-      v = b < 2 ? 0 : (b == 2 ? 1 : -1);
+      // exactly N/2 nonzeros (sample_Z0, aoa-ntt/src/cuda/sampler.cu:119-150):
+      // six Feistel rounds permute the index, H of ceil(logN/2) bits and Lo
+      // of floor(logN/2); i is nonzero when its image has H's top bit clear
+      int logN = 0;
+      while ((1 << logN) < N) ++logN;
+      const int hb = (logN + 1) / 2, lb = logN / 2;
+      uint32_t h = (uint32_t)i >> lb, lo = (uint32_t)i & ((1u << lb) - 1u);
+      for (uint32_t r = 0; r < 6; ++r) {
+        const uint32_t tag = ((uint32_t)S_ZO << 24) | (0xFFFF00u + r);
+        if (r & 1)
+          h ^= draw(seed, nonce, tag, lo).v[0] & ((1u << hb) - 1u);
+        else
+          lo ^= draw(seed, nonce, tag, h).v[0] & ((1u << lb) - 1u);
+      }
+      v = (h >> (hb - 1)) ? 0 : ((c.v[1] & 1u) ? -1 : 1);
     } else {
       v = (int64_t)(uint64_t)(((unsigned __int128)u * 3) >> 64) - 1;
     }

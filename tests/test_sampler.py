@@ -85,7 +85,60 @@ def test_gaussian_moments_and_pmf(ora):
     assert chi2 < 70, chi2  # 24 degrees of freedom
 
 
-@pytest.mark.parametrize("kind,probs", [(ZO, {0: 0.5, 1: 0.25, -1: 0.25}), (NARROW, {0: 1 / 3, 1: 1 / 3, -1: 1 / 3})])
+def _zo_feistel(seed, nonce, logN, i):
+    """The oracle's ZO permutation restated from Philox4x32-10 outputs: six
+    rounds on (H, Lo), H the high ceil(logN/2) bits of i, Lo the rest."""
+    hb, lb = (logN + 1) // 2, logN // 2
+    h, lo = i >> lb, i & ((1 << lb) - 1)
+    for r in range(6):
+        tag = (ZO << 24) | (0xFFFF00 + r)
+        if r & 1:
+            h ^= orc.philox4x32_10([lo, tag, nonce, 0], [seed, 0])[0] & ((1 << hb) - 1)
+        else:
+            lo ^= orc.philox4x32_10([h, tag, nonce, 0], [seed, 0])[0] & ((1 << lb) - 1)
+    return (h << lb) | lo
+
+
+def test_zo_fixed_weight(ora):
+    """ZO as sample_Z0 draws it (aoa-ntt/src/cuda/sampler.cu:119-150: N/2
+    values +-1 shuffled over the N positions): exactly N/2 nonzeros, the same
+    signed value in every limb, signs balanced, nonzeros spread evenly over
+    the positions."""
+    N, R = ora.N, ora.k + ora.kl
+    for nonce in range(3):
+        out = ora.sample(ZO, 5, nonce).reshape(R, N)
+        v = _signed(out[0], ora.primes[0])
+        assert int((v != 0).sum()) == N // 2
+        assert set(np.unique(v)) == {-1, 0, 1}
+        for l in range(1, R):
+            assert np.array_equal(_signed(out[l], ora.primes[l]), v)
+        nz = v[v != 0]
+        assert abs(nz.mean()) < 5 / math.sqrt(N // 2)
+        # nonzeros per block of 1024 positions: hypergeometric, sd about 16
+        blocks = (v != 0).reshape(-1, 1024).sum(axis=1)
+        assert np.abs(blocks - 512).max() < 6 * 16, blocks
+
+
+@pytest.mark.parametrize("N", [2048, 8192])
+def test_zo_permutation_restated(N):
+    """The positions and signs at N = 2^11 and 2^13 (odd and even index
+    widths) against a Python restatement of the Feistel permutation over
+    Philox outputs, which is a bijection of [0, N)."""
+    o = orc.Oracle(N, 2, 1, 55)
+    seed, nonce = 9, 4
+    logN = N.bit_length() - 1
+    perm = [_zo_feistel(seed, nonce, logN, i) for i in range(N)]
+    assert sorted(perm) == list(range(N))
+    want = np.zeros(N, dtype=np.int64)
+    for i, p in enumerate(perm):
+        if p < N // 2:
+            sign = orc.philox4x32_10([i, (ZO << 24) | 0xFFFFFF, nonce, 0], [seed, 0])[1] & 1
+            want[i] = -1 if sign else 1
+    v = _signed(o.sample(ZO, seed, nonce).reshape(3, N)[0], o.primes[0])
+    assert np.array_equal(v, want)
+
+
+@pytest.mark.parametrize("kind,probs", [(NARROW, {0: 1 / 3, 1: 1 / 3, -1: 1 / 3})])
 def test_ternary_frequencies(ora, kind, probs):
     N, R = ora.N, ora.k + ora.kl
     out = ora.sample(kind, 5, 3).reshape(R, N)
