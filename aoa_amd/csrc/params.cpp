@@ -83,6 +83,11 @@ static std::string select_primes(int k, int kl, int prec, std::vector<aoa_prime_
   typedef unsigned __int128 u128;
   if ((u128)kl * bmax + ((u128)1 << 57) + 2 * (u128)kl > ((u128)1 << 62))
     return "special primes too large for ModDown's lazy sums";
+  // ModUp's lazy sums for b_j hold at most k - 1 products below b_j^2 and
+  // q0's two terms below (2^56 + 2^7) b_j: (k - 1) b_max + 2^56 + 2^7 <= 2^62
+  // keeps them below 2^(62+nb) for reduce_sums_lazy.
+  if ((u128)(k > 1 ? k - 1 : 0) * bmax + ((u128)1 << 56) + 128 > ((u128)1 << 62))
+    return "special primes too large for ModUp's lazy sums";
   // ModDown's rescale forms q_i + add_i - top with top < q_l for i < l, which
   // needs q_l < q_i: the Q primes decrease (q0 first, then the bucket in
   // decreasing order).
@@ -221,19 +226,24 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
   // ModUp's constants (Q_l/q_i) mod b_j split at bit 28, in the order the
   // kernel reads them: one block of kp = K rounded up to 16 entries per (l, i),
   // special limb j = g + 4t (g = j % 4) at position (t / 4) 16 + 4 g + t % 4,
-  // so that the limbs of one wave's pass are consecutive entries.
+  // so that the limbs of one wave's pass are consecutive entries.  Row i = k
+  // holds (2^56 (Q_l/q_0)) mod b_j: the kernel splits q0's value v at bit 56
+  // and multiplies its top bits by it.
   const int kp = (kl + 15) & ~15;
-  P.mu_terms.assign((size_t)k * k * kp * 4, 0);
+  P.mu_terms.assign((size_t)k * (k + 1) * kp * 4, 0);
+  auto put_mu = [&](int l, int row, int j, uint64_t w) {
+    const int t = j / 4, pos = (t / 4) * 16 + 4 * (j % 4) + t % 4;
+    uint32_t* e = &P.mu_terms[(((size_t)l * (k + 1) + row) * kp + pos) * 4];
+    e[0] = (uint32_t)(w & ((1u << 28) - 1));
+    e[1] = (uint32_t)(w >> 28);
+    e[2] = e[0] + e[1];
+  };
   for (int l = 0; l < k; ++l)
-    for (int i = 0; i <= l; ++i)
-      for (int j = 0; j < kl; ++j) {
-        const uint64_t w = P.qhat_mod_b[((size_t)l * kl + j) * k + i];
-        const int t = j / 4, pos = (t / 4) * 16 + 4 * (j % 4) + t % 4;
-        uint32_t* e = &P.mu_terms[(((size_t)l * k + i) * kp + pos) * 4];
-        e[0] = (uint32_t)(w & ((1u << 28) - 1));
-        e[1] = (uint32_t)(w >> 28);
-        e[2] = e[0] + e[1];
-      }
+    for (int j = 0; j < kl; ++j) {
+      for (int i = 0; i <= l; ++i) put_mu(l, i, j, P.qhat_mod_b[((size_t)l * kl + j) * k + i]);
+      const uint64_t w0 = P.qhat_mod_b[((size_t)l * kl + j) * k];
+      put_mu(l, k, j, (uint64_t)(((unsigned __int128)w0 << 56) % P.primes[k + j]));
+    }
   P.inv_bhat.assign((size_t)kl * 2, 0);
   for (int j = 0; j < kl; ++j) {
     uint64_t bj = Bp[j];
@@ -248,7 +258,7 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
   P.md_const.assign((size_t)k * kl, MdConst{});
   P.md_terms.assign((size_t)kl * k, MdTerm{});
   P.md_thr.assign((size_t)k, 0);
-  P.md_red.assign((size_t)k * 2, 0);
+  P.md_red.assign((size_t)(k + kl) * 2, 0);
   P.md_fin.assign((size_t)k * k * 2, 0);
   P.md_negb.assign((size_t)k * 4, 0);
   P.inv_b_mod_q.assign((size_t)k * 2, 0);
@@ -339,6 +349,12 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
     }
     P.b_mod_q[2 * i] = bfull;
     P.b_mod_q[2 * i + 1] = shoup_quot(bfull, qi);
+  }
+  // ModUp reduces its lazy sums per special prime (below 2^56, so nb <= 56)
+  for (int j = 0; j < kl; ++j) {
+    const DevPrime& d = P.dprimes[k + j];
+    P.md_red[2 * (k + j)] = (uint64_t)((((unsigned __int128)1) << (63 + d.nb)) / d.p);
+    P.md_red[2 * (k + j) + 1] = 5 * d.p;
   }
   // Encoder tables: the roots exp(2 pi i t / 2N) and the rotation group
   // 5^j mod 2N (aoa-ntt/src/ckks/encoder.cu:36-48).  The reference's ksiPows

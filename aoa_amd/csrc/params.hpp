@@ -76,15 +76,17 @@ struct DevParams {
   const uint64_t* gauss_cdt;     // [2 kGaussHalf]: round(2^64 P(e <= j - kGaussHalf)), e = round(3.2 z)
   const uint32_t* md_terms;      // [K][k][6]: ModDown product constants (MdTerm), Q primes below 2^56
   const uint32_t* md_thr;        // [k]: ModDown screen threshold of q_i
-  // [k][2]: per Q prime, floor(2^(63+nb) / q_i) (reduce_sums_lazy; 0 for q0)
-  // and the bound of ModDown's lazy sums, 5 q_i (q0: q0, whose sums are exact)
+  // [k+K][2]: per prime, floor(2^(63+nb) / p) (reduce_sums_lazy; 0 for q0)
+  // and the bound of the lazy sums, 5 p (q0: q0, whose sums are exact): Q
+  // primes for ModDown, special primes for ModUp
   const uint64_t* md_red;
   const uint64_t* md_fin;        // [k][k][2]: (l,i) -> B^-1 q_l^-1 mod q_i, Shoup
   // [k][4]: 28-bit halves (low, high, their sum, 0) of (-B) mod q_i, Q primes
   // below 2^57: ModDown adds z (-B) to its lazy sums as one more term
   const uint32_t* md_negb;
-  // [k][k][kp][4], kp = K rounded up to 16: (l, i, position of j) -> 28-bit
-  // halves of (Q_l/q_i) mod b_j and their sum; positions as in build_host_params
+  // [k][k+1][kp][4], kp = K rounded up to 16: (l, i, position of j) -> 28-bit
+  // halves of (Q_l/q_i) mod b_j and their sum, row i = k: of
+  // (2^56 Q_l/q_0) mod b_j; positions as in build_host_params
   const uint32_t* mu_terms;
 };
 
@@ -117,10 +119,10 @@ struct HostParams {
   std::vector<MdConst> md_const;    // k * K entries, [i][j]
   std::vector<MdTerm> md_terms;     // K * k entries, [j][i]
   std::vector<uint32_t> md_thr;     // k entries
-  std::vector<uint64_t> md_red;     // k * 2
+  std::vector<uint64_t> md_red;     // (k + K) * 2
   std::vector<uint64_t> md_fin;     // k * k * 2
   std::vector<uint32_t> md_negb;    // k * 4
-  std::vector<uint32_t> mu_terms;   // k * k * kp * 4
+  std::vector<uint32_t> mu_terms;   // k * (k + 1) * kp * 4
 };
 
 // Builds every table.  Returns an empty string on success, else the reason

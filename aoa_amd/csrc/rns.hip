@@ -202,26 +202,27 @@ constexpr int BC_GROUPS = 4;
 // q_i > b_j (always for q0).  With v, w < 2^56 split at bit 28 and
 // vs = vl + vh, ws = wl + wh: v w = vl wl + (vs ws - vl wl - vh wh) 2^28 +
 // vh wh 2^56, three 32x32->64 multiply-adds per term into 64-bit sums that
-// cannot overflow for up to 64 terms, one 128-bit reduction per limb.  The
-// halves of w come split from the mu_terms table, the NL limbs' entries
-// side by side, so that a term takes one scalar load and no scalar ALU work.
+// cannot overflow for up to 64 terms, one lazy reduction per limb
+// (reduce_sums_lazy; the host checks its bound).  The halves of w come split
+// from the mu_terms table, the NL limbs' entries side by side, so that a term
+// takes one scalar load and no scalar ALU work.
 template <int NL>
 __device__ __forceinline__ void modup_pass(const DevParams& P, const uint64_t* tl, int c, const int* jl, int level,
                                            uint64_t* outB, size_t cid) {
   constexpr uint32_t M28 = (1u << 28) - 1;
-  const int L = level + 1;
+  const int L = level + 1, k = P.k;
   // The pass's limbs jl[n] = jl[0] + 4n are consecutive entries of mu_terms:
   // (level, i, pos0 + n) sits at e0 + i kp 4 + 4n.
   const int kp = (P.K + 15) & ~15;
   const int t0 = jl[0] / BC_GROUPS;
   const int pos0 = (t0 / 4) * 16 + 4 * (jl[0] % BC_GROUPS) + t0 % 4;
-  const cptr<uint32_t> e0 = as_const(P.mu_terms) + ((size_t)level * P.k * kp + pos0) * 4;
+  const cptr<uint32_t> e0 = as_const(P.mu_terms) + ((size_t)level * (k + 1) * kp + pos0) * 4;
   DevPrime d[NL];
   uint32_t red = 0;
   uint64_t s0[NL], s2[NL], sm[NL];
 #pragma unroll
   for (int n = 0; n < NL; ++n) {
-    d[n] = ld_prime(P.primes, P.k + jl[n]);
+    d[n] = ld_prime(P.primes, k + jl[n]);
     red |= d[n].modup_reduce;
     s0[n] = s2[n] = sm[n] = 0;
   }
@@ -231,17 +232,29 @@ __device__ __forceinline__ void modup_pass(const DevParams& P, const uint64_t* t
     s2[n] += (uint64_t)vh * e[1];
     sm[n] += (uint64_t)(vl + vh) * e[2];
   };
-  auto reduced = [&](int i) {
-    const uint64_t v = tl[i * BC_COEF + c];
-    const uint64_t qi = ld_prime_p(P.primes, i);
-    const cptr<uint32_t> e = e0 + (size_t)i * kp * 4;
+  {
+    // q0's value v < 2^63, wider than every special prime: v = lo + hi 2^56,
+    // lo against (Q_l/q_0) mod b_j and hi (7 bits) against 2^56 (Q_l/q_0)
+    // mod b_j (row k), hi taken as the low half of the split with no high half
+    const uint64_t v = tl[c];
+    const uint32_t hi = (uint32_t)(v >> 56);
+    const cptr<uint32_t> eh = e0 + (size_t)k * kp * 4;
 #pragma unroll
-    for (int n = 0; n < NL; ++n)
-      acc(n, qi > d[n].p ? barrett_reduce(0, v, d[n].p, d[n].mu, (int)d[n].nb) : v, e + 4 * n);
-  };
-  reduced(0);  // q0 is wider than every special prime
+    for (int n = 0; n < NL; ++n) {
+      acc(n, v & ((1ull << 56) - 1), e0 + 4 * n);
+      s0[n] += (uint64_t)hi * eh[4 * n];
+      sm[n] += (uint64_t)hi * eh[4 * n + 2];
+    }
+  }
   if (red) {
-    for (int i = 1; i < L; ++i) reduced(i);
+    for (int i = 1; i < L; ++i) {
+      const uint64_t v = tl[i * BC_COEF + c];
+      const uint64_t qi = ld_prime_p(P.primes, i);
+      const cptr<uint32_t> e = e0 + (size_t)i * kp * 4;
+#pragma unroll
+      for (int n = 0; n < NL; ++n)
+        acc(n, qi > d[n].p ? barrett_reduce(0, v, d[n].p, d[n].mu, (int)d[n].nb) : v, e + 4 * n);
+    }
   } else {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 #pragma unroll 2
@@ -259,11 +272,14 @@ __device__ __forceinline__ void modup_pass(const DevParams& P, const uint64_t* t
       }
     }
   }
+  // one lazy reduction per limb (below 5 b_j), then down to [0, b_j)
 #pragma unroll
   for (int n = 0; n < NL; ++n) {
-    uint64_t h, l;
-    join_28(s0[n], sm[n] - s0[n] - s2[n], s2[n], &h, &l);
-    outB[(size_t)jl[n] * P.N + cid] = reduce128(h, l, d[n]);
+    const uint64_t b = d[n].p;
+    uint64_t r = reduce_sums_lazy(s0[n], sm[n] - s0[n] - s2[n], s2[n], b, as_const(P.md_red)[2 * (k + jl[n])],
+                                  (int)d[n].nb);
+    for (uint64_t m = 4 * b; m >= b; m >>= 1) r = r >= m ? r - m : r;
+    outB[(size_t)jl[n] * P.N + cid] = r;
   }
 }
 

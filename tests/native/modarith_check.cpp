@@ -352,6 +352,73 @@ int64_t chk_moddown_finish(int N, int k, int kl, int prec, int64_t trials, uint6
   return bad;
 }
 
+// ModUp's tables and sums against 128-bit arithmetic, for every level l and
+// special prime b_j: the mu_terms entries (the 28-bit halves of
+// (Q_l/q_i) mod b_j at the pass position of j, row k: of 2^56 (Q_l/q_0) mod
+// b_j) and md_red of the special primes; then the kernel's sums -- q0's value
+// split at bit 56, the other terms at bit 28 -- reduced by reduce_sums_lazy
+// and brought below b_j with three conditional subtractions, against
+// sum_i v_i (Q_l/q_i mod b_j) mod b_j, with v_i = q_i - 1 (the largest values)
+// in trial 0 and random residues of q_i otherwise.  Returns the number of
+// failures.
+int64_t chk_modup_sums(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
+  HostParams H;
+  if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  kl = H.K;  // kl = 0 selects k + 1
+  std::mt19937_64 rng(seed);
+  typedef unsigned __int128 u128;
+  const uint32_t M28 = (1u << 28) - 1;
+  const int kp = (kl + 15) & ~15;
+  int64_t bad = 0;
+  auto entry = [&](int l, int row, int j) {
+    const int t = j / 4, pos = (t / 4) * 16 + 4 * (j % 4) + t % 4;
+    return &H.mu_terms[(((size_t)l * (k + 1) + row) * kp + pos) * 4];
+  };
+  for (int j = 0; j < kl; ++j) {
+    const DevPrime& d = H.dprimes[k + j];
+    const uint64_t b = d.p;
+    if (H.md_red[2 * (k + j) + 1] != 5 * b) ++bad;
+    if (H.md_red[2 * (k + j)] != (uint64_t)((((u128)1) << (63 + d.nb)) / b)) ++bad;
+    for (int l = 0; l < k; ++l) {
+      for (int i = 0; i <= l; ++i) {
+        const uint64_t w = H.qhat_mod_b[((size_t)l * kl + j) * k + i];
+        const uint32_t* e = entry(l, i, j);
+        if (e[0] != (uint32_t)(w & M28) || e[1] != (uint32_t)(w >> 28) || e[2] != e[0] + e[1]) ++bad;
+      }
+      const uint64_t w0 = H.qhat_mod_b[((size_t)l * kl + j) * k];
+      const uint64_t wh = (uint64_t)(((u128)w0 << 56) % b);
+      const uint32_t* eh = entry(l, k, j);
+      if (eh[0] != (uint32_t)(wh & M28) || eh[1] != (uint32_t)(wh >> 28) || eh[2] != eh[0] + eh[1]) ++bad;
+      for (int64_t t = 0; t < trials; ++t) {
+        uint64_t s0 = 0, s2 = 0, sm = 0;
+        u128 want = 0;
+        for (int i = 0; i <= l; ++i) {
+          const uint64_t qi = H.primes[i];
+          const uint64_t v = t == 0 ? qi - 1 : rng() % qi;
+          want = (want + (u128)(v % b) * H.qhat_mod_b[((size_t)l * kl + j) * k + i]) % b;
+          const uint32_t* e = entry(l, i, j);
+          // the kernel reduces v mod b_j first only where q_i > b_j (i >= 1)
+          uint64_t vv = (i > 0 && qi > b) ? v % b : v;
+          if (i == 0) {
+            const uint32_t hi = (uint32_t)(v >> 56);
+            vv = v & ((((uint64_t)1) << 56) - 1);
+            s0 += (uint64_t)hi * eh[0];
+            sm += (uint64_t)hi * eh[2];
+          }
+          const uint32_t vl = (uint32_t)vv & M28, vh = (uint32_t)(vv >> 28);
+          s0 += (uint64_t)vl * e[0];
+          s2 += (uint64_t)vh * e[1];
+          sm += (uint64_t)(vl + vh) * e[2];
+        }
+        uint64_t r = reduce_sums_lazy(s0, sm - s0 - s2, s2, b, H.md_red[2 * (k + j)], (int)d.nb);
+        for (uint64_t m = 4 * b; m >= b; m >>= 1) r = r >= m ? r - m : r;
+        if ((u128)r != want) ++bad;
+      }
+    }
+  }
+  return bad;
+}
+
 // The product's host tables (same numbering as aoa_params_table), built
 // without a GPU.  Returns the number of words (out may be null).
 int64_t chk_table(int N, int k, int kl, int prec, int which, uint64_t* out) {

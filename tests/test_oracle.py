@@ -296,6 +296,17 @@ def test_moddown_lazy_finish():
         assert chk.chk_moddown_finish(*shape, 200, 19) == 0, shape
 
 
+def test_modup_tables_and_lazy_sums():
+    """ModUp's pass-ordered constants (row k: 2^56 (Q_l/q_0) mod b_j for q0's top
+    bits), the special primes' reduction constants, and the kernel's sums and
+    reduction against the textbook basis conversion in 128-bit arithmetic, at
+    the largest values and at random ones, on every level."""
+    chk = native_chk.load()
+    for shape in [(65536, 30, 31, 48), (4096, 4, 5, 52), (16384, 5, 5, 55), (4096, 10, 11, 45),
+                  (8192, 20, 64, 48)]:
+        assert chk.chk_modup_sums(*shape, 4, 23) == 0, shape
+
+
 def test_product_exact_mults():
     chk = native_chk.load()
     for shape in [(65536, 30, 31, 48), (4096, 4, 5, 52), (4096, 5, 5, 55)]:
