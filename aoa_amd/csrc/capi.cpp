@@ -59,7 +59,7 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
     size_t bytes;
     size_t off;
   };
-  Sec secs[] = {
+  std::vector<Sec> secs = {
       {H.dprimes.data(), H.dprimes.size() * sizeof(DevPrime), 0},
       {H.tw_fwd.data(), H.tw_fwd.size() * 8, 0},
       {H.tw_inv.data(), H.tw_inv.size() * 8, 0},
@@ -80,7 +80,17 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
       {H.md_negb.data(), H.md_negb.size() * 4, 0},
       {H.mu_terms.data(), H.mu_terms.size() * 4, 0},
   };
-  const int nsec = sizeof(secs) / sizeof(secs[0]);
+  // Row-pass twiddle orders, one per two-pass split of N and direction
+  // (device only: the host keeps the natural tables).
+  std::vector<uint64_t> rowtw[3][2];
+  int rowsec[3][2];
+  for (int si = 0; si < 3; ++si)
+    for (int dir = 0; dir < 2; ++dir) {
+      row_twiddle_order(dir ? H.tw_inv : H.tw_fwd, H.N, H.k + H.K, 64 << si, &rowtw[si][dir]);
+      rowsec[si][dir] = rowtw[si][dir].empty() ? -1 : (int)secs.size();
+      if (!rowtw[si][dir].empty()) secs.push_back({rowtw[si][dir].data(), rowtw[si][dir].size() * 8, 0});
+    }
+  const int nsec = (int)secs.size();
   size_t total = 0;
   for (int i = 0; i < nsec; ++i) {
     secs[i].off = total;
@@ -123,6 +133,9 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
   D.md_fin = (const uint64_t*)(b + secs[16].off);
   D.md_negb = (const uint32_t*)(b + secs[17].off);
   D.mu_terms = (const uint32_t*)(b + secs[18].off);
+  for (int si = 0; si < 3; ++si)
+    for (int dir = 0; dir < 2; ++dir)
+      D.tw_row[si][dir] = rowsec[si][dir] < 0 ? nullptr : (const uint64_t*)(b + secs[rowsec[si][dir]].off);
   *out = h;
   return AOA_OK;
 }

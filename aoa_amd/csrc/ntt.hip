@@ -137,6 +137,8 @@ __device__ __forceinline__ void fwd_row_body(const DevParams& P, const uint64_t*
   const int row0 = tile * C;
   const uint64_t p = P.primes[pr].p;
   const uint64_t* T = P.tw_fwd + (size_t)pr * P.N * 2;
+  // the second round's stages: entries in row order (row_twiddle_order)
+  const uint64_t* TR = P.tw_row[LOGS - 6][0] + (size_t)pr * P.N * 2;
   const int tid = threadIdx.x;
   const int rl = tid / TS;
   const int sl = tid % TS;
@@ -162,8 +164,9 @@ __device__ __forceinline__ void fwd_row_body(const DevParams& P, const uint64_t*
   for (int k = 0; k < 16; ++k) L[swz(sl + TS * k)] = x[k];
   const int sh = sl;
   // The next round's twiddle is requested before the barrier, so its latency
-  // overlaps the exchange.
-  const tw2 t4 = load_tw(T, ((size_t)1 << (g0 + 4)) + ((size_t)row << 4) + ((size_t)sh << (4 - (LOGS - 4))));
+  // overlaps the exchange.  In this round thread sh's m-th entry of a stage
+  // sits at m TS + sh of the row's block.
+  const tw2 t4 = load_tw(TR, ((size_t)1 << (g0 + 4)) + ((size_t)row << 4) + sh);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < 16; ++k) x[k] = L[swz(16 * sh + k)];
@@ -175,19 +178,19 @@ __device__ __forceinline__ void fwd_row_body(const DevParams& P, const uint64_t*
       if (LOGS == 8) {
         ct_bfly<LZ>(x[k], x[k + half], t4, p);
       } else {
-        const size_t base = ((size_t)1 << (g0 + 4)) + ((size_t)row << 4) + ((size_t)sh << (4 - (LOGS - 4)));
-        ct_bfly<LZ>(x[k], x[k + half], load_tw(T, base + (k >> (LOGS - 4))), p);
+        const size_t base = ((size_t)1 << (g0 + 4)) + ((size_t)row << 4) + sh;
+        ct_bfly<LZ>(x[k], x[k + half], load_tw(TR, base + (size_t)(k >> (LOGS - 4)) * TS), p);
       }
     }
   }
 #pragma unroll
   for (int r = 5; r < LOGS; ++r) {
     const int half = 1 << (LOGS - 1 - r);
-    const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r) + ((size_t)sh << (r - (LOGS - 4)));
+    const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r) + sh;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       if (k & half) continue;
-      const tw2 t = load_tw(T, base + (k >> (LOGS - r)));
+      const tw2 t = load_tw(TR, base + (size_t)(k >> (LOGS - r)) * TS);
       ct_bfly<LZ>(x[k], x[k + half], t, p);
     }
   }
@@ -207,9 +210,13 @@ __device__ __forceinline__ void fwd_row_body(const DevParams& P, const uint64_t*
 // INTT(x * key_b), INTT(x * key_a) from one input).
 // The inverse row pass after the input limb is in registers (x[k] = element
 // sl + TS*k of the thread's row): stages g0+log2(S)-1 .. g0, result to dst.
+// T is the prime's inverse table, TR its row-ordered copy (row_twiddle_order)
+// for the first round, in which thread sh's m-th entry of a stage sits at
+// m TS + sh of the row's block.
 template <int S, bool LZ>
 __device__ __forceinline__ void inv_row_core(const DevParams& P, uint64_t (&x)[16], uint64_t* dst, uint64_t p,
-                                             const uint64_t* T, int row, int sl, int g0, uint64_t* L) {
+                                             const uint64_t* T, const uint64_t* TR, int row, int sl, int g0,
+                                             uint64_t* L) {
   using G = Geo<S>;
   constexpr int TS = G::TS, LOGS = G::LOGS;
 #pragma unroll
@@ -219,9 +226,9 @@ __device__ __forceinline__ void inv_row_core(const DevParams& P, uint64_t (&x)[1
   // their latency overlaps the exchange.
   tw2 t0[8];
   {
-    const size_t base = ((size_t)1 << (g0 + LOGS - 1)) + ((size_t)row << (LOGS - 1)) + ((size_t)sh << 3);
+    const size_t base = ((size_t)1 << (g0 + LOGS - 1)) + ((size_t)row << (LOGS - 1)) + sh;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) t0[q] = load_tw(T, base + q);
+    for (int q = 0; q < 8; ++q) t0[q] = load_tw(TR, base + (size_t)q * TS);
   }
   __syncthreads();
 #pragma unroll
@@ -231,11 +238,11 @@ __device__ __forceinline__ void inv_row_core(const DevParams& P, uint64_t (&x)[1
 #pragma unroll
   for (int r = LOGS - 2; r >= 4; --r) {
     const int half = 1 << (LOGS - 1 - r);
-    const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r) + ((size_t)sh << (r - (LOGS - 4)));
+    const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r) + sh;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       if (k & half) continue;
-      const tw2 t = load_tw(T, base + (k >> (LOGS - r)));
+      const tw2 t = load_tw(TR, base + (size_t)(k >> (LOGS - r)) * TS);
       gs_bfly<LZ>(x[k], x[k + half], t, p);
     }
   }
@@ -299,6 +306,7 @@ __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t*
   constexpr int C = G::C, TS = G::TS, PS = S + S / 16;
   const DevPrime dp = P.primes[pr];
   const uint64_t* T = P.tw_inv + (size_t)pr * P.N * 2;
+  const uint64_t* TR = P.tw_row[G::LOGS - 6][1] + (size_t)pr * P.N * 2;
   const int rl = threadIdx.x / TS, sl = threadIdx.x % TS;
   const int row = tile * C + rl;
   src += (size_t)row * S;
@@ -311,7 +319,7 @@ __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t*
     for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(x[k], kp[sl + TS * k], dp);
     if (add) add_bscaled<LZ>(x, add + (size_t)row * S + sl, TS, P, pr, dp.p);
   }
-  inv_row_core<S, LZ>(P, x, dst + (size_t)row * S, dp.p, T, row, sl, g0, lds + rl * PS);
+  inv_row_core<S, LZ>(P, x, dst + (size_t)row * S, dp.p, T, TR, row, sl, g0, lds + rl * PS);
 }
 
 // Key switching's two products from one input limb: dst0 = first inverse
@@ -329,6 +337,7 @@ __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint
   const DevPrime dp = P.primes[pr];
   const uint64_t p = dp.p;
   const uint64_t* T = P.tw_inv + (size_t)pr * P.N * 2;
+  const uint64_t* TR = P.tw_row[G::LOGS - 6][1] + (size_t)pr * P.N * 2;
   const int rl = threadIdx.x / TS, sl = threadIdx.x % TS;
   const int row = tile * C + rl;
   const size_t off = (size_t)row * S;
@@ -342,7 +351,7 @@ __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint
     for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(src[off + sl + TS * k], kp[sl + TS * k], dp);
     if (ap) add_bscaled<LZ>(x, ap + off + sl, TS, P, pr, p);  // set for Q limbs only
     if (y && ysel < 0) __syncthreads();  // the row's LDS block is reused
-    inv_row_core<S, LZ>(P, x, (y ? dst1 : dst0) + off, p, T, row, sl, g0, lds + rl * PS);
+    inv_row_core<S, LZ>(P, x, (y ? dst1 : dst0) + off, p, T, TR, row, sl, g0, lds + rl * PS);
   }
 }
 
@@ -361,6 +370,7 @@ __device__ __forceinline__ void inv_row_tensor_body(const DevParams& P, const ui
   const DevPrime dp = P.primes[pr];
   const uint64_t p = dp.p;
   const uint64_t* T = P.tw_inv + (size_t)pr * P.N * 2;
+  const uint64_t* TR = P.tw_row[G::LOGS - 6][1] + (size_t)pr * P.N * 2;
   const int rl = threadIdx.x / TS, sl = threadIdx.x % TS;
   const int row = tile * C + rl;
   const size_t off = (size_t)row * S;
@@ -380,7 +390,7 @@ __device__ __forceinline__ void inv_row_tensor_body(const DevParams& P, const ui
     xd[i] = c2;
     x[k] = c2;
   }
-  inv_row_core<S, LZ>(P, x, dst + off, p, T, row, sl, g0, lds + rl * PS);
+  inv_row_core<S, LZ>(P, x, dst + off, p, T, TR, row, sl, g0, lds + rl * PS);
 }
 
 // Column pass of the inverse: global stages log2(S)-1 .. 0, N^-1 folded into

@@ -124,6 +124,34 @@ uint32_t moddown_screen_thr(uint64_t thr, uint64_t p) {
   return t > 0xFFFFFFFFu ? 0xFFFFFFFFu : (uint32_t)t;
 }
 
+void row_twiddle_order(const std::vector<uint64_t>& tw, int N, int R, int S, std::vector<uint64_t>* out) {
+  out->clear();
+  const int N1 = S > 0 ? N / S : 0;
+  auto ok = [](int v) { return v == 64 || v == 128 || v == 256; };
+  if (!ok(S) || !ok(N1) || N1 * S != N || tw.size() != (size_t)R * N * 2) return;
+  int g0 = 0, logs = 0;
+  while ((1 << g0) < N1) ++g0;
+  while ((1 << logs) < S) ++logs;
+  const int ts = S / 16;
+  out->assign(tw.begin(), tw.end());
+  for (int pr = 0; pr < R; ++pr) {
+    const uint64_t* src = &tw[(size_t)pr * N * 2];
+    uint64_t* dst = &(*out)[(size_t)pr * N * 2];
+    for (int r = 4; r < logs; ++r) {
+      const int blk = 1 << r, c = blk / ts;
+      for (int b = 0; b < N1; ++b) {
+        const size_t base = ((size_t)1 << (g0 + r)) + (size_t)b * blk;
+        for (int sh = 0; sh < ts; ++sh)
+          for (int m = 0; m < c; ++m) {
+            const size_t from = base + (size_t)sh * c + m, to = base + (size_t)m * ts + sh;
+            dst[2 * to] = src[2 * from];
+            dst[2 * to + 1] = src[2 * from + 1];
+          }
+      }
+    }
+  }
+}
+
 std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
   if (N < 2048 || N > 65536 || (N & (N - 1)) != 0) return "NTT is not working for this N";
   if (kl <= 0) kl = k + 1;

@@ -64,6 +64,12 @@ struct DevParams {
   const DevPrime* primes;        // [k+K]: Q primes then B primes
   const uint64_t* tw_fwd;        // [k+K][N][2]: psi^brev(i) and Shoup quotient
   const uint64_t* tw_inv;        // [k+K][N][2]: psi^-brev(i) and Shoup quotient
+  // [log2(S) - 6][0: forward, 1: inverse]: for the row pass of the split
+  // N = (N/S) x S, the twiddle table with the entries of the stages done on
+  // 16 consecutive values per thread reordered (row_twiddle_order), so that
+  // the S/16 threads of a row load adjacent entries; null where N/S is not a
+  // valid split
+  const uint64_t* tw_row[3][2];
   const uint64_t* inv_qhat;      // [k][k][2]: (l,i) -> (Q_l/q_i)^-1 mod q_i, Shoup
   const uint64_t* qhat_mod_b;    // [k][K][k]: (l,j,i) -> (Q_l/q_i) mod b_j
   const uint64_t* inv_bhat;      // [K][2]: (B/b_j)^-1 mod b_j, Shoup
@@ -136,6 +142,16 @@ std::string select_rns_primes(int k, int kl, int prec, std::vector<uint64_t>* pr
 uint64_t inv_mod(uint64_t a, uint64_t p);
 
 uint32_t bit_reverse(uint32_t x, int bits);
+
+// The row pass of an N = N1 x S split (g0 = log2 N1, S = 2^LOGS, TS = S/16
+// threads per row) does global stages g0 + r, 4 <= r < LOGS, with each thread
+// holding 16 consecutive values of its row: thread sh of row b uses the
+// entries 2^(g0+r) + b 2^r + sh c + m, m < c = 2^r / TS.  The reordered table
+// keeps entry sh c + m of each such block at m TS + sh, so that for each m
+// the TS threads of a row read TS adjacent entries.  Other stages keep their
+// order.  `tw` is one direction's [R][N][2] table; empty output when N/S is
+// not a two-pass split (64, 128, 256 each way).
+void row_twiddle_order(const std::vector<uint64_t>& tw, int N, int R, int S, std::vector<uint64_t>* out);
 
 // The 32-bit words (w0, w1, w2) of floor(c 2^96 / p), c < p < 2^63
 // (moddown_screen's constant).

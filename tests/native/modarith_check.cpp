@@ -421,6 +421,58 @@ int64_t chk_modup_sums(int N, int k, int kl, int prec, int64_t trials, uint64_t 
 
 // The product's host tables (same numbering as aoa_params_table), built
 // without a GPU.  Returns the number of words (out may be null).
+// The row-ordered twiddle tables against the index each row-pass thread
+// uses: for every two-pass split N = N1 x S, direction, prime, row, stage
+// 4 <= r < log2 S of the row pass, thread sh < S/16 and m < 2^r / (S/16), the
+// kernel's entry 2^(g0+r) + row 2^r + m (S/16) + sh of the reordered table
+// must be the natural table's entry for thread sh's m-th butterfly, which
+// holds 16 consecutive values: 2^(g0+r) + row 2^r + sh 2^r/(S/16) + m.  Every
+// other entry is unchanged.  Returns the number of entries compared, or -1
+// on the first mismatch, -2 when a split that must have a table has none.
+int64_t chk_row_twiddles(int N, int k, int kl, int prec) {
+  HostParams H;
+  if (!build_host_params(N, k, kl, prec, &H).empty()) return -3;
+  const int R = H.k + H.K;
+  int64_t n = 0;
+  for (int S = 64; S <= 256; S *= 2) {
+    const int N1 = N / S;
+    const bool valid = N1 * S == N && (N1 == 64 || N1 == 128 || N1 == 256);
+    int g0 = 0, logs = 0;
+    while ((1 << g0) < N1) ++g0;
+    while ((1 << logs) < S) ++logs;
+    const int ts = S / 16;
+    for (int dir = 0; dir < 2; ++dir) {
+      const std::vector<uint64_t>& T = dir ? H.tw_inv : H.tw_fwd;
+      std::vector<uint64_t> TR;
+      row_twiddle_order(T, N, R, S, &TR);
+      if (!valid) {
+        if (!TR.empty()) return -2;
+        continue;
+      }
+      if (TR.size() != T.size()) return -2;
+      std::vector<char> seen((size_t)N, 0);
+      for (int pr = 0; pr < R; ++pr) {
+        const uint64_t* t = &T[(size_t)pr * N * 2];
+        const uint64_t* tr = &TR[(size_t)pr * N * 2];
+        std::fill(seen.begin(), seen.end(), 0);
+        for (int r = 4; r < logs; ++r)
+          for (int row = 0; row < N1; ++row)
+            for (int sh = 0; sh < ts; ++sh)
+              for (int m = 0; m < (1 << r) / ts; ++m) {
+                const size_t base = ((size_t)1 << (g0 + r)) + ((size_t)row << r);
+                const size_t kern = base + (size_t)m * ts + sh, nat = base + (size_t)sh * ((1 << r) / ts) + m;
+                if (tr[2 * kern] != t[2 * nat] || tr[2 * kern + 1] != t[2 * nat + 1]) return -1;
+                seen[kern] = 1;
+                ++n;
+              }
+        for (int i = 0; i < N; ++i)
+          if (!seen[i] && (tr[2 * i] != t[2 * i] || tr[2 * i + 1] != t[2 * i + 1])) return -1;
+      }
+    }
+  }
+  return n;
+}
+
 int64_t chk_table(int N, int k, int kl, int prec, int which, uint64_t* out) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;

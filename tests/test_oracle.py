@@ -307,6 +307,17 @@ def test_modup_tables_and_lazy_sums():
         assert chk.chk_modup_sums(*shape, 4, 23) == 0, shape
 
 
+def test_row_twiddle_order():
+    """The row-ordered twiddle tables of every two-pass split (N = 2^12 ..
+    2^16): each entry a row-pass thread reads is the natural table's entry for
+    its butterfly; the other entries are unchanged; splits that do not exist
+    get no table."""
+    chk = native_chk.load()
+    for shape in [(4096, 2, 2, 55), (8192, 3, 3, 48), (16384, 3, 0, 48), (32768, 2, 2, 48), (65536, 3, 3, 48)]:
+        n = chk.chk_row_twiddles(*shape)
+        assert n > 0, (shape, n)
+
+
 def test_product_exact_mults():
     chk = native_chk.load()
     for shape in [(65536, 30, 31, 48), (4096, 4, 5, 52), (4096, 5, 5, 55)]:
