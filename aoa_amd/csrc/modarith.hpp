@@ -102,6 +102,18 @@ AOA_HD uint64_t mul_barrett(uint64_t a, uint64_t b, uint64_t p, uint64_t mu, int
   return barrett_reduce(mulhi64(a, b), a * b, p, mu, nb);
 }
 
+// (s0 + s1 2^28 + s2 2^56 + s3 2^84) mod q for a total below q 2^nb: the sums
+// of wide_accumulate and wide_correct (s3 < 2^44), by barrett_reduce of the
+// 128-bit total.
+AOA_HD uint64_t wide_reduce(const uint64_t s[4], uint64_t q, uint64_t mu, int nb) {
+  const uint64_t t1 = s[1] << 28, t2 = s[2] << 56;
+  uint64_t lo = s[0] + t1;
+  uint64_t hi = (s[1] >> 36) + (s[2] >> 8) + (s[3] << 20) + (lo < t1 ? 1 : 0);
+  lo += t2;
+  hi += lo < t2 ? 1 : 0;
+  return barrett_reduce(hi, lo, q, mu, nb);
+}
+
 // T = s0 + s1 2^28 + s2 2^56 mod q, left up to four multiples of q too large
 // (in [0, 5q)), for T < 2^(62+nb) with 33 <= nb <= 56 the bit length of q and
 // mu = floor(2^(63+nb) / q) (below 2^64).  The quotient estimate is the high
@@ -118,6 +130,29 @@ AOA_HD uint64_t reduce_sums_lazy(uint64_t s0, uint64_t s1, uint64_t s2, uint64_t
   const uint64_t y = (uint64_t)t1 * m0 + (uint32_t)x;
   const uint64_t qh = (uint64_t)t1 * m1 + (x >> 32) + (y >> 32);
   return s0 + (s1 << 28) + (s2 << 56) - qh * q;
+}
+
+// Exact sums of products a c for a prime q wider than 2^56 (ModDown's q0):
+// a < 2^56 split at bit 28 (al, ah), c < 2^63 split at bits 28 and 56
+// (c0, c1 < 2^28, c2 < 2^7):
+// a c = al c0 + (al c1 + ah c0) 2^28 + (al c2 + ah c1) 2^56 + ah c2 2^84,
+// six 32x32->64 multiply-adds into s[0..3]; for up to 64 terms no sum
+// reaches 2^63.
+AOA_HD void wide_accumulate(uint64_t s[4], uint64_t a, uint32_t c0, uint32_t c1, uint32_t c2) {
+  const uint32_t al = (uint32_t)a & ((1u << 28) - 1), ah = (uint32_t)(a >> 28);
+  s[0] += (uint64_t)al * c0;
+  s[1] += (uint64_t)al * c1;
+  s[1] += (uint64_t)ah * c0;
+  s[2] += (uint64_t)al * c2;
+  s[2] += (uint64_t)ah * c1;
+  s[3] += (uint64_t)ah * c2;
+}
+
+// Adds d < 2^64 to the sums' total, in 28-bit pieces.
+AOA_HD void wide_correct(uint64_t s[4], uint64_t d) {
+  s[0] += d & ((1u << 28) - 1);
+  s[1] += (d >> 28) & ((1u << 28) - 1);
+  s[2] += d >> 56;
 }
 
 // Full 128-bit product from four 32x32->64 multiply-adds.

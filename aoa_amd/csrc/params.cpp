@@ -88,6 +88,15 @@ static std::string select_primes(int k, int kl, int prec, std::vector<aoa_prime_
   // keeps them below 2^(62+nb) for reduce_sums_lazy.
   if ((u128)(k > 1 ? k - 1 : 0) * bmax + ((u128)1 << 56) + 128 > ((u128)1 << 62))
     return "special primes too large for ModUp's lazy sums";
+  // ModDown's exact sums for a Q prime wider than 2^56 (q0) total below
+  // q (K b_max + 2K) (K products and K screen corrections below 2q); one
+  // Barrett reduction needs that below q 2^nb.
+  for (const aoa_prime_entry& e : *q) {
+    int nb = 0;
+    while (nb < 64 && (e.p >> nb)) ++nb;
+    if (nb > 56 && (u128)kl * bmax + 2 * (u128)kl > ((u128)1 << nb))
+      return "special primes too large for ModDown's sums of a wide Q prime";
+  }
   // ModDown's rescale forms q_i + add_i - top with top < q_l for i < l, which
   // needs q_l < q_i: the Q primes decrease (q0 first, then the bucket in
   // decreasing order).
@@ -330,13 +339,24 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
       d.quirk_lo = 0;
       d.quirk_thr = ~(uint64_t)0;
     }
-    // Karatsuba split and screen words of every c, for Q primes below 2^56
-    // (the 63-bit q0 reduces each product instead).  One screen threshold per
-    // prime: a bound per (i, j) would pass fewer products but costs the
-    // kernel's inner loop one more scalar operand per product.
-    if (d.nb <= 56) {
-      const uint64_t wthr = d.quirk_thr < qi ? d.quirk_thr : qi;
-      const uint32_t sthr = moddown_screen_thr(wthr, qi);
+    // Karatsuba split and screen words of every c, for Q primes below 2^56.
+    // One screen threshold per prime: a bound per (i, j) would pass fewer
+    // products but costs the kernel's inner loop one more scalar operand per
+    // product.  A wider prime (the 63-bit q0) gets c in three pieces, at bits
+    // 28 and 56, and the screen words in the device table only.
+    const uint64_t wthr = d.quirk_thr < qi ? d.quirk_thr : qi;
+    const uint32_t sthr = moddown_screen_thr(wthr, qi);
+    if (d.nb > 56) {
+      for (int j = 0; j < kl; ++j) {
+        const uint64_t c = P.bhat_mod_q[((size_t)i * kl + j) * 2];
+        uint32_t w[3];
+        moddown_screen_words(c, qi, w);
+        P.md_terms[(size_t)j * k + i] = MdTerm{(uint32_t)(c & ((1u << 28) - 1)),
+                                               (uint32_t)((c >> 28) & ((1u << 28) - 1)), (uint32_t)(c >> 56), w[0],
+                                               w[1], w[2]};
+      }
+      P.md_thr[i] = sthr;
+    } else {
       for (int j = 0; j < kl; ++j) {
         const uint64_t c = P.bhat_mod_q[((size_t)i * kl + j) * 2];
         MdConst& m = P.md_const[(size_t)i * kl + j];

@@ -349,14 +349,10 @@ void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int ma
 // With A = a*c - qhat*p = delta*p + r (r the exact remainder, delta the
 // Barrett quotient error, 0 <= delta*eps < 2^nb, eps = 2^nb - p), its result
 // is r - eps when eps <= r < delta*eps and r otherwise; delta_max (from the
-// largest product and 2^(2nb) mod p, params.cpp) bounds delta.  For the
-// 63-bit q0, moddown_term evaluates the literal formula only when r falls in
-// [eps, delta_max*eps); primes below 2^57 take the exact sums of
-// moddown_conv, corrected term by term where the screen flags a product.
-// exact != 0 selects the exact remainder for every term (an empty window).
-__device__ __forceinline__ uint64_t moddown_term(uint64_t a, const tw2 w, const DevPrime& dq, int exact) {
-  return ref_mulmod_shoup(a, w.w, w.q, dq.p, dq.mu, (int)dq.nb, exact ? 0 : dq.quirk_lo, exact ? 0 : dq.quirk_thr);
-}
+// largest product and 2^(2nb) mod p, params.cpp) bounds delta.  Every limb
+// takes exact sums of its products (moddown_conv for primes below 2^57,
+// moddown_wide for the 63-bit q0), corrected term by term where the screen
+// flags a product; moddown_exact skips the screen and its corrections.
 
 // Sums over j in [j0, j1) of the ModDown products for the NL limbs
 // i0, i0 + 4, ... (primes below 2^57), reduced mod q_i up to four multiples
@@ -461,6 +457,44 @@ __device__ __forceinline__ void moddown_convs(const DevParams& P, const uint64_t
     moddown_conv<NL, true>(P, aux, c, i0, j0, j1, res, fold);
 }
 
+// Sum over j in [j0, j1) of the ModDown products of limb i of a prime wider
+// than 2^56 (the 63-bit q0), as the reference reduces them, canonical.  With
+// c = (B/b_j) mod q_i split at bits 28 and 56 (c0, c1 < 2^28, c2 < 2^7; the
+// md_terms entry holds c0, c1, c2 and the screen words) and a < 2^56 at bit
+// 28: six 32x32->64 multiply-adds per term into four 64-bit sums
+// (wide_accumulate).  SCREEN as in moddown_conv: the lanes whose product
+// passes add ref(a c) - (a c mod q_i) + q_i (below 2^64) in 28-bit pieces
+// (wide_correct).  The total, below q_i (K b_max + 2K) <= q_i 2^nb (the host
+// checks it), is reduced once with Barrett (wide_reduce).
+template <bool SCREEN>
+__device__ __forceinline__ uint64_t moddown_wide(const DevParams& P, const uint64_t* aux, int c, int i, int j0,
+                                                 int j1) {
+  constexpr int TW = (int)(sizeof(MdTerm) / 4);
+  const int k = P.k, K = P.K;
+  const cptr<uint32_t> terms = as_const(P.md_terms) + (size_t)i * TW;
+  const uint32_t thr = as_const(P.md_thr)[i];
+  uint64_t bias = 2ull << 32;
+  asm volatile("" : "+v"(bias));
+  uint64_t s[4] = {0, 0, 0, 0};
+  for (int j = j0; j < j1; ++j) {
+    const uint64_t a = aux[j * BC_COEF + c];
+    const cptr<uint32_t> e = terms + (size_t)j * k * TW;
+    wide_accumulate(s, a, e[0], e[1], e[2]);
+    if (SCREEN) {
+      const bool h = moddown_screen((uint32_t)a, (uint32_t)(a >> 32), e[3], e[4], e[5], bias) <= thr;
+      if (__builtin_amdgcn_ballot_w64(h) && h) {
+        const DevPrime dq = ld_prime(P.primes, i);
+        const tw2 w = ld_tw2(reinterpret_cast<const tw2*>(P.bhat_mod_q) + (size_t)i * K, j);
+        const uint64_t x = mul_shoup(a, w.w, w.q, dq.p);
+        const uint64_t f = ref_barrett(mulhi64(a, w.w), a * w.w, dq.p, dq.mu, (int)dq.nb);
+        wide_correct(s, f + dq.p - x);
+      }
+    }
+  }
+  const DevPrime dq = ld_prime(P.primes, i);
+  return wide_reduce(s, dq.p, dq.mu, (int)dq.nb);
+}
+
 // No folded term.
 struct NoFold {
   __device__ bool operator()(int, uint64_t*) const { return false; }
@@ -483,19 +517,10 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   }
   __syncthreads();
   const tw2* ibq = reinterpret_cast<const tw2*>(P.inv_b_mod_q);
-  const tw2* bq = reinterpret_cast<const tw2*>(P.bhat_mod_q);
   // Sum over j in [j0, j1) of the reference's products for limb i of a prime
-  // wider than 56 bits (q0), reduced term by term.
+  // wider than 56 bits (q0), canonical.
   auto conv = [&](int i, int j0, int j1) -> uint64_t {
-    const DevPrime dq = ld_prime(P.primes, i);
-    const tw2* ci = bq + (size_t)i * K;
-    uint64_t acc = 0;
-#pragma unroll 4
-    for (int j = j0; j < j1; ++j) {
-      const tw2 w = ld_tw2(ci, j);
-      acc = add_mod(acc, moddown_term(aux[j * BC_COEF + c], w, dq, P.moddown_exact), dq.p);
-    }
-    return acc;
+    return P.moddown_exact ? moddown_wide<false>(P, aux, c, i, j0, j1) : moddown_wide<true>(P, aux, c, i, j0, j1);
   };
   // (x_i - acc) B^-1 (+ add_i) mod q_i from the sum acc of limb i, which is
   // below md_red[i][1] (5 q_i, or q0 for the canonical q0 sums).
@@ -512,8 +537,8 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
     return inQ[(size_t)i * N + cid] + as_const(P.md_red)[2 * i + 1] - acc;
   };
   // This wave's limbs g, g+4, ... below iend, in passes of four, three, two or one.
-  // A limb of a prime wider than 56 bits (q0) is summed term by term with
-  // conv and written by wide(i, acc); the others take the lazy sums of
+  // A limb of a prime wider than 56 bits (q0) is summed exactly by conv
+  // (canonical) and written by wide(i, acc); the others take the lazy sums of
   // moddown_convs, with the term z (-B) when fold(i, &z) says so, and are
   // written by narrow(i, acc).
   auto limbs = [&](int iend, auto wide, auto fold, auto narrow) {
@@ -593,7 +618,7 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   //       = (x_i - acc_i + z_i B) (B^-1 q_l^-1),  z_i = q_i + add_i - top  (mod q_i)
   // with top < q_l < q_i (the Q primes decrease): 0 < z_i < 2 q_i.  The
   // narrow limbs add the term z_i (-B) to their sums and take one Shoup
-  // product; q0 (summed term by term) multiplies both parts.
+  // product; q0 (summed by moddown_wide) multiplies both parts.
   const tw2* iql = reinterpret_cast<const tw2*>(P.inv_ql_mod_qi) + (size_t)level * k;
   const tw2* fin = reinterpret_cast<const tw2*>(P.md_fin) + (size_t)level * k;
   auto zval = [&](int i) { return ld_prime_p(P.primes, i) + (add ? add[(size_t)i * N + cid] : 0) - top; };

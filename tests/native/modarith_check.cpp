@@ -107,11 +107,12 @@ int chk_moddown_terms(int N, int k, int kl, int prec, int64_t trials, uint64_t s
       if (lazy >= 2 * p || lazy % p != ref) ++bad;
       if (ref != exact) ++quirks;
       if (exact >= d.quirk_lo && exact < d.quirk_thr) ++slow;
-      if (d.nb <= 56) {
-        // the screen must pass every product that deviates
-        const MdConst& m = H.md_const[(size_t)i * kl + j];
+      {
+        // the screen must pass every product that deviates (the device
+        // table's words: q0's are not in the host's [i][j] table)
+        const MdTerm& m = H.md_terms[(size_t)j * k + i];
         const uint32_t f = moddown_screen((uint32_t)a, (uint32_t)(a >> 32), m.w0, m.w1, m.w2);
-        const bool in = f <= m.thr;
+        const bool in = f <= H.md_thr[i];
         if (ref != exact && !in) ++bad;
         if (in) ++screened;
       }
@@ -204,6 +205,66 @@ int64_t chk_moddown_screen_words(int64_t trials, uint64_t seed) {
 // summed in 64 bits from the 28-bit halves of a_j and the MdConst halves of
 // c_ij, compared with the 128-bit sum, for every Q prime below 2^56.
 // Returns the number of mismatches (or -1 on bad params).
+// ModDown's sums for a Q prime wider than 2^56 (q0) as moddown_wide forms
+// them (wide_accumulate over the md_terms pieces of c, the screen and
+// wide_correct for the products it passes, wide_reduce) against the
+// reference's term-by-term sum of ref_barrett products mod q.  Each trial
+// draws K operands a_j < b_j: random, the largest (b_j - 1), or aimed so that
+// a_j c_j mod q lies at the edges of or inside the deviation window; trial 0
+// takes the largest everywhere (the largest total).  Also checks that the
+// pieces make up c.  Returns the number of mismatches, -1 on bad params, -2
+// when the screen passed no product (the aimed products missed the window).
+// (For the reference's 63-bit q0 the quotient error stays below 2, so none
+// of its products deviates; the screened ones add q through wide_correct.)
+int64_t chk_moddown_wide(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
+  HostParams H;
+  if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;
+  kl = H.K;
+  std::mt19937_64 rng(seed);
+  int64_t bad = 0, screened = 0;
+  for (int i = 0; i < k; ++i) {
+    const DevPrime& d = H.dprimes[i];
+    if (d.nb <= 56) continue;
+    const uint64_t p = d.p;
+    for (int j = 0; j < kl; ++j) {
+      const MdTerm& m = H.md_terms[(size_t)j * k + i];
+      const uint64_t c = H.bhat_mod_q[((size_t)i * kl + j) * 2];
+      if (m.cl + ((uint64_t)m.ch << 28) + ((uint64_t)m.cs << 56) != c || m.cl >> 28 || m.ch >> 28) ++bad;
+    }
+    for (int64_t t = 0; t < trials; ++t) {
+      uint64_t s[4] = {0, 0, 0, 0};
+      uint64_t want = 0;
+      for (int j = 0; j < kl; ++j) {
+        const uint64_t bj = H.primes[k + j];
+        const uint64_t c = H.bhat_mod_q[((size_t)i * kl + j) * 2];
+        const uint64_t cq = H.bhat_mod_q[((size_t)i * kl + j) * 2 + 1];
+        const int kind = t == 0 ? 1 : (int)(rng() % 4);
+        uint64_t a = kind == 1 ? bj - 1 : rng() % bj;
+        if (kind >= 2 && c != 0) {
+          const uint64_t span = d.quirk_thr > d.quirk_lo ? d.quirk_thr - d.quirk_lo : 1;
+          uint64_t target = kind == 2 ? d.quirk_lo + (rng() % 3) - 1 : d.quirk_lo + rng() % span;
+          target %= p;
+          const uint64_t base = (uint64_t)(((unsigned __int128)target * inv_mod(c, p)) % p);
+          a = base < bj ? base : base % bj;
+        }
+        const uint64_t ref = ref_barrett(mulhi64(a, c), a * c, p, d.mu, (int)d.nb);
+        const uint64_t exact = mul_shoup(a, c, cq, p);
+        want = add_mod(want, ref, p);
+        const MdTerm& m = H.md_terms[(size_t)j * k + i];
+        wide_accumulate(s, a, m.cl, m.ch, m.cs);
+        if (moddown_screen((uint32_t)a, (uint32_t)(a >> 32), m.w0, m.w1, m.w2) <= H.md_thr[i]) {
+          wide_correct(s, ref + p - exact);
+          ++screened;
+        } else if (ref != exact)
+          ++bad;  // a deviating product the screen let through
+      }
+      if (wide_reduce(s, p, d.mu, (int)d.nb) != want) ++bad;
+    }
+  }
+  if (bad == 0 && screened == 0) return -2;
+  return bad;
+}
+
 int64_t chk_moddown_accumulate(int N, int k, int kl, int prec, int64_t trials, uint64_t seed) {
   HostParams H;
   if (!build_host_params(N, k, kl, prec, &H).empty()) return -1;

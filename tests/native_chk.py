@@ -45,6 +45,8 @@ def load():
     lib.chk_moddown_finish.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int64, ctypes.c_uint64]
     lib.chk_modup_sums.restype = ctypes.c_int64
     lib.chk_modup_sums.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int64, ctypes.c_uint64]
+    lib.chk_moddown_wide.restype = ctypes.c_int64
+    lib.chk_moddown_wide.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int64, ctypes.c_uint64]
     lib.chk_row_twiddles.restype = ctypes.c_int64
     lib.chk_row_twiddles.argtypes = [ctypes.c_int] * 4
     lib.chk_table.restype = ctypes.c_int64

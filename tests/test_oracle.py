@@ -307,6 +307,17 @@ def test_modup_tables_and_lazy_sums():
         assert chk.chk_modup_sums(*shape, 4, 23) == 0, shape
 
 
+def test_moddown_wide_sums():
+    """ModDown's exact sums for the 63-bit q0 (six multiply-adds per term, the
+    screen's corrections, one Barrett reduction) equal the reference's
+    term-by-term sum of its Barrett products, with operands aimed into the
+    deviation window, random and at the largest values.  (With prec 45 the
+    special primes are 45-bit, and no a_j < b_j can be aimed at q0's window.)"""
+    chk = native_chk.load()
+    for shape in [(65536, 30, 31, 48), (4096, 4, 5, 52), (16384, 5, 5, 55), (8192, 20, 64, 48)]:
+        assert chk.chk_moddown_wide(*shape, 3000, 29) == 0, shape
+
+
 def test_row_twiddle_order():
     """The row-ordered twiddle tables of every two-pass split (N = 2^12 ..
     2^16): each entry a row-pass thread reads is the natural table's entry for
