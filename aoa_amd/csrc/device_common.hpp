@@ -23,6 +23,17 @@ __device__ __forceinline__ cptr<T> as_const(const T* p) {
   return (cptr<T>)p;
 }
 
+// A pointer read from a descriptor in memory, marked as pointing to global
+// memory.  The compiler otherwise treats it as generic and accesses it with
+// flat instructions, which count against the LDS / scalar-load counter as
+// well, so that every wait for an LDS exchange or a scalar load also waits
+// for them, and take 64-bit addresses in vector registers.  The cast is
+// undone at once; the compiler carries the address space to every use.
+template <class T>
+__device__ __forceinline__ T* glob(T* p) {
+  return (T*)(T __attribute__((address_space(1)))*)(uintptr_t)p;
+}
+
 // Entry i of a read-only table of tw2 / DevPrime through the constant
 // address space (word by word: structs cannot be copied out of it).
 __device__ __forceinline__ tw2 ld_tw2(const tw2* base, size_t i) {

@@ -540,9 +540,9 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col_d(DevParams P, const NttDes
   const int tiles = N2 / Geo<S>::C;
   const NttDesc e = d[wid / tiles];
   if (use_lazy<MODE>(P, e.prime))
-    fwd_col_body<S, true>(P, e.src, e.dst, e.prime, wid % tiles, N2, lds);
+    fwd_col_body<S, true>(P, glob(e.src), glob(e.dst), e.prime, wid % tiles, N2, lds);
   else if (MODE != 1)
-    fwd_col_body<S, false>(P, e.src, e.dst, e.prime, wid % tiles, N2, lds);
+    fwd_col_body<S, false>(P, glob(e.src), glob(e.dst), e.prime, wid % tiles, N2, lds);
 }
 
 template <int S, int MODE>
@@ -552,9 +552,9 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row_d(DevParams P, const NttDes
   const int tiles = (P.N / S) / Geo<S>::C;
   const NttDesc e = d[wid / tiles];
   if (use_lazy<MODE>(P, e.prime))
-    fwd_row_body<S, true>(P, e.dst, e.dst, e.prime, wid % tiles, g0, lds);
+    fwd_row_body<S, true>(P, glob(e.dst), glob(e.dst), e.prime, wid % tiles, g0, lds);
   else if (MODE != 1)
-    fwd_row_body<S, false>(P, e.dst, e.dst, e.prime, wid % tiles, g0, lds);
+    fwd_row_body<S, false>(P, glob(e.dst), glob(e.dst), e.prime, wid % tiles, g0, lds);
 }
 
 template <int S, int MODE>
@@ -564,9 +564,9 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_d(DevParams P, const NttDes
   const int tiles = (P.N / S) / Geo<S>::C;
   const NttDesc e = d[wid / tiles];
   if (use_lazy<MODE>(P, e.prime))
-    inv_row_body<S, true>(P, e.src, e.dst, e.key, nullptr, e.prime, wid % tiles, g0, lds);
+    inv_row_body<S, true>(P, glob(e.src), glob(e.dst), glob(e.key), nullptr, e.prime, wid % tiles, g0, lds);
   else if (MODE != 1)
-    inv_row_body<S, false>(P, e.src, e.dst, e.key, nullptr, e.prime, wid % tiles, g0, lds);
+    inv_row_body<S, false>(P, glob(e.src), glob(e.dst), glob(e.key), nullptr, e.prime, wid % tiles, g0, lds);
 }
 
 // One workgroup per tile and key half: workgroups w and w + 8 take the two
@@ -582,9 +582,9 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const K
   const int y = (w >> 3) & 1, b = ((w >> 4) << 3) | (w & 7);
   if (b >= ntiles) return;
   const KeyDesc2& e = d[b / tiles];
-  const uint64_t* src = e.src;
-  uint64_t *d0 = e.dst[0], *d1 = e.dst[1];
-  const uint64_t *k0 = e.key[0], *k1 = e.key[1], *a0 = e.add[0], *a1 = e.add[1];
+  const uint64_t* src = glob(e.src);
+  uint64_t *d0 = glob(e.dst[0]), *d1 = glob(e.dst[1]);
+  const uint64_t *k0 = glob(e.key[0]), *k1 = glob(e.key[1]), *a0 = glob(e.add[0]), *a1 = glob(e.add[1]);
   const int pr = e.prime;
   if (use_lazy<MODE>(P, pr))
     inv_row_key2_body<S, true>(P, src, d0, d1, k0, k1, a0, a1, pr, b % tiles, g0, lds, y);
@@ -598,8 +598,8 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_tensor_d(DevParams P, const
   const int wid = work_id(P);
   const int tiles = (P.N / S) / Geo<S>::C;
   const TensorRowDesc& e = d[wid / tiles];
-  const uint64_t *a0 = e.a0, *a1 = e.a1, *b0 = e.b0, *b1 = e.b1;
-  uint64_t *xd = e.x, *o0 = e.dst[0], *o1 = e.dst[1], *o2 = e.dst[2];
+  const uint64_t *a0 = glob(e.a0), *a1 = glob(e.a1), *b0 = glob(e.b0), *b1 = glob(e.b1);
+  uint64_t *xd = glob(e.x), *o0 = glob(e.dst[0]), *o1 = glob(e.dst[1]), *o2 = glob(e.dst[2]);
   const int pr = e.prime;
   if (use_lazy<MODE>(P, pr))
     inv_row_tensor_body<S, true>(P, a0, a1, b0, b1, xd, o0, o1, o2, pr, wid % tiles, g0, lds);
@@ -614,9 +614,9 @@ __global__ __launch_bounds__(256) void k_ntt_inv_col_d(DevParams P, const NttDes
   const int tiles = N2 / Geo<S>::C;
   const NttDesc e = d[wid / tiles];
   if (use_lazy<MODE>(P, e.prime))
-    inv_col_body<S, true>(P, e.dst, e.dst, e.prime, wid % tiles, N2, lds);
+    inv_col_body<S, true>(P, glob(e.dst), glob(e.dst), e.prime, wid % tiles, N2, lds);
   else if (MODE != 1)
-    inv_col_body<S, false>(P, e.dst, e.dst, e.prime, wid % tiles, N2, lds);
+    inv_col_body<S, false>(P, glob(e.dst), glob(e.dst), e.prime, wid % tiles, N2, lds);
 }
 
 // ------------------------------------------- single-workgroup transform --
@@ -785,8 +785,8 @@ template <int LOGN>
 __global__ __launch_bounds__(SGeo<LOGN>::TH) void k_ntt_fwd_single_d(DevParams P, const NttDesc* d) {
   __shared__ uint64_t lds[SGeo<LOGN>::LDS];
   const NttDesc& e = d[blockIdx.x];
-  const uint64_t* src = e.src;
-  uint64_t* dst = e.dst;
+  const uint64_t* src = glob(e.src);
+  uint64_t* dst = glob(e.dst);
   const int pr = e.prime;
   if (lazy_prime(P, pr))
     fwd_single_body<LOGN, true>(P, src, dst, pr, lds);
@@ -798,9 +798,9 @@ template <int LOGN>
 __global__ __launch_bounds__(SGeo<LOGN>::TH) void k_ntt_inv_single_d(DevParams P, const NttDesc* d) {
   __shared__ uint64_t lds[SGeo<LOGN>::LDS];
   const NttDesc& e = d[blockIdx.x];
-  const uint64_t* src = e.src;
-  uint64_t* dst = e.dst;
-  const uint64_t* key = e.key;
+  const uint64_t* src = glob(e.src);
+  uint64_t* dst = glob(e.dst);
+  const uint64_t* key = glob(e.key);
   const int pr = e.prime;
   if (lazy_prime(P, pr))
     inv_single_body<LOGN, true>(P, src, dst, key, pr, lds);
@@ -814,9 +814,9 @@ __global__ __launch_bounds__(SGeo<LOGN>::TH) void k_ntt_inv_single_key2_d(DevPar
   __shared__ uint64_t lds[SGeo<LOGN>::LDS];
   const KeyDesc2& e = d[blockIdx.x >> 1];
   const int y = blockIdx.x & 1;
-  const uint64_t* src = e.src;
-  uint64_t* dst = y ? e.dst[1] : e.dst[0];
-  const uint64_t* key = y ? e.key[1] : e.key[0];
+  const uint64_t* src = glob(e.src);
+  uint64_t* dst = glob(y ? e.dst[1] : e.dst[0]);
+  const uint64_t* key = glob(y ? e.key[1] : e.key[0]);
   const int pr = e.prime;
   if (lazy_prime(P, pr))
     inv_single_body<LOGN, true>(P, src, dst, key, pr, lds);

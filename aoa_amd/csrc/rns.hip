@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void k_tensor_d(DevParams P, const TensorDesc*
   const int per = P.N / 512;
   const TensorDesc t = dd[blockIdx.x / per];
   const size_t e = (size_t)(blockIdx.x % per) * 512 + (size_t)threadIdx.x * 2;
-  tensor_body(P.primes[t.prime], t.d0, t.d1, t.d2, t.a0, t.a1, t.b0, t.b1, e);
+  tensor_body(P.primes[t.prime], glob(t.d0), glob(t.d1), glob(t.d2), glob(t.a0), glob(t.a1), glob(t.b0), glob(t.b1), e);
 }
 
 void launch_tensor_desc(const DevParams& P, const TensorDesc* d, int ndesc, hipStream_t st) {
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   extern __shared__ uint64_t tl[];
   const int per = P.N / BC_COEF;
   const ModUpDesc u = dd[blockIdx.x / per];
-  modup_body(P, u.x, u.outB, u.level, blockIdx.x % per, tl);
+  modup_body(P, glob(u.x), glob(u.outB), u.level, blockIdx.x % per, tl);
 }
 
 void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int max_level, hipStream_t st) {
@@ -658,6 +658,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   extern __shared__ uint64_t aux[];
   const int per = P.N / BC_COEF;
   const ModDownDesc m = dd[blockIdx.x / per];
+  // generic (flat) accesses here: the global form ran 1 % slower (more
+  // scalar registers held for the bases, which this kernel already spills)
   moddown_body(P, m.inQ, m.inB, m.add, m.out, m.level, m.rescale, blockIdx.x % per, aux);
 }
 
@@ -735,7 +737,7 @@ __global__ __launch_bounds__(256) void k_dot(DevParams P, uint64_t* c, const uin
   const DevPrime d = P.primes[(int)(((size_t)blockIdx.x * 256) >> P.logN)];
   uint64_t hi = 0, lo = 0;
   for (int i = 0; i < k; ++i) {
-    const uint64_t x = a[i][e], y = b[i][e];
+    const uint64_t x = glob(a[i])[e], y = glob(b[i])[e];
     const uint64_t l = x * y;
     lo += l;
     hi += mulhi64(x, y) + (lo < l ? 1u : 0u);
@@ -822,9 +824,11 @@ __global__ __launch_bounds__(256) void k_automorphism_d(DevParams P, const AutoD
   const uint64_t j = (uint64_t)(blk % per) * 256 + threadIdx.x;
   const uint64_t p = P.primes[a.prime].p;
   const uint64_t t = (j * a.ginv) & (2 * N - 1);
-  uint64_t v = t < N ? a.in[t] : neg_mod(a.in[t - N], p);
-  if (a.addsrc) v = add_mod(v, a.addsrc[j], p);
-  a.out[j] = v;
+  const uint64_t* in = glob(a.in);
+  const uint64_t* addsrc = glob(a.addsrc);
+  uint64_t v = t < N ? in[t] : neg_mod(in[t - N], p);
+  if (addsrc) v = add_mod(v, addsrc[j], p);
+  glob(a.out)[j] = v;
 }
 
 void launch_automorphism_desc(const DevParams& P, const AutoDesc* d, int ndesc, hipStream_t st) {
