@@ -38,10 +38,8 @@ struct BatchGroup {
   int n_auto = 0;
   NttDesc* d_intt3 = nullptr;  // mul: INTT of d0, d1, d2
   int n_intt3 = 0;
-  TensorRowDesc* d_trow = nullptr;  // mul (N >= 8192): tensor fused into the first inverse pass
+  NttDesc* d_trow = nullptr;  // mul (N >= 8192): inverse NTT of d2 = a1 b1, the product in its first pass
   int n_trow = 0, ns_trow = -1;
-  NttDesc* d_tcol = nullptr;  // ... and the column pass of d2
-  int n_tcol = 0;
   ModUpDesc* d_modup = nullptr;
   int n_modup = 0;
   int max_level = 0;
@@ -99,10 +97,13 @@ static BatchGroup& group_for(aoa_batch* b, const aoa_params* p, const uint64_t* 
 // domain) times both key halves into P0 / P1, one KeyDesc2 per limb for the
 // first inverse pass and one NttDesc per output limb for the second.  A0 / A1
 // (Lq limbs, NTT domain, or null) join the Q limbs of P0 / P1 times B mod q,
-// so that ModDown adds them (two-pass transforms only).
+// so that ModDown adds them (two-pass transforms only).  With c1 / c2 set
+// (HomMul's operands, two-pass transforms) the Q limbs take X = a1 b1 and
+// the adds d0, d1 from the operands in the pass instead (KeyDesc2::ab).
 static void add_key_descs(std::vector<KeyDesc2>& key2, std::vector<NttDesc>& keycol, const uint64_t* X,
                           uint64_t* P0, uint64_t* P1, const uint64_t* kb, const uint64_t* ka, const uint64_t* A0,
-                          const uint64_t* A1, int Lq, int K, int k, size_t N) {
+                          const uint64_t* A1, int Lq, int K, int k, size_t N, const uint64_t* const* c1 = nullptr,
+                          const uint64_t* const* c2 = nullptr) {
   for (int m = 0; m < Lq + K; ++m) {
     const int pr = m < Lq ? m : k + (m - Lq);
     const size_t o = (size_t)m * N;
@@ -114,6 +115,11 @@ static void add_key_descs(std::vector<KeyDesc2>& key2, std::vector<NttDesc>& key
     e.key[1] = ka + (size_t)pr * N;
     e.add[0] = A0 && m < Lq ? A0 + o : nullptr;
     e.add[1] = A1 && m < Lq ? A1 + o : nullptr;
+    const bool t = c1 && c2 && m < Lq;
+    e.ab[0] = t ? c1[0] + o : nullptr;
+    e.ab[1] = t ? c1[1] + o : nullptr;
+    e.ab[2] = t ? c2[0] + o : nullptr;
+    e.ab[3] = t ? c2[1] + o : nullptr;
     e.prime = pr;
     e.pad = 0;
     key2.push_back(e);
@@ -135,7 +141,7 @@ void batch_sizes(const aoa_batch* b, size_t* ws_bytes, size_t* desc_bytes) {
       for (auto& it : g.muls) {
         const size_t Lq = it.level + 1;
         s += mul_ws_limbs((int)Lq, (int)K) * N * 8;
-        desc += Lq * (std::max(sizeof(TensorDesc), sizeof(TensorRowDesc)) + 3 * sizeof(NttDesc)) + sizeof(ModUpDesc) +
+        desc += Lq * (sizeof(TensorDesc) + 3 * sizeof(NttDesc)) + sizeof(ModUpDesc) +
                 K * sizeof(NttDesc) +
                 (Lq + K) * sizeof(KeyDesc2) + 2 * (Lq + K) * sizeof(NttDesc) + 2 * sizeof(ModDownDesc);
       }
@@ -177,8 +183,7 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
       const size_t N = (size_t)P.N;
       const int k = P.k, K = P.K;
       std::vector<TensorDesc> tens;
-      std::vector<NttDesc> prefwd, preinv, intt3, fwd, keycol, tcol;
-      std::vector<TensorRowDesc> trow;
+      std::vector<NttDesc> prefwd, preinv, intt3, fwd, keycol, trow;
       std::vector<KeyDesc2> key2;
       std::vector<ModUpDesc> mu;
       std::vector<ModDownDesc> md;
@@ -201,28 +206,17 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
         uint64_t* P1 = P0 + (size_t)(Lq + K) * N;
         w += mul_ws_limbs(Lq, K) * N;
         // From N = 8192 up the transforms are two-pass and the tensor product
-        // runs inside the first inverse pass of d2, while d0 and d1 stay in
-        // the NTT domain and join the key products as B d0, B d1: ModDown is
-        // linear in its Q limbs and its B-limb sum does not see them, so
+        // is never stored: d2 = a1 b1 is formed in the first inverse pass of
+        // d2 (for ModUp), and again with d0 and d1 in the key products' first
+        // pass, where d0, d1 join the Q limbs as B d0, B d1: ModDown is linear
+        // in its Q limbs and its B-limb sum does not see them, so
         // ModDown(X + B d) = ModDown(X) + d, residue for residue, and d0, d1
-        // need no inverse transform.  Below, a tensor kernel, the
-        // single-workgroup transforms, and the additions in ModDown.
+        // need no transform.  Below, a tensor kernel, the single-workgroup
+        // transforms, and the additions in ModDown.
         const bool fused = P.N >= 8192;
         for (int i = 0; i < Lq && fused; ++i) {
           const size_t o = (size_t)i * N;
-          TensorRowDesc t;
-          t.a0 = it.c1[0] + o;
-          t.a1 = it.c1[1] + o;
-          t.b0 = it.c2[0] + o;
-          t.b1 = it.c2[1] + o;
-          t.x = X + o;
-          t.dst[0] = D0 + o;
-          t.dst[1] = D1 + o;
-          t.dst[2] = D2 + o;
-          t.prime = i;
-          t.pad = 0;
-          trow.push_back(t);
-          tcol.push_back({D2 + o, D2 + o, nullptr, i, 0});
+          trow.push_back({it.c1[1] + o, D2 + o, it.c2[1] + o, i, 0});
         }
         for (int i = 0; i < Lq && !fused; ++i) {
           const size_t o = (size_t)i * N;
@@ -244,8 +238,8 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
         mu.push_back({D2, X + (size_t)Lq * N, lv, 0});
         for (int j = 0; j < K; ++j)
           fwd.push_back({X + (size_t)(Lq + j) * N, X + (size_t)(Lq + j) * N, nullptr, k + j, 0});
-        add_key_descs(key2, keycol, X, P0, P1, g.evk_b, g.evk_a, fused ? D0 : nullptr, fused ? D1 : nullptr, Lq,
-                      K, k, N);
+        add_key_descs(key2, keycol, X, P0, P1, g.evk_b, g.evk_a, nullptr, nullptr, Lq, K, k, N,
+                      fused ? it.c1 : nullptr, fused ? it.c2 : nullptr);
         for (int y = 0; y < 2; ++y)
           md.push_back({y ? P1 : P0, (y ? P1 : P0) + (size_t)Lq * N, fused ? nullptr : (y ? D1 : D0), it.out[y],
                         lv, 1});
@@ -292,7 +286,6 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
       by_prime(key2);
       by_prime(keycol);
       by_prime(trow);
-      by_prime(tcol);
       // The butterflies of primes of 2^61 and above (the 63-bit q0) are not
       // lazy; they sort first, and the key stage's first pass over the other
       // limbs is launched in its lazy-only form (98 instead of 160 registers,
@@ -318,9 +311,7 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
       g.ns_key2 = split_at(key2);
       g.ns_trow = split_at(trow);
       g.n_trow = (int)trow.size();
-      g.n_tcol = (int)tcol.size();
-      g.d_trow = (TensorRowDesc*)(trow.empty() ? nullptr : put(trow.data(), trow.size() * sizeof(TensorRowDesc)));
-      g.d_tcol = (NttDesc*)(tcol.empty() ? nullptr : put(tcol.data(), tcol.size() * sizeof(NttDesc)));
+      g.d_trow = (NttDesc*)(trow.empty() ? nullptr : put(trow.data(), trow.size() * sizeof(NttDesc)));
       g.n_pre_fwd = (int)prefwd.size();
       g.n_pre_inv = (int)preinv.size();
       g.n_tensor = (int)tens.size();
@@ -407,9 +398,7 @@ int batch_enqueue(aoa_batch* b, hipStream_t st, StageTimer* tm) {
       return 0;
     });
     run(AOA_STAGE_INTT, [](BatchGroup& g) { return g.n_intt3 > 0 || g.n_trow > 0; }, [&](BatchGroup& g) {
-      if (g.n_trow > 0 &&
-          ntt_inverse_tensor_desc(launch_params(b, g), g.d_trow, g.n_trow, g.d_tcol, g.n_tcol, 0, st, g.ns_trow))
-        return -1;
+      if (g.n_trow > 0 && ntt_inverse_desc(launch_params(b, g), g.d_trow, g.n_trow, 0, st, g.ns_trow)) return -1;
       return g.n_intt3 > 0 ? ntt_inverse_desc(launch_params(b, g), g.d_intt3, g.n_intt3, 0, st) : 0;
     });
     run(AOA_STAGE_MODUP, [](BatchGroup& g) { return g.n_modup > 0; }, [&](BatchGroup& g) {

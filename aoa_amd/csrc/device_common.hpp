@@ -92,32 +92,21 @@ struct NttDesc {
   int pad;
 };
 
-// HomMul's tensor product fused into the first inverse pass, per Q limb:
-// d0 = a0 b0, d1 = a0 b1 + a1 b0, d2 = a1 b1 (NTT domain).  dst[0] and dst[1]
-// receive d0 and d1 (NTT domain, in [0, 2p)), x receives d2 (canonical, NTT
-// domain: the key product's input) and dst[2] the first inverse pass of d2
-// (the column pass then runs in place on dst[2]).
-struct TensorRowDesc {
-  const uint64_t* a0;
-  const uint64_t* a1;
-  const uint64_t* b0;
-  const uint64_t* b1;
-  uint64_t* x;
-  uint64_t* dst[3];
-  int prime;
-  int pad;
-};
-
 // Key switching's inverse first pass with both key halves: dst[y] = first
 // inverse pass of (src * key[y] + (B mod q) add[y]) for y = 0, 1 (the column
 // pass then runs on dst[0] and dst[1] as ordinary NttDesc entries).  add is
 // null, or (Q limbs of a two-pass transform only) NTT-domain limbs whose
 // ModDown-scaled sum joins the product: ModDown(X + B d) = ModDown(X) + d.
+// With ab[0] set (HomMul's Q limbs, two-pass transforms), ab = a0, a1, b0,
+// b1 (NTT domain, canonical) stand for src and add: the input is d2 = a1 b1
+// and add[y] is d0 = a0 b0 for y = 0, d1 = a0 b1 + a1 b0 for y = 1, computed
+// in the pass from the operands, so that the tensor product is never stored.
 struct KeyDesc2 {
   const uint64_t* src;
   uint64_t* dst[2];
   const uint64_t* key[2];
   const uint64_t* add[2];
+  const uint64_t* ab[4];
   int prime;
   int pad;
 };

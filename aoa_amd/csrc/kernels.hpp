@@ -105,10 +105,6 @@ int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
 int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const NttDesc* dcol, int ncol, int N1,
                           hipStream_t st,
                           int n2strict = -1, int ncolstrict = -1);
-// HomMul's tensor product fused into the first inverse pass (TensorRowDesc),
-// then the column pass over dcol; two-pass splits only (-1 otherwise).
-int ntt_inverse_tensor_desc(const DevParams& P, const TensorRowDesc* dt, int nt, const NttDesc* dcol, int ncol,
-                            int N1, hipStream_t st, int ntstrict = -1);
 void launch_tensor_desc(const DevParams& P, const TensorDesc* d, int ndesc, hipStream_t st);
 void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int max_level, hipStream_t st);
 void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, hipStream_t st);

@@ -281,6 +281,12 @@ __device__ __forceinline__ uint64_t key_mul(uint64_t x, uint64_t k, const DevPri
   return LZ ? mul_barrett62_lazy(x, k, dp.p, dp.mu2, (int)dp.nb) : mul_barrett(x, k, dp.p, dp.mu, (int)dp.nb);
 }
 
+// Same, canonical: the operand of a further product.
+template <bool LZ>
+__device__ __forceinline__ uint64_t key_mul_canon(uint64_t x, uint64_t k, const DevPrime& dp) {
+  return LZ ? mul_barrett62(x, k, dp.p, dp.mu2, (int)dp.nb) : mul_barrett(x, k, dp.p, dp.mu, (int)dp.nb);
+}
+
 // x (products in [0, 2p) when LZ, else canonical) plus (B mod q) times the
 // NTT-domain addend of a Q limb pr, left in the same range.
 template <bool LZ>
@@ -326,12 +332,17 @@ __device__ __forceinline__ void inv_row_body(const DevParams& P, const uint64_t*
 // pass of (src * key0 + (B mod q) add0), dst1 = same with key1 and add1 (the
 // adds may be null).  The workgroup reads its src tile twice; the second read
 // is served by the L2 cache, so HBM sees the input once while the registers
-// hold one tile at a time.
+// hold one tile at a time.  With HomMul's operands (a0 set: Q limbs of the
+// batched form) the input is d2 = a1 b1 and the adds are d0 = a0 b0 and
+// d1 = a0 b1 + a1 b0 (congruent to the reference's (a0+a1)(b0+b1) - d0 - d2),
+// formed here from the four operand tiles, which the two key halves' workgroups
+// of a tile read from HBM once between them (the second from L2).
 template <int S, bool LZ>
 __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint64_t* src, uint64_t* dst0,
                                                   uint64_t* dst1, const uint64_t* key0, const uint64_t* key1,
-                                                  const uint64_t* add0, const uint64_t* add1, int pr, int tile,
-                                                  int g0, uint64_t* lds, int ysel) {
+                                                  const uint64_t* add0, const uint64_t* add1, const uint64_t* a0,
+                                                  const uint64_t* a1, const uint64_t* b0, const uint64_t* b1, int pr,
+                                                  int tile, int g0, uint64_t* lds, int ysel) {
   using G = Geo<S>;
   constexpr int C = G::C, TS = G::TS, PS = S + S / 16;
   const DevPrime dp = P.primes[pr];
@@ -345,52 +356,28 @@ __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint
   for (int y = 0; y < 2; ++y) {
     if (ysel >= 0 && y != ysel) continue;
     const uint64_t* kp = (y ? key1 : key0) + off;
-    const uint64_t* ap = y ? add1 : add0;
     uint64_t x[16];
+    if (a0) {
+      const tw2 bq = reinterpret_cast<const tw2*>(P.b_mod_q)[pr];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(src[off + sl + TS * k], kp[sl + TS * k], dp);
-    if (ap) add_bscaled<LZ>(x, ap + off + sl, TS, P, pr, p);  // set for Q limbs only
+      for (int k = 0; k < 16; ++k) {
+        const size_t i = off + sl + TS * k;
+        const uint64_t A0 = a0[i], A1 = a1[i], B0 = b0[i], B1 = b1[i];
+        const uint64_t d2 = key_mul_canon<LZ>(A1, B1, dp);
+        // lazy: d in [0, 4p), the sum in [0, 3p) -> [0, 2p); strict: below 2p, then canonical
+        const uint64_t d = y ? key_mul<LZ>(A0, B1, dp) + key_mul<LZ>(A1, B0, dp) : key_mul<LZ>(A0, B0, dp);
+        const uint64_t t = key_mul<LZ>(d2, kp[sl + TS * k], dp) + mul_shoup(d, bq.w, bq.q, p);
+        x[k] = LZ ? (t >= 2 * p ? t - 2 * p : t) : (t >= p ? t - p : t);
+      }
+    } else {
+      const uint64_t* ap = y ? add1 : add0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x[k] = key_mul<LZ>(src[off + sl + TS * k], kp[sl + TS * k], dp);
+      if (ap) add_bscaled<LZ>(x, ap + off + sl, TS, P, pr, p);  // set for Q limbs only
+    }
     if (y && ysel < 0) __syncthreads();  // the row's LDS block is reused
     inv_row_core<S, LZ>(P, x, (y ? dst1 : dst0) + off, p, T, TR, row, sl, g0, lds + rl * PS);
   }
-}
-
-// HomMul's tensor product and the first inverse pass of d2, one workgroup per
-// tile: d0 = a0 b0 and d1 = a0 b1 + a1 b0 (congruent to the reference's
-// (a0+a1)(b0+b1) - d0 - d2) go to d0out / d1out in the NTT domain, in
-// [0, 2p); d2 = a1 b1 goes canonical to xd and through the inverse row pass
-// to dst.
-template <int S, bool LZ>
-__device__ __forceinline__ void inv_row_tensor_body(const DevParams& P, const uint64_t* a0, const uint64_t* a1,
-                                                    const uint64_t* b0, const uint64_t* b1, uint64_t* xd,
-                                                    uint64_t* d0out, uint64_t* d1out, uint64_t* dst, int pr,
-                                                    int tile, int g0, uint64_t* lds) {
-  using G = Geo<S>;
-  constexpr int C = G::C, TS = G::TS, PS = S + S / 16;
-  const DevPrime dp = P.primes[pr];
-  const uint64_t p = dp.p;
-  const uint64_t* T = P.tw_inv + (size_t)pr * P.N * 2;
-  const uint64_t* TR = P.tw_row[G::LOGS - 6][1] + (size_t)pr * P.N * 2;
-  const int rl = threadIdx.x / TS, sl = threadIdx.x % TS;
-  const int row = tile * C + rl;
-  const size_t off = (size_t)row * S;
-  uint64_t x[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const size_t i = off + sl + TS * k;
-    const uint64_t A0 = a0[i], A1 = a1[i], B0 = b0[i], B1 = b1[i];
-    uint64_t c0 = key_mul<LZ>(A0, B0, dp);
-    const uint64_t t = key_mul<LZ>(A0, B1, dp) + key_mul<LZ>(A1, B0, dp);
-    uint64_t c2 = key_mul<LZ>(A1, B1, dp);
-    // lazy: products in [0, 2p), the sum in [0, 4p) -> [0, 2p); strict: canonical
-    const uint64_t c1 = LZ ? (t >= 2 * p ? t - 2 * p : t) : (t >= p ? t - p : t);
-    if (LZ) c2 = c2 >= p ? c2 - p : c2;
-    d0out[i] = c0;
-    d1out[i] = c1;
-    xd[i] = c2;
-    x[k] = c2;
-  }
-  inv_row_core<S, LZ>(P, x, dst + off, p, T, TR, row, sl, g0, lds + rl * PS);
 }
 
 // Column pass of the inverse: global stages log2(S)-1 .. 0, N^-1 folded into
@@ -585,26 +572,12 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const K
   const uint64_t* src = glob(e.src);
   uint64_t *d0 = glob(e.dst[0]), *d1 = glob(e.dst[1]);
   const uint64_t *k0 = glob(e.key[0]), *k1 = glob(e.key[1]), *a0 = glob(e.add[0]), *a1 = glob(e.add[1]);
+  const uint64_t *t0 = glob(e.ab[0]), *t1 = glob(e.ab[1]), *t2 = glob(e.ab[2]), *t3 = glob(e.ab[3]);
   const int pr = e.prime;
   if (use_lazy<MODE>(P, pr))
-    inv_row_key2_body<S, true>(P, src, d0, d1, k0, k1, a0, a1, pr, b % tiles, g0, lds, y);
+    inv_row_key2_body<S, true>(P, src, d0, d1, k0, k1, a0, a1, t0, t1, t2, t3, pr, b % tiles, g0, lds, y);
   else if (MODE != 1)
-    inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, a0, a1, pr, b % tiles, g0, lds, y);
-}
-
-template <int S, int MODE>
-__global__ __launch_bounds__(256) void k_ntt_inv_row_tensor_d(DevParams P, const TensorRowDesc* d, int g0) {
-  __shared__ uint64_t lds[Geo<S>::ROW_LDS];
-  const int wid = work_id(P);
-  const int tiles = (P.N / S) / Geo<S>::C;
-  const TensorRowDesc& e = d[wid / tiles];
-  const uint64_t *a0 = glob(e.a0), *a1 = glob(e.a1), *b0 = glob(e.b0), *b1 = glob(e.b1);
-  uint64_t *xd = glob(e.x), *o0 = glob(e.dst[0]), *o1 = glob(e.dst[1]), *o2 = glob(e.dst[2]);
-  const int pr = e.prime;
-  if (use_lazy<MODE>(P, pr))
-    inv_row_tensor_body<S, true>(P, a0, a1, b0, b1, xd, o0, o1, o2, pr, wid % tiles, g0, lds);
-  else if (MODE != 1)
-    inv_row_tensor_body<S, false>(P, a0, a1, b0, b1, xd, o0, o1, o2, pr, wid % tiles, g0, lds);
+    inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, a0, a1, t0, t1, t2, t3, pr, b % tiles, g0, lds, y);
 }
 
 template <int S, int MODE>
@@ -1032,26 +1005,6 @@ int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const 
   {
     const int tiles = N2 / (4096 / N1);
 #define CALLC(S) AOA_SPLIT_LAUNCH(k_ntt_inv_col_d, S, tiles, dcol, ncol, ncolstrict, N2)
-    AOA_DISPATCH_S(N1, CALLC)
-#undef CALLC
-  }
-  return 0;
-}
-
-int ntt_inverse_tensor_desc(const DevParams& P, const TensorRowDesc* dt, int nt, const NttDesc* dcol, int ncol,
-                            int N1, hipStream_t st, int ntstrict) {
-  N1 = pick_n1(P, N1);
-  if (!N1 || N1 == P.N || nt <= 0 || ncol <= 0) return -1;  // two-pass transforms only
-  const int N2 = P.N / N1, g0 = log2i(N1);
-  {
-    const int tiles = N1 / (4096 / N2);
-#define CALLR(S) AOA_SPLIT_LAUNCH(k_ntt_inv_row_tensor_d, S, tiles, dt, nt, ntstrict, g0)
-    AOA_DISPATCH_S(N2, CALLR)
-#undef CALLR
-  }
-  {
-    const int tiles = N2 / (4096 / N1);
-#define CALLC(S) AOA_SPLIT_LAUNCH(k_ntt_inv_col_d, S, tiles, dcol, ncol, -1, N2)
     AOA_DISPATCH_S(N1, CALLC)
 #undef CALLC
   }
