@@ -541,12 +541,29 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   // (canonical) and written by wide(i, acc); the others take the lazy sums of
   // moddown_convs, with the term z (-B) when fold(i, &z) says so, and are
   // written by narrow(i, acc).
+  // q0's sum (a 63-bit prime's terms cost about 1.4 times a narrow limb's)
+  // is split over the four waves, as the rescale's top limb is below, so that
+  // the wave that writes limb 0 carries no more than the others: the partial
+  // sums meet in LDS and acc0 is their total, valid after the next barrier.
+  uint64_t* part = aux + (size_t)K * BC_COEF;            // [4][64]: the top limb's partial sums
+  uint64_t* part0 = part + (size_t)BC_GROUPS * BC_COEF;  // [4][64]: q0's
+  const int jq = (K + BC_GROUPS - 1) / BC_GROUPS;
+  const int j0 = g * jq, j1 = min(K, j0 + jq);
+  const bool wide0 = ld_prime_nb(P.primes, 0) > 56;
+  if (wide0) part0[g * BC_COEF + c] = j1 > j0 ? conv(0, j0, j1) : 0;
+  auto acc0 = [&]() {
+    const uint64_t q0 = ld_prime_p(P.primes, 0);
+    uint64_t a = 0;
+#pragma unroll
+    for (int t = 0; t < BC_GROUPS; ++t) a = add_mod(a, part0[t * BC_COEF + c], q0);
+    return a;
+  };
   auto limbs = [&](int iend, auto wide, auto fold, auto narrow) {
     int i = g;
     while (i < iend) {
       const int left = (iend - i + BC_GROUPS - 1) / BC_GROUPS;
       if (ld_prime_nb(P.primes, i) > 56) {
-        wide(i, conv(i, 0, K));
+        wide(i, i == 0 ? acc0() : conv(i, 0, K));
         i += BC_GROUPS;
         continue;
       }
@@ -578,6 +595,7 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
     }
   };
   if (!rescale) {
+    if (wide0) __syncthreads();
     // add_i joins the sums as z = add_i: (x_i - acc + add_i B) B^-1 = (x_i - acc) B^-1 + add_i
     limbs(
         level + 1, [&](int i, uint64_t acc) { out[(size_t)i * N + cid] = finish(i, acc); },
@@ -594,9 +612,6 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   }
   // Rescale needs limb `level` everywhere: its sum over j is split across the
   // four groups and combined through LDS.
-  uint64_t* part = aux + (size_t)K * BC_COEF;
-  const int jq = (K + BC_GROUPS - 1) / BC_GROUPS;
-  const int j0 = g * jq, j1 = min(K, j0 + jq);
   {
     uint64_t r = 0;
     if (ld_prime_nb(P.primes, level) > 56) {
@@ -641,13 +656,13 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
 
 // Eight waves per SIMD, as k_moddown_d: 2.5 % more plain HomMuls per second.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_moddown(DevParams P, ModDownArgs A) {
-  extern __shared__ uint64_t aux[];  // [K][64] then [4][64] partial sums
+  extern __shared__ uint64_t aux[];  // [K][64], then [4][64] partial sums of the top limb and of q0
   const int y = blockIdx.y;
   moddown_body(P, A.inQ[y], A.inB[y], A.add[y], A.out[y], A.level, A.rescale, blockIdx.x, aux);
 }
 
 void launch_moddown(const DevParams& P, const ModDownArgs& a, hipStream_t st) {
-  const size_t lds = (size_t)(P.K + BC_GROUPS) * BC_COEF * sizeof(uint64_t);
+  const size_t lds = (size_t)(P.K + 2 * BC_GROUPS) * BC_COEF * sizeof(uint64_t);
   k_moddown<<<dim3(P.N / BC_COEF, a.npolys), 256, lds, st>>>(P, a);
 }
 
@@ -664,7 +679,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 }
 
 void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, hipStream_t st) {
-  const size_t lds = (size_t)(P.K + BC_GROUPS) * BC_COEF * sizeof(uint64_t);
+  const size_t lds = (size_t)(P.K + 2 * BC_GROUPS) * BC_COEF * sizeof(uint64_t);
   k_moddown_d<<<dim3((unsigned)(ndesc * (P.N / BC_COEF))), 256, lds, st>>>(P, d);
 }
 
