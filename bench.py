@@ -580,8 +580,8 @@ def main():
                          "the N=2^16 rotate-and-sum batch")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every GPU box here has one GPU and this build has no "
-              "multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -41,7 +41,7 @@ class CUDAManager {
   // (aoa_params_create), so this only checks that init() has run.
   static void precompute();
   // Products of the Q and special primes.  The reference returns NTL::ZZ,
-  // which this build does not use: here they are little-endian 64-bit words.
+  // which this library does not use: here they are little-endian 64-bit words.
   static std::vector<uint64_t> get_q_product();
   static std::vector<uint64_t> get_b_product();
 

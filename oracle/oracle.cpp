@@ -829,7 +829,7 @@ static void sample(const Oracle& o, uint64_t* out, int res0, int nres, int kind,
 // ------------------------------------------------ C1 pipeline
 // One plaintext product end to end, on the sequence of the reference's
 // ckks_keygen / ckks_encrypt / ckks_mul / ckks_decrypt (ckkskeys.cpp:61-131,
-// ckks.cpp:87-105, :273-358, :107-132) with this build's sampler: keys, then
+// ckks.cpp:87-105, :273-358, :107-132) with the library's sampler: keys, then
 // encode -> encrypt -> HomMul+relin+rescale -> decrypt -> decode.  Public
 // key on Q limbs: b = e - a s; relinearisation key on QB limbs:
 // b = rho(s^2) + e - a s; all in the NTT domain.  Nonces: s 0, pk a 1, pk e
