@@ -206,7 +206,7 @@ constexpr int BC_GROUPS = 4;
 // (reduce_sums_lazy; the host checks its bound).  The halves of w come split
 // from the mu_terms table, the NL limbs' entries side by side, so that a term
 // takes one scalar load and no scalar ALU work.
-template <int NL>
+template <int NL, bool LAZY>
 __device__ __forceinline__ void modup_pass(const DevParams& P, const uint64_t* tl, int c, const int* jl, int level,
                                            uint64_t* outB, size_t cid) {
   constexpr uint32_t M28 = (1u << 28) - 1;
@@ -272,17 +272,19 @@ __device__ __forceinline__ void modup_pass(const DevParams& P, const uint64_t* t
       }
     }
   }
-  // one lazy reduction per limb (below 5 b_j), then down to [0, b_j)
+  // one lazy reduction per limb (below 5 b_j), then down to [0, b_j), or with
+  // LAZY to [0, 4 b_j), which the forward NTT's lazy butterflies take
 #pragma unroll
   for (int n = 0; n < NL; ++n) {
     const uint64_t b = d[n].p;
     uint64_t r = reduce_sums_lazy(s0[n], sm[n] - s0[n] - s2[n], s2[n], b, as_const(P.md_red)[2 * (k + jl[n])],
                                   (int)d[n].nb);
-    for (uint64_t m = 4 * b; m >= b; m >>= 1) r = r >= m ? r - m : r;
+    for (uint64_t m = 4 * b; m >= (LAZY ? 4 * b : b); m >>= 1) r = r >= m ? r - m : r;
     outB[(size_t)jl[n] * P.N + cid] = r;
   }
 }
 
+template <bool LAZY>
 __device__ __forceinline__ void modup_body(const DevParams& P, const uint64_t* x, uint64_t* outB, int level,
                                            int blk, uint64_t* tl) {
   const int L = level + 1;
@@ -304,13 +306,13 @@ __device__ __forceinline__ void modup_body(const DevParams& P, const uint64_t* x
     const int left = (P.K - j + BC_GROUPS - 1) / BC_GROUPS;
     const int jl[4] = {j, j + BC_GROUPS, j + 2 * BC_GROUPS, j + 3 * BC_GROUPS};
     if (left >= 4) {
-      modup_pass<4>(P, tl, c, jl, level, outB, cid);
+      modup_pass<4, LAZY>(P, tl, c, jl, level, outB, cid);
       j += 4 * BC_GROUPS;
     } else if (left >= 2) {
-      modup_pass<2>(P, tl, c, jl, level, outB, cid);
+      modup_pass<2, LAZY>(P, tl, c, jl, level, outB, cid);
       j += 2 * BC_GROUPS;
     } else {
-      modup_pass<1>(P, tl, c, jl, level, outB, cid);
+      modup_pass<1, LAZY>(P, tl, c, jl, level, outB, cid);
       j += BC_GROUPS;
     }
   }
@@ -319,7 +321,7 @@ __device__ __forceinline__ void modup_body(const DevParams& P, const uint64_t* x
 // Seven waves per SIMD (72 registers, one spilled): 3 % faster than six.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_modup(DevParams P, const uint64_t* x, uint64_t* outB, int level) {
   extern __shared__ uint64_t tl[];  // [L][64]
-  modup_body(P, x, outB, level, blockIdx.x, tl);
+  modup_body<false>(P, x, outB, level, blockIdx.x, tl);
 }
 
 void launch_modup(const DevParams& P, const uint64_t* x, uint64_t* outB, int level, hipStream_t st) {
@@ -331,7 +333,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   extern __shared__ uint64_t tl[];
   const int per = P.N / BC_COEF;
   const ModUpDesc u = dd[blockIdx.x / per];
-  modup_body(P, glob(u.x), glob(u.outB), u.level, blockIdx.x % per, tl);
+  // the batched form's outputs go only to the forward NTT: left in [0, 4 b_j)
+  modup_body<true>(P, glob(u.x), glob(u.outB), u.level, blockIdx.x % per, tl);
 }
 
 void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int max_level, hipStream_t st) {
