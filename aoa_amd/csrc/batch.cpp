@@ -398,7 +398,9 @@ int batch_enqueue(aoa_batch* b, hipStream_t st, StageTimer* tm) {
       return 0;
     });
     run(AOA_STAGE_INTT, [](BatchGroup& g) { return g.n_intt3 > 0 || g.n_trow > 0; }, [&](BatchGroup& g) {
-      if (g.n_trow > 0 && ntt_inverse_desc(launch_params(b, g), g.d_trow, g.n_trow, 0, st, g.ns_trow)) return -1;
+      // d2 goes to ModUp only, whose Shoup products take [0, 2p)
+      if (g.n_trow > 0 && ntt_inverse_desc(launch_params(b, g), g.d_trow, g.n_trow, 0, st, g.ns_trow, false))
+        return -1;
       return g.n_intt3 > 0 ? ntt_inverse_desc(launch_params(b, g), g.d_intt3, g.n_intt3, 0, st) : 0;
     });
     run(AOA_STAGE_MODUP, [](BatchGroup& g) { return g.n_modup > 0; }, [&](BatchGroup& g) {
@@ -408,7 +410,9 @@ int batch_enqueue(aoa_batch* b, hipStream_t st, StageTimer* tm) {
     run(AOA_STAGE_NTT, [](BatchGroup& g) { return g.n_fwd > 0; },
         [&](BatchGroup& g) { return ntt_forward_desc(launch_params(b, g), g.d_fwd, g.n_fwd, 0, st); });
     run(AOA_STAGE_KEY_INTT, [](BatchGroup& g) { return g.n_key2 > 0; }, [&](BatchGroup& g) {
-      return ntt_inverse_key2_desc(launch_params(b, g), g.d_key2, g.n_key2, g.d_keycol, g.n_keycol, 0, st, g.ns_key2);
+      // the key products go to ModDown only, which takes [0, 2p) except for q0
+      return ntt_inverse_key2_desc(launch_params(b, g), g.d_key2, g.n_key2, g.d_keycol, g.n_keycol, 0, st, g.ns_key2,
+                                   -1, false);
     });
     run(AOA_STAGE_MODDOWN, [](BatchGroup& g) { return g.n_moddown > 0; }, [&](BatchGroup& g) {
       launch_moddown_desc(launch_params(b, g), g.d_moddown, g.n_moddown, st);

@@ -99,12 +99,15 @@ void launch_dr2(const DevParams& P, uint64_t* c21, uint64_t* c22, uint64_t* c23,
 // nstrict: the first nstrict descriptors use primes of 2^61 and above and
 // the rest do not (a prime-major list); -1 when not known.
 int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st, int nstrict = -1);
-int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st, int nstrict = -1);
+// canon = false: the two-pass form's lazy limbs may end in [0, 2p) (for
+// ModUp and ModDown, which take them so); q0 and the single-workgroup form
+// stay canonical.
+int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st, int nstrict = -1,
+                     bool canon = true);
 // Key-switching inverse NTT: first pass from KeyDesc2 entries (one input limb,
 // two keys, two outputs), second pass over the NttDesc list of outputs.
 int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const NttDesc* dcol, int ncol, int N1,
-                          hipStream_t st,
-                          int n2strict = -1, int ncolstrict = -1);
+                          hipStream_t st, int n2strict = -1, int ncolstrict = -1, bool canon = true);
 void launch_tensor_desc(const DevParams& P, const TensorDesc* d, int ndesc, hipStream_t st);
 void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int max_level, hipStream_t st);
 void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, hipStream_t st);

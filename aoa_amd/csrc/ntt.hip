@@ -382,7 +382,9 @@ __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint
 
 // Column pass of the inverse: global stages log2(S)-1 .. 0, N^-1 folded into
 // stage 0.
-template <int S, bool LZ>
+// CANON = false (lazy primes only) leaves the outputs in [0, 2p) for a
+// consumer that takes them so (ModUp, ModDown).
+template <int S, bool LZ, bool CANON = true>
 __device__ __forceinline__ void inv_col_body(const DevParams& P, const uint64_t* src, uint64_t* dst, int pr,
                                              int tile, int N2, uint64_t* lds) {
   using G = Geo<S>;
@@ -429,8 +431,13 @@ __device__ __forceinline__ void inv_col_body(const DevParams& P, const uint64_t*
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     uint64_t u = x[k], v = x[k + 8];
-    x[k] = mul_shoup(u + v, dp.ninv, dp.ninv_q, p);
-    x[k + 8] = mul_shoup(LZ ? u + 2 * p - v : u + (p - v), dp.w1ninv, dp.w1ninv_q, p);
+    if (LZ && !CANON) {
+      x[k] = mul_shoup_lazy(u + v, dp.ninv, dp.ninv_q, p);
+      x[k + 8] = mul_shoup_lazy(u + 2 * p - v, dp.w1ninv, dp.w1ninv_q, p);
+    } else {
+      x[k] = mul_shoup(u + v, dp.ninv, dp.ninv_q, p);
+      x[k + 8] = mul_shoup(LZ ? u + 2 * p - v : u + (p - v), dp.w1ninv, dp.w1ninv_q, p);
+    }
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) dst[(size_t)(sl + TS * k) * N2 + c] = x[k];
@@ -580,16 +587,28 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const K
     inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, a0, a1, t0, t1, t2, t3, pr, b % tiles, g0, lds, y);
 }
 
-template <int S, int MODE>
-__global__ __launch_bounds__(256) void k_ntt_inv_col_d(DevParams P, const NttDesc* d, int N2) {
-  __shared__ uint64_t lds[Geo<S>::COL_LDS];
+template <int S, int MODE, bool CANON>
+__device__ __forceinline__ void inv_col_d(const DevParams& P, const NttDesc* d, int N2, uint64_t* lds) {
   const int wid = work_id(P);
   const int tiles = N2 / Geo<S>::C;
   const NttDesc e = d[wid / tiles];
   if (use_lazy<MODE>(P, e.prime))
-    inv_col_body<S, true>(P, glob(e.dst), glob(e.dst), e.prime, wid % tiles, N2, lds);
+    inv_col_body<S, true, CANON>(P, glob(e.dst), glob(e.dst), e.prime, wid % tiles, N2, lds);
   else if (MODE != 1)
     inv_col_body<S, false>(P, glob(e.dst), glob(e.dst), e.prime, wid % tiles, N2, lds);
+}
+
+template <int S, int MODE>
+__global__ __launch_bounds__(256) void k_ntt_inv_col_d(DevParams P, const NttDesc* d, int N2) {
+  __shared__ uint64_t lds[Geo<S>::COL_LDS];
+  inv_col_d<S, MODE, true>(P, d, N2, lds);
+}
+
+// Same, the lazy limbs' outputs left in [0, 2p) (the q0 limbs stay canonical).
+template <int S, int MODE>
+__global__ __launch_bounds__(256) void k_ntt_inv_col_lazy_d(DevParams P, const NttDesc* d, int N2) {
+  __shared__ uint64_t lds[Geo<S>::COL_LDS];
+  inv_col_d<S, MODE, false>(P, d, N2, lds);
 }
 
 // ------------------------------------------- single-workgroup transform --
@@ -951,7 +970,7 @@ int ntt_forward_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
   return 0;
 }
 
-int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st, int nstrict) {
+int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hipStream_t st, int nstrict, bool canon) {
   N1 = pick_n1(P, N1);
   if (!N1 || ndesc <= 0) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
@@ -969,7 +988,9 @@ int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
   }
   {
     const int tiles = N2 / (4096 / N1);
-#define CALLC(S) AOA_SPLIT_LAUNCH(k_ntt_inv_col_d, S, tiles, d, ndesc, nstrict, N2)
+#define CALLC(S)                                                              \
+  if (canon) AOA_SPLIT_LAUNCH(k_ntt_inv_col_d, S, tiles, d, ndesc, nstrict, N2); \
+  else AOA_SPLIT_LAUNCH(k_ntt_inv_col_lazy_d, S, tiles, d, ndesc, nstrict, N2)
     AOA_DISPATCH_S(N1, CALLC)
 #undef CALLC
   }
@@ -977,7 +998,7 @@ int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
 }
 
 int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const NttDesc* dcol, int ncol, int N1,
-                          hipStream_t st, int n2strict, int ncolstrict) {
+                          hipStream_t st, int n2strict, int ncolstrict, bool canon) {
   N1 = pick_n1(P, N1);
   if (!N1 || n2 <= 0 || ncol <= 0) return -1;
   const int N2 = P.N / N1, g0 = log2i(N1);
@@ -1004,7 +1025,9 @@ int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const 
   }
   {
     const int tiles = N2 / (4096 / N1);
-#define CALLC(S) AOA_SPLIT_LAUNCH(k_ntt_inv_col_d, S, tiles, dcol, ncol, ncolstrict, N2)
+#define CALLC(S)                                                                    \
+  if (canon) AOA_SPLIT_LAUNCH(k_ntt_inv_col_d, S, tiles, dcol, ncol, ncolstrict, N2); \
+  else AOA_SPLIT_LAUNCH(k_ntt_inv_col_lazy_d, S, tiles, dcol, ncol, ncolstrict, N2)
     AOA_DISPATCH_S(N1, CALLC)
 #undef CALLC
   }
