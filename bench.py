@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # `bench.py --profile` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0; the factors make a
 # 1 GiB device copy read 1 GiB and write 1 GiB); the run is summarised in
 # profiles/moddown_pmc_traffic_n16_k30.md.
-MODDOWN_TRAFFIC_BYTES = {32: 3025857525}
+MODDOWN_TRAFFIC_BYTES = {32: 3025902773}
 # VALU wave-instructions per k_moddown_d launch at 32 pairs (SQ_INSTS_VALU summed
 # over the counter instances of one dispatch, rocprofv3 --pmc of
 # `bench.py --profile`; profiles/hommul_batched_n16_k30_valu.md), and a
