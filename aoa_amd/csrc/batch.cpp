@@ -59,7 +59,10 @@ struct BatchLayer {
   std::vector<BatchGroup> groups;
 };
 
-size_t mul_ws_limbs(int L, int K) { return 6 * (size_t)L + 3 * (size_t)K; }
+// Workspace limbs of one HomMul item: X (L + K), d0, d1, d2 (L each), P0, P1
+// (L + K each); from N = 8192 up (fused) the tensor product is never stored
+// and X's Q limbs are not needed: X's special limbs (K), d2 (L), P0, P1.
+size_t mul_ws_limbs(int L, int K, bool fused) { return (fused ? 3 : 6) * (size_t)L + 3 * (size_t)K; }
 size_t rot_ws_limbs(int L, int K) { return 5 * (size_t)L + 3 * (size_t)K; }
 
 }  // namespace aoa
@@ -93,22 +96,23 @@ static BatchGroup& group_for(aoa_batch* b, const aoa_params* p, const uint64_t* 
   return g;
 }
 
-// Key-switching product descriptors of one item: X (Lq + K limbs, NTT
-// domain) times both key halves into P0 / P1, one KeyDesc2 per limb for the
-// first inverse pass and one NttDesc per output limb for the second.  A0 / A1
-// (Lq limbs, NTT domain, or null) join the Q limbs of P0 / P1 times B mod q,
-// so that ModDown adds them (two-pass transforms only).  With c1 / c2 set
-// (HomMul's operands, two-pass transforms) the Q limbs take X = a1 b1 and
-// the adds d0, d1 from the operands in the pass instead (KeyDesc2::ab).
-static void add_key_descs(std::vector<KeyDesc2>& key2, std::vector<NttDesc>& keycol, const uint64_t* X,
-                          uint64_t* P0, uint64_t* P1, const uint64_t* kb, const uint64_t* ka, const uint64_t* A0,
-                          const uint64_t* A1, int Lq, int K, int k, size_t N, const uint64_t* const* c1 = nullptr,
-                          const uint64_t* const* c2 = nullptr) {
+// Key-switching product descriptors of one item: X (Lq Q limbs at XQ, K
+// special limbs at XB, NTT domain) times both key halves into P0 / P1, one
+// KeyDesc2 per limb for the first inverse pass and one NttDesc per output
+// limb for the second.  A0 / A1 (Lq limbs, NTT domain, or null) join the Q
+// limbs of P0 / P1 times B mod q, so that ModDown adds them (two-pass
+// transforms only).  With c1 / c2 set (HomMul's operands, two-pass
+// transforms) the Q limbs take X = a1 b1 and the adds d0, d1 from the
+// operands in the pass instead (KeyDesc2::ab), and XQ is not read (null).
+static void add_key_descs(std::vector<KeyDesc2>& key2, std::vector<NttDesc>& keycol, const uint64_t* XQ,
+                          const uint64_t* XB, uint64_t* P0, uint64_t* P1, const uint64_t* kb, const uint64_t* ka,
+                          const uint64_t* A0, const uint64_t* A1, int Lq, int K, int k, size_t N,
+                          const uint64_t* const* c1 = nullptr, const uint64_t* const* c2 = nullptr) {
   for (int m = 0; m < Lq + K; ++m) {
     const int pr = m < Lq ? m : k + (m - Lq);
     const size_t o = (size_t)m * N;
     KeyDesc2 e;
-    e.src = X + o;
+    e.src = m < Lq ? (XQ ? XQ + o : nullptr) : XB + (size_t)(m - Lq) * N;
     e.dst[0] = P0 + o;
     e.dst[1] = P1 + o;
     e.key[0] = kb + (size_t)pr * N;
@@ -140,7 +144,7 @@ void batch_sizes(const aoa_batch* b, size_t* ws_bytes, size_t* desc_bytes) {
       const size_t N = g.params->host.N, K = g.params->host.K;
       for (auto& it : g.muls) {
         const size_t Lq = it.level + 1;
-        s += mul_ws_limbs((int)Lq, (int)K) * N * 8;
+        s += mul_ws_limbs((int)Lq, (int)K, N >= 8192) * N * 8;
         desc += Lq * (sizeof(TensorDesc) + 3 * sizeof(NttDesc)) + sizeof(ModUpDesc) +
                 K * sizeof(NttDesc) +
                 (Lq + K) * sizeof(KeyDesc2) + 2 * (Lq + K) * sizeof(NttDesc) + 2 * sizeof(ModDownDesc);
@@ -198,13 +202,16 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
       for (auto& it : g.muls) {
         const int lv = it.level, Lq = lv + 1;
         g.max_level = std::max(g.max_level, lv);
-        uint64_t* X = w;
-        uint64_t* D0 = X + (size_t)(Lq + K) * N;
-        uint64_t* D1 = D0 + (size_t)Lq * N;
-        uint64_t* D2 = D1 + (size_t)Lq * N;
+        const bool fused = P.N >= 8192;
+        // unfused: X (Lq + K), D0, D1, D2, P0, P1; fused: X's special limbs, D2, P0, P1
+        uint64_t* X = fused ? nullptr : w;
+        uint64_t* XB = fused ? w : X + (size_t)Lq * N;
+        uint64_t* D0 = fused ? nullptr : XB + (size_t)K * N;
+        uint64_t* D1 = fused ? nullptr : D0 + (size_t)Lq * N;
+        uint64_t* D2 = fused ? XB + (size_t)K * N : D1 + (size_t)Lq * N;
         uint64_t* P0 = D2 + (size_t)Lq * N;
         uint64_t* P1 = P0 + (size_t)(Lq + K) * N;
-        w += mul_ws_limbs(Lq, K) * N;
+        w += mul_ws_limbs(Lq, K, fused) * N;
         // From N = 8192 up the transforms are two-pass and the tensor product
         // is never stored: d2 = a1 b1 is formed in the first inverse pass of
         // d2 (for ModUp), and again with d0 and d1 in the key products' first
@@ -213,7 +220,6 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
         // ModDown(X + B d) = ModDown(X) + d, residue for residue, and d0, d1
         // need no transform.  Below, a tensor kernel, the single-workgroup
         // transforms, and the additions in ModDown.
-        const bool fused = P.N >= 8192;
         for (int i = 0; i < Lq && fused; ++i) {
           const size_t o = (size_t)i * N;
           trow.push_back({it.c1[1] + o, D2 + o, it.c2[1] + o, i, 0});
@@ -235,10 +241,9 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
           intt3.push_back({D1 + o, D1 + o, nullptr, i, 0});
           intt3.push_back({X + o, D2 + o, nullptr, i, 0});
         }
-        mu.push_back({D2, X + (size_t)Lq * N, lv, 0});
-        for (int j = 0; j < K; ++j)
-          fwd.push_back({X + (size_t)(Lq + j) * N, X + (size_t)(Lq + j) * N, nullptr, k + j, 0});
-        add_key_descs(key2, keycol, X, P0, P1, g.evk_b, g.evk_a, nullptr, nullptr, Lq, K, k, N,
+        mu.push_back({D2, XB, lv, 0});
+        for (int j = 0; j < K; ++j) fwd.push_back({XB + (size_t)j * N, XB + (size_t)j * N, nullptr, k + j, 0});
+        add_key_descs(key2, keycol, X, XB, P0, P1, g.evk_b, g.evk_a, nullptr, nullptr, Lq, K, k, N,
                       fused ? it.c1 : nullptr, fused ? it.c2 : nullptr);
         for (int y = 0; y < 2; ++y)
           md.push_back({y ? P1 : P0, (y ? P1 : P0) + (size_t)Lq * N, fused ? nullptr : (y ? D1 : D0), it.out[y],
@@ -263,7 +268,7 @@ size_t batch_fill(aoa_batch* b, uint64_t* ws, char* host, char* dev, size_t desc
         mu.push_back({R1, X + (size_t)Lq * N, lv, 0});
         for (int j = 0; j < K; ++j)
           fwd.push_back({X + (size_t)(Lq + j) * N, X + (size_t)(Lq + j) * N, nullptr, k + j, 0});
-        add_key_descs(key2, keycol, X, P0, P1, it.rtk_b, it.rtk_a, nullptr, nullptr, Lq, K, k, N);
+        add_key_descs(key2, keycol, X, X + (size_t)Lq * N, P0, P1, it.rtk_b, it.rtk_a, nullptr, nullptr, Lq, K, k, N);
         for (int y = 0; y < 2; ++y) {
           const uint64_t* add = y == 0 ? R0 : (it.accumulate ? it.in[1] : nullptr);
           md.push_back({y ? P1 : P0, (y ? P1 : P0) + (size_t)Lq * N, add, it.out[y], lv, 0});
