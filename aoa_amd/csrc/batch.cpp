@@ -420,7 +420,10 @@ int batch_enqueue(aoa_batch* b, hipStream_t st, StageTimer* tm) {
                                    -1, false);
     });
     run(AOA_STAGE_MODDOWN, [](BatchGroup& g) { return g.n_moddown > 0; }, [&](BatchGroup& g) {
-      launch_moddown_desc(launch_params(b, g), g.d_moddown, g.n_moddown, st);
+      // the key stage above (canon = false) hands over special limbs
+      // already multiplied by (B/b_j)^-1 in its two-pass form
+      const DevParams P = launch_params(b, g);
+      launch_moddown_desc(P, g.d_moddown, g.n_moddown, ntt_key2_bscaled(P, 0), st);
       return 0;
     });
   }

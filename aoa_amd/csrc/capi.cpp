@@ -79,6 +79,7 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
       {H.md_fin.data(), H.md_fin.size() * 8, 0},
       {H.md_negb.data(), H.md_negb.size() * 4, 0},
       {H.mu_terms.data(), H.mu_terms.size() * 4, 0},
+      {H.bscale.data(), H.bscale.size() * 8, 0},
   };
   // Row-pass twiddle orders, one per two-pass split of N and direction
   // (device only: the host keeps the natural tables).
@@ -133,6 +134,7 @@ extern "C" int aoa_params_create(int N, int k, int kl, int prec, aoa_params** ou
   D.md_fin = (const uint64_t*)(b + secs[16].off);
   D.md_negb = (const uint32_t*)(b + secs[17].off);
   D.mu_terms = (const uint32_t*)(b + secs[18].off);
+  D.bscale = (const uint64_t*)(b + secs[19].off);
   for (int si = 0; si < 3; ++si)
     for (int dir = 0; dir < 2; ++dir)
       D.tw_row[si][dir] = rowsec[si][dir] < 0 ? nullptr : (const uint64_t*)(b + secs[rowsec[si][dir]].off);
@@ -192,6 +194,7 @@ extern "C" int64_t aoa_params_table(const aoa_params* h, int which, uint64_t* ou
       for (uint32_t g : H.rot_group) v.push_back(g);
       break;
     case 13: v = H.gauss_cdt; break;
+    case 15: take(H.bscale, 2); break;
     case 14:  // ModDown constants, eight 32-bit words per (i, j) widened to 64 bits
       for (const MdConst& m : H.md_const)
         for (uint32_t x : {m.cl, m.ch, m.cs, m.w0, m.w1, m.w2, m.thr, m.pad}) v.push_back(x);

@@ -106,11 +106,19 @@ int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
                      bool canon = true);
 // Key-switching inverse NTT: first pass from KeyDesc2 entries (one input limb,
 // two keys, two outputs), second pass over the NttDesc list of outputs.
+// canon = false as above, and in the two-pass form the special limbs (prime
+// index k + j) come out multiplied by (B/b_j)^-1 mod b_j, in [0, 2 b_j), for
+// a ModDown that takes them so (ModDownDesc::bscaled).
 int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const NttDesc* dcol, int ncol, int N1,
                           hipStream_t st, int n2strict = -1, int ncolstrict = -1, bool canon = true);
+// Whether ntt_inverse_key2_desc(..., N1, ..., canon = false) scales the
+// special limbs (the two-pass form).
+bool ntt_key2_bscaled(const DevParams& P, int N1);
 void launch_tensor_desc(const DevParams& P, const TensorDesc* d, int ndesc, hipStream_t st);
 void launch_modup_desc(const DevParams& P, const ModUpDesc* d, int ndesc, int max_level, hipStream_t st);
-void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, hipStream_t st);
+// bscaled: every descriptor's special limbs come from ntt_inverse_key2_desc's
+// scaled form (already multiplied by (B/b_j)^-1 mod b_j, in [0, 2 b_j)).
+void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, bool bscaled, hipStream_t st);
 void launch_automorphism_desc(const DevParams& P, const AutoDesc* d, int ndesc, hipStream_t st);
 
 }  // namespace aoa

@@ -383,8 +383,10 @@ __device__ __forceinline__ void inv_row_key2_body(const DevParams& P, const uint
 // Column pass of the inverse: global stages log2(S)-1 .. 0, N^-1 folded into
 // stage 0.
 // CANON = false (lazy primes only) leaves the outputs in [0, 2p) for a
-// consumer that takes them so (ModUp, ModDown).
-template <int S, bool LZ, bool CANON = true>
+// consumer that takes them so (ModUp, ModDown).  BS (with CANON = false):
+// the special limbs (pr >= k) also take ModDown's factor (B/b_j)^-1 mod b_j
+// in this stage (bscale), so that ModDown does not multiply them again.
+template <int S, bool LZ, bool CANON = true, bool BS = false>
 __device__ __forceinline__ void inv_col_body(const DevParams& P, const uint64_t* src, uint64_t* dst, int pr,
                                              int tile, int N2, uint64_t* lds) {
   using G = Geo<S>;
@@ -428,12 +430,20 @@ __device__ __forceinline__ void inv_col_body(const DevParams& P, const uint64_t*
     }
   }
   // stage 0 with N^-1: a' = (u+v) N^-1, b' = (u-v) psi^-brev(1) N^-1
+  uint64_t n0 = dp.ninv, n0q = dp.ninv_q, n1 = dp.w1ninv, n1q = dp.w1ninv_q;
+  if (BS && pr >= P.k) {
+    const cptr<uint64_t> bs = as_const(P.bscale) + 4 * (pr - P.k);
+    n0 = bs[0];
+    n0q = bs[1];
+    n1 = bs[2];
+    n1q = bs[3];
+  }
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     uint64_t u = x[k], v = x[k + 8];
     if (LZ && !CANON) {
-      x[k] = mul_shoup_lazy(u + v, dp.ninv, dp.ninv_q, p);
-      x[k + 8] = mul_shoup_lazy(u + 2 * p - v, dp.w1ninv, dp.w1ninv_q, p);
+      x[k] = mul_shoup_lazy(u + v, n0, n0q, p);
+      x[k + 8] = mul_shoup_lazy(u + 2 * p - v, n1, n1q, p);
     } else {
       x[k] = mul_shoup(u + v, dp.ninv, dp.ninv_q, p);
       x[k + 8] = mul_shoup(LZ ? u + 2 * p - v : u + (p - v), dp.w1ninv, dp.w1ninv_q, p);
@@ -587,13 +597,13 @@ __global__ __launch_bounds__(256) void k_ntt_inv_row_key2_d(DevParams P, const K
     inv_row_key2_body<S, false>(P, src, d0, d1, k0, k1, a0, a1, t0, t1, t2, t3, pr, b % tiles, g0, lds, y);
 }
 
-template <int S, int MODE, bool CANON>
+template <int S, int MODE, bool CANON, bool BS = false>
 __device__ __forceinline__ void inv_col_d(const DevParams& P, const NttDesc* d, int N2, uint64_t* lds) {
   const int wid = work_id(P);
   const int tiles = N2 / Geo<S>::C;
   const NttDesc e = d[wid / tiles];
   if (use_lazy<MODE>(P, e.prime))
-    inv_col_body<S, true, CANON>(P, glob(e.dst), glob(e.dst), e.prime, wid % tiles, N2, lds);
+    inv_col_body<S, true, CANON, BS>(P, glob(e.dst), glob(e.dst), e.prime, wid % tiles, N2, lds);
   else if (MODE != 1)
     inv_col_body<S, false>(P, glob(e.dst), glob(e.dst), e.prime, wid % tiles, N2, lds);
 }
@@ -609,6 +619,13 @@ template <int S, int MODE>
 __global__ __launch_bounds__(256) void k_ntt_inv_col_lazy_d(DevParams P, const NttDesc* d, int N2) {
   __shared__ uint64_t lds[Geo<S>::COL_LDS];
   inv_col_d<S, MODE, false>(P, d, N2, lds);
+}
+
+// Same, the special limbs also multiplied by (B/b_j)^-1 mod b_j for ModDown.
+template <int S, int MODE>
+__global__ __launch_bounds__(256) void k_ntt_inv_col_bscale_d(DevParams P, const NttDesc* d, int N2) {
+  __shared__ uint64_t lds[Geo<S>::COL_LDS];
+  inv_col_d<S, MODE, false, true>(P, d, N2, lds);
 }
 
 // ------------------------------------------- single-workgroup transform --
@@ -997,6 +1014,11 @@ int ntt_inverse_desc(const DevParams& P, const NttDesc* d, int ndesc, int N1, hi
   return 0;
 }
 
+bool ntt_key2_bscaled(const DevParams& P, int N1) {
+  N1 = pick_n1(P, N1);
+  return N1 != 0 && N1 != P.N;
+}
+
 int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const NttDesc* dcol, int ncol, int N1,
                           hipStream_t st, int n2strict, int ncolstrict, bool canon) {
   N1 = pick_n1(P, N1);
@@ -1027,7 +1049,7 @@ int ntt_inverse_key2_desc(const DevParams& P, const KeyDesc2* d2, int n2, const 
     const int tiles = N2 / (4096 / N1);
 #define CALLC(S)                                                                    \
   if (canon) AOA_SPLIT_LAUNCH(k_ntt_inv_col_d, S, tiles, dcol, ncol, ncolstrict, N2); \
-  else AOA_SPLIT_LAUNCH(k_ntt_inv_col_lazy_d, S, tiles, dcol, ncol, ncolstrict, N2)
+  else AOA_SPLIT_LAUNCH(k_ntt_inv_col_bscale_d, S, tiles, dcol, ncol, ncolstrict, N2)
     AOA_DISPATCH_S(N1, CALLC)
 #undef CALLC
   }

@@ -291,6 +291,16 @@ std::string build_host_params(int N, int k, int kl, int prec, HostParams* out) {
     P.inv_bhat[2 * j] = inv;
     P.inv_bhat[2 * j + 1] = shoup_quot(inv, bj);
   }
+  P.bscale.assign((size_t)kl * 4, 0);
+  for (int j = 0; j < kl; ++j) {
+    const uint64_t bj = Bp[j], inv = P.inv_bhat[2 * j];
+    const DevPrime& d = P.dprimes[k + j];
+    const uint64_t a = mulm(d.ninv, inv, bj), b = mulm(d.w1ninv, inv, bj);
+    P.bscale[4 * j] = a;
+    P.bscale[4 * j + 1] = shoup_quot(a, bj);
+    P.bscale[4 * j + 2] = b;
+    P.bscale[4 * j + 3] = shoup_quot(b, bj);
+  }
   P.bhat_mod_q.assign((size_t)k * kl * 2, 0);
   P.md_const.assign((size_t)k * kl, MdConst{});
   P.md_terms.assign((size_t)kl * k, MdTerm{});

@@ -73,6 +73,10 @@ struct DevParams {
   const uint64_t* inv_qhat;      // [k][k][2]: (l,i) -> (Q_l/q_i)^-1 mod q_i, Shoup
   const uint64_t* qhat_mod_b;    // [k][K][k]: (l,j,i) -> (Q_l/q_i) mod b_j
   const uint64_t* inv_bhat;      // [K][2]: (B/b_j)^-1 mod b_j, Shoup
+  // [K][4]: N^-1 (B/b_j)^-1 and psi^-brev(1) N^-1 (B/b_j)^-1 mod b_j, each
+  // with its Shoup quotient: the last inverse stage of a special limb whose
+  // only consumer is ModDown, which then takes the limb as it comes
+  const uint64_t* bscale;
   const uint64_t* bhat_mod_q;    // [k][K][2]: (i,j) -> (B/b_j) mod q_i, Shoup
   const uint64_t* inv_b_mod_q;   // [k][2]: B^-1 mod q_i, Shoup
   const uint64_t* b_mod_q;       // [k][2]: B mod q_i, Shoup
@@ -119,6 +123,7 @@ struct HostParams {
   std::vector<DevPrime> dprimes;
   std::vector<uint64_t> tw_fwd, tw_inv, inv_qhat, qhat_mod_b, inv_bhat, bhat_mod_q,
       inv_b_mod_q, b_mod_q, inv_ql_mod_qi;
+  std::vector<uint64_t> bscale;     // K * 4
   std::vector<double> enc_w;        // 2N complex roots, interleaved (re, im)
   std::vector<uint32_t> rot_group;  // N/2 entries
   std::vector<uint64_t> gauss_cdt;  // 2 kGaussHalf entries

@@ -503,9 +503,10 @@ struct NoFold {
   __device__ bool operator()(int, uint64_t*) const { return false; }
 };
 
+// bscaled: inB's limbs already carry (B/b_j)^-1 and lie in [0, 2 b_j).
 __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t* inQ, const uint64_t* inB,
                                              const uint64_t* add, uint64_t* out, int level, int rescale,
-                                             int blk, uint64_t* aux) {
+                                             int bscaled, int blk, uint64_t* aux) {
   const size_t N = (size_t)P.N;
   const int c = threadIdx.x % BC_COEF;
   // wave-uniform limb group: keeps limb indices and constant loads scalar
@@ -513,10 +514,18 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
   const size_t cid = (size_t)blk * BC_COEF + c;
   const int K = P.K, k = P.k;
   const tw2* ib = reinterpret_cast<const tw2*>(P.inv_bhat);
-  for (int j = g; j < K; j += BC_GROUPS) {
-    const uint64_t bp = P.primes[k + j].p;
-    const tw2 w = ib[j];
-    aux[j * BC_COEF + c] = mul_shoup(inB[(size_t)j * N + cid], w.w, w.q, bp);
+  if (bscaled) {
+    for (int j = g; j < K; j += BC_GROUPS) {
+      const uint64_t bp = P.primes[k + j].p;
+      const uint64_t x = inB[(size_t)j * N + cid];
+      aux[j * BC_COEF + c] = x >= bp ? x - bp : x;
+    }
+  } else {
+    for (int j = g; j < K; j += BC_GROUPS) {
+      const uint64_t bp = P.primes[k + j].p;
+      const tw2 w = ib[j];
+      aux[j * BC_COEF + c] = mul_shoup(inB[(size_t)j * N + cid], w.w, w.q, bp);
+    }
   }
   __syncthreads();
   const tw2* ibq = reinterpret_cast<const tw2*>(P.inv_b_mod_q);
@@ -661,7 +670,7 @@ __device__ __forceinline__ void moddown_body(const DevParams& P, const uint64_t*
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_moddown(DevParams P, ModDownArgs A) {
   extern __shared__ uint64_t aux[];  // [K][64], then [4][64] partial sums of the top limb and of q0
   const int y = blockIdx.y;
-  moddown_body(P, A.inQ[y], A.inB[y], A.add[y], A.out[y], A.level, A.rescale, blockIdx.x, aux);
+  moddown_body(P, A.inQ[y], A.inB[y], A.add[y], A.out[y], A.level, A.rescale, 0, blockIdx.x, aux);
 }
 
 void launch_moddown(const DevParams& P, const ModDownArgs& a, hipStream_t st) {
@@ -672,18 +681,19 @@ void launch_moddown(const DevParams& P, const ModDownArgs& a, hipStream_t st) {
 // Eight waves per SIMD (58 registers): the batched grid is many times the
 // machine, and more waves hide the product loop's table loads; 4 % faster
 // than seven waves (70 registers) and than five (81, the default).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_moddown_d(DevParams P, const ModDownDesc* dd) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_moddown_d(DevParams P, const ModDownDesc* dd,
+                                                                                           int bscaled) {
   extern __shared__ uint64_t aux[];
   const int per = P.N / BC_COEF;
   const ModDownDesc m = dd[blockIdx.x / per];
   // generic (flat) accesses here: the global form ran 1 % slower (more
   // scalar registers held for the bases, which this kernel already spills)
-  moddown_body(P, m.inQ, m.inB, m.add, m.out, m.level, m.rescale, blockIdx.x % per, aux);
+  moddown_body(P, m.inQ, m.inB, m.add, m.out, m.level, m.rescale, bscaled, blockIdx.x % per, aux);
 }
 
-void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, hipStream_t st) {
+void launch_moddown_desc(const DevParams& P, const ModDownDesc* d, int ndesc, bool bscaled, hipStream_t st) {
   const size_t lds = (size_t)(P.K + 2 * BC_GROUPS) * BC_COEF * sizeof(uint64_t);
-  k_moddown_d<<<dim3((unsigned)(ndesc * (P.N / BC_COEF))), 256, lds, st>>>(P, d);
+  k_moddown_d<<<dim3((unsigned)(ndesc * (P.N / BC_COEF))), 256, lds, st>>>(P, d, bscaled ? 1 : 0);
 }
 
 // -------------------------------------------------------------- checksum ----

@@ -69,7 +69,9 @@ int aoa_params_primes(const aoa_params* params, uint64_t* out, int n);
  * 14 ModDown product constants [k][kl][8] for c = (B/b_j) mod q_i (c mod 2^28,
  * c >> 28, their sum, the 32-bit words w0 w1 w2 of floor(c 2^96 / q_i), q_i's
  * screen threshold, 2 (the high word of the screen's bias 2^33); all zero for
- * primes of 57 bits or more).
+ * primes of 57 bits or more), 15 [kl][2]: N^-1 (B/b_j)^-1 and psi^-brev(1) N^-1
+ * (B/b_j)^-1 mod b_j (the last inverse stage of the special limbs of a batch's
+ * key products).
  * Returns the number of words written (or needed, if out is NULL). */
 int64_t aoa_params_table(const aoa_params* params, int which, uint64_t* out, int64_t n);
 

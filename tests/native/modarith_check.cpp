@@ -556,6 +556,7 @@ int64_t chk_table(int N, int k, int kl, int prec, int which, uint64_t* out) {
       for (auto& d : H.dprimes) v.push_back(d.mu);
       break;
     case 10: v = H.primes; break;
+    case 15: take(H.bscale, 2); break;
     case 11:  // encoder roots exp(2 pi i t / 2N) as (re, im) bit patterns
       v.resize(H.enc_w.size());
       std::memcpy(v.data(), H.enc_w.data(), v.size() * 8);

@@ -318,6 +318,28 @@ def test_moddown_wide_sums():
         assert chk.chk_moddown_wide(*shape, 3000, 29) == 0, shape
 
 
+def test_special_limb_inverse_scale():
+    """The last inverse stage of a batch's key products multiplies special
+    limb j by N^-1 (B/b_j)^-1 (a'), and by psi^-brev(1) N^-1 (B/b_j)^-1 (b'),
+    so that ModDown (manager.cu:312-341) takes [x_bj (B/b_j)^-1]_bj as the
+    limb comes.  Pinned by multiplying the table back to 1 with big integers."""
+    for N, k, kl, prec in ((65536, 30, 31, 48), (4096, 4, 5, 52), (8192, 8, 9, 55)):
+        primes = [int(v) for v in native_chk.table(N, k, kl, prec, 10)]
+        kl = len(primes) - k
+        bs = native_chk.table(N, k, kl, prec, 15).reshape(kl, 2)
+        tw_inv = native_chk.table(N, k, kl, prec, 1).reshape(k + kl, N)
+        B = math.prod(primes[k:])
+        for j in range(kl):
+            b = primes[k + j]
+            s0, s1 = int(bs[j, 0]), int(bs[j, 1])
+            assert s0 < b and s1 < b
+            assert s0 * N * (B // b) % b == 1
+            # psi^-brev(1) is the inverse of psi^brev(1) = psi^(N/2), a square root of -1
+            w = int(tw_inv[k + j, 1])
+            assert w * w % b == b - 1
+            assert s1 == w * s0 % b
+
+
 def test_row_twiddle_order():
     """The row-ordered twiddle tables of every two-pass split (N = 2^12 ..
     2^16): each entry a row-pass thread reads is the natural table's entry for
