@@ -44,7 +44,7 @@ MODDOWN_TRAFFIC_BYTES = {32: 3025902773}
 # cycle.  64-bit integer multiply-adds issue at about this rate; 32-bit adds
 # and logic issue faster (about 1.6 times), so the fraction is a proxy that a
 # mix of both can exceed.
-MODDOWN_VALU_INSTS = {32: 9.583e8}
+MODDOWN_VALU_INSTS = {32: 9.186e8}
 VALU_PEAK_INSTS_S = 256 * 2.4e9
 
 
